@@ -1,0 +1,167 @@
+"""Benchmark: batched planar TE+TM wavelength sweep on one MI355X.
+
+Workload (one step = one pass of the whole hot path over one batch, inputs resident in HBM):
+  grating (period 0.574, thickness 0.38, fill 0.3, n 1.46 / 1.0) on a 20-layer quarter-wave mirror
+  alternating n = 2.1 / 1.46 (each 1.053 / (4n) thick), substrate n = 1.46, incidence n = 1.0,
+  theta = 66.36 deg, M = 60 (N = 121), 4096 wavelengths uniform in [0.95, 1.15], TE and TM:
+  8192 complex128 solves per step.
+
+Prints one JSON line: solves/s, the dominant kernel's FP64 roofline fraction (HIP-event timed),
+the reference CPU path timed on this host's cores on a bounded sample, and the plain
+one-solve-per-launch GPU form for comparison.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6          # MI355X dense FP64 (matrix and vector)
+HBM_PEAK_GBS = 8000.0
+METRIC = "solves/sec at max|ΔDE|≤1e-9 vs FP64 CPU oracle; p50/p99 request latency"
+
+
+def mirror_stack():
+    from stablercwa_amd.types import Layer, Stack
+    layers = [Layer.grating(0.38, [0.0, 0.3 * 0.574, 0.574], [1.46, 1.0])]
+    for k in range(20):
+        n = 2.1 if k % 2 == 0 else 1.46
+        layers.append(Layer.uniform(1.053 / (4 * n), n))
+    return Stack(1.0, layers, 1.46, 0.574)
+
+
+def cpu_baseline(stack, M, n_points):
+    """The reference's own solver (oracle/_ref) on this host's cores, or the numpy oracle when absent."""
+    lams = np.linspace(0.95, 1.15, n_points)
+    thetas = np.full(n_points, 66.36)
+    from oracle import ref
+    if ref.available():
+        lib = ref.load()
+        cores = max(1, min(16, os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        for pol in (0, 1):
+            ref.sweep1d(lib, stack.flat(), lams, thetas, M, pol, cores)
+        dt = time.perf_counter() - t0
+        return dict(value=2 * n_points / dt, unit="solves/s", cores=cores, kind="reference",
+                    sample=f"{n_points} wavelengths x TE+TM at M={M} (reference rcwa::sweep, {cores} threads)")
+    from oracle import rcwa1d
+    k = max(4, n_points // 16)
+    t0 = time.perf_counter()
+    for pol in (0, 1):
+        for l in lams[:k]:
+            rcwa1d.solve(stack.flat(), float(l), 66.36, M, pol)
+    dt = time.perf_counter() - t0
+    return dict(value=2 * k / dt, unit="solves/s", cores=1, kind="port",
+                sample=f"{k} wavelengths x TE+TM at M={M} (numpy oracle, 1 thread)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--points", type=int, default=4096)
+    ap.add_argument("--M", type=int, default=60)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-points", type=int, default=128)
+    ap.add_argument("--plain-sample", type=int, default=32)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not provided"}))
+        sys.exit(2)
+
+    from stablercwa_amd import rcwa
+    st = mirror_stack()
+    M = args.M
+    N = 2 * M + 1
+    lams = np.linspace(0.95, 1.15, args.points)
+    probs = [(0, float(l), 66.36, M, pol) for pol in (0, 1) for l in lams]
+    batch = rcwa.PlanarBatch([st], probs)
+    solves = len(probs)
+    print(f"# plan: {solves} solves, {batch.device_bytes() / 2**30:.2f} GiB device, {batch.chunks()} chunk(s)",
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        batch.execute()
+    batch.sync()
+    stage_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.execute()
+        batch.sync()
+        stage_ms.append(batch.stage_ms())
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = solves * args.steps / (t1 - t0)
+    st_ms = np.mean(np.array(stage_ms), axis=0)
+    print(f"# {ms_per_step:.2f} ms/step, stages (assembly, eig, S-matrix, total) = {np.round(st_ms, 3).tolist()} ms",
+          file=sys.stderr, flush=True)
+
+    # correctness of the measured run: energy on every solve, oracle on a sample
+    res = batch.results()
+    assert np.all(res["status"] == 0), "a benchmark solve failed"
+    der = res["DEr"].reshape(-1, N)
+    det = res["DEt"].reshape(-1, N)
+    energy = float(np.max(np.abs(der.sum(1) + det.sum(1) - 1.0)))
+    from oracle import rcwa1d
+    dmax = 0.0
+    for idx in np.linspace(0, solves - 1, 6).astype(int):
+        o = rcwa1d.solve(st.flat(), probs[idx][1], 66.36, M, probs[idx][4])
+        dmax = max(dmax, float(np.max(np.abs(der[idx] - o["DEr"]))), float(np.max(np.abs(det[idx] - o["DEt"]))))
+
+    # roofline of the dominant kernel
+    fl = batch.flops()
+    eig_ms, smat_ms = float(st_ms[1]), float(st_ms[2])
+    if eig_ms >= smat_ms:
+        name, flops, ms = "k_eig1d", fl[0], eig_ms
+    else:
+        name, flops, ms = "k_smatrix1d (+k_modes1d)", fl[1], smat_ms
+    achieved = flops / (ms * 1e-3) / 1e12
+    roofline = dict(bound="mfma", kernel=name, achieved=achieved, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                    frac=achieved / FP64_PEAK_TFLOPS, traffic=None,
+                    flops_per_launch=float(flops), launch_ms=ms)
+
+    # plain form: one solve per launch, inputs resident
+    plain = None
+    if args.plain_sample > 0:
+        singles = [rcwa.PlanarBatch([st], [probs[i]]) for i in range(args.plain_sample)]
+        singles[0].execute(); singles[0].sync()
+        tp = time.perf_counter()
+        for b in singles:
+            b.execute()
+            b.sync()
+        tq = time.perf_counter()
+        plain = dict(value=args.plain_sample / (tq - tp), unit="solves/s", sample=f"{args.plain_sample} solves, one per launch")
+        for b in singles:
+            b.close()
+
+    cpu = None if args.no_cpu else cpu_baseline(st, M, args.cpu_points)
+    line = {
+        "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "batched 1-D TE+TM wavelength sweep: binary grating (period 0.574, d 0.38, fill 0.3, "
+                               "n 1.46/1.0) on a 20-layer quarter-wave mirror (2.1/1.46), substrate 1.46, "
+                               f"{args.points} wavelengths in [0.95,1.15], theta 66.36 deg, M={M} (N={N}), complex128",
+                   "solves_per_step": solves, "M": M, "N": N, "parallelism": "single GPU, batched"},
+        "roofline": roofline,
+        "stages_ms": {"setup_assembly": float(st_ms[0]), "eig": eig_ms, "modes_smatrix_eff": smat_ms,
+                      "total_events": float(st_ms[3])},
+        "parity": {"max_abs_dDE_vs_oracle_sample": dmax, "max_abs_R_plus_T_minus_1": energy},
+        "cpu_baseline": cpu,
+        "plain_gpu": plain,
+        "batched_over_plain": (value / plain["value"]) if plain else None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,108 @@
+/* srcwa.h -- C-ABI of the MI355X-native RCWA solver (libsrcwa_amd.so).
+ *
+ * Plain pointers and sizes only; every entry point is exception-free and returns an int code
+ * (SRCWA_OK or a negative SRCWA_E*), with a message for the calling thread in srcwa_last_error().
+ * Complex numbers are interleaved (re, im) doubles; matrices are column-major, as the reference's
+ * stablercwa::Matrix (include/stablercwa/matrix.hpp:1-5).
+ *
+ * Flat stack records (doubles):
+ *   planar / conical stack (reference rcwa::Stack, rcwa1d.hpp:80-114):
+ *     period, nI.re, nI.im, nII.re, nII.im, L, then per layer
+ *       d, kind (0 uniform | 1 grating),
+ *         uniform: n.re, n.im
+ *         grating: K, edges[K+1], (n.re, n.im)[K], fourier_method (0 analytic | 1 FFT), fft_samples
+ *   crossed-grating stack (reference rcwa::Stack2D, rcwa2d_types.hpp:39-78):
+ *     Lx, Ly, nI, nII.re, nII.im, L, then per layer
+ *       d, kind (0 uniform | 1 patterned),
+ *         uniform: n.re, n.im
+ *         patterned: Ky, y_edges[Ky+1], then per y segment Kx, x_edges[Kx+1], (n.re, n.im)[Kx]
+ *
+ * Status words per solve: 0 ok, 1 numerical failure (results NaN; the reference's NumericalFailure,
+ * rcwa1d.hpp:37-38), 3 Rayleigh anomaly (crossed gratings, rcwa2d_types.hpp:32-37).
+ */
+#ifndef SRCWA_H
+#define SRCWA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SRCWA_OK = 0,
+    SRCWA_EINVAL = -1,        /* invalid argument: the reference throws std::invalid_argument */
+    SRCWA_EDEVICE = -2,       /* HIP runtime error */
+    SRCWA_ENOMEM = -3,        /* device allocation failed */
+    SRCWA_EUNSUPPORTED = -4,  /* option not provided by this build */
+};
+
+typedef struct srcwa_context srcwa_context;
+typedef struct srcwa_plan srcwa_plan;
+
+const char* srcwa_last_error(void);
+const char* srcwa_version(void);
+
+/* One context per device: a HIP stream and the device it runs on. */
+int srcwa_context_create(int device, srcwa_context** out);
+void srcwa_context_destroy(srcwa_context* ctx);
+int srcwa_context_sync(srcwa_context* ctx);
+
+/* ---------------------------------------------------------------------------------------------
+ * Batched planar (TE/TM) solves.  Replaces rcwa::solve / rcwa::sweep (rcwa1d.hpp:1077-1091): each
+ * problem is (stack, wavelength, theta_deg, M, pol) with pol 0 = TE, 1 = TM, and yields N = 2M+1
+ * orders of R, T (complex), DEr, DEt (real), a status word and eig_info (Result, rcwa1d.hpp:145-164).
+ * Results of problem i start at order offset sum_{k<i} (2 M_k + 1).
+ *
+ * stacks: n_stacks flat records concatenated; stack_offsets[s] = start (in doubles) of record s.
+ * max_workspace_bytes bounds the device workspace of one launch chunk (0 = 32 GiB); problems are
+ * admitted into chunks in order until the bound is reached.
+ */
+int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                        int n_problems, const int* stack_index, const double* wavelength, const double* theta_deg,
+                        const int* M, const int* pol, int64_t max_workspace_bytes, srcwa_plan** out);
+
+/* Launch all chunks of the plan on the context stream (asynchronous). */
+int srcwa_plan_execute(srcwa_plan* plan);
+/* Wait for the last execute. */
+int srcwa_plan_sync(srcwa_plan* plan);
+/* Copy results to host buffers (implies sync).  Any pointer may be NULL. */
+int srcwa_plan_results(srcwa_plan* plan, double* R, double* T, double* DEr, double* DEt, int* status, int* eig_info);
+/* Total result orders (sum of N over problems), device bytes held, number of launch chunks. */
+int64_t srcwa_plan_orders(const srcwa_plan* plan);
+int64_t srcwa_plan_device_bytes(const srcwa_plan* plan);
+int srcwa_plan_chunks(const srcwa_plan* plan);
+/* Per-stage GPU milliseconds of the last execute, from HIP events on the plan's stream:
+ * [0] setup + assembly, [1] eigensolves, [2] modes + S-matrix + efficiencies, [3] total. */
+int srcwa_plan_stage_ms(const srcwa_plan* plan, double* ms4);
+/* Algorithmic FP64 flops of one execute of the plan, by the accounting convention
+ * (complex GEMM m k n: 8mkn; LU n: 8n^3/3; triangular solves with k RHS: 8 n^2 k; eig with vectors: 100 n^3):
+ * [0] eigensolves, [1] everything else. */
+int srcwa_plan_flops(const srcwa_plan* plan, double* flops2);
+void srcwa_plan_destroy(srcwa_plan* plan);
+
+/* One-shot convenience: create, execute, fetch, destroy. */
+int srcwa_solve1d_batch(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                        int n_problems, const int* stack_index, const double* wavelength, const double* theta_deg,
+                        const int* M, const int* pol, double* R, double* T, double* DEr, double* DEt, int* status,
+                        int* eig_info);
+
+/* ---------------------------------------------------------------------------------------------
+ * Batched dense kernels in the shapes of the reference's hooks (Options::custom_eig / custom_solve /
+ * custom_multiply, rcwa1d.hpp:122-136; cuda_backend.hpp:387-512 is the reference's NVIDIA wiring).
+ * Host buffers, items packed back to back, one workgroup per item.
+ *   zgeev : A_i n_i x n_i -> w_i (n_i), V_i (n_i x n_i, unit 2-norm columns), info_i (EigResult::info)
+ *   zgesv : A_i n_i x n_i, B_i n_i x nrhs_i -> X_i; info_i = 1 on an exact zero pivot
+ *   zgemm : C_i (m_i x n_i) = A_i (m_i x k_i) B_i (k_i x n_i)
+ */
+int srcwa_zgeev_batched(srcwa_context* ctx, int count, const int* n, const double* A, double* w, double* V, int* info);
+int srcwa_zgesv_batched(srcwa_context* ctx, int count, const int* n, const int* nrhs, const double* A, const double* B,
+                        double* X, int* info);
+int srcwa_zgemm_batched(srcwa_context* ctx, int count, const int* m, const int* n, const int* k, const double* A,
+                        const double* B, double* C);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SRCWA_H */

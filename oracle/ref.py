@@ -1,0 +1,66 @@
+"""ctypes loader of oracle/_ref/libsrcwa_ref.so: the reference's own solver built from its headers by
+oracle/Makefile.  Test infrastructure only; absent wherever the reference tree was absent at build time."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_ref", "libsrcwa_ref.so")
+
+_lib = None
+
+
+def available() -> bool:
+    return os.path.exists(LIB)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        _lib = ctypes.CDLL(LIB)
+        _lib.ref_last_error.restype = ctypes.c_char_p
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def solve1d(lib, stack_flat, wavelength, theta, M, pol, fast_path=True):
+    N = 2 * M + 1
+    f = np.ascontiguousarray(stack_flat, dtype=np.float64)
+    R = np.zeros(2 * N); T = np.zeros(2 * N); DEr = np.zeros(N); DEt = np.zeros(N)
+    info = ctypes.c_int(); res = ctypes.c_double()
+    rc = lib.ref_solve1d(_p(f), ctypes.c_double(wavelength), ctypes.c_double(theta), M, pol, int(fast_path),
+                         _p(R), _p(T), _p(DEr), _p(DEt), ctypes.byref(info), ctypes.byref(res))
+    if rc != 0:
+        raise ValueError(lib.ref_last_error().decode())
+    return dict(R=R.view(np.complex128), T=T.view(np.complex128), DEr=DEr, DEt=DEt, eig_info=info.value,
+                max_eig_residual=res.value)
+
+
+def sweep1d(lib, stack_flat, wavelengths, thetas, M, pol, threads):
+    lib.ref_set_threads(int(threads))
+    N = 2 * M + 1
+    f = np.ascontiguousarray(stack_flat, dtype=np.float64)
+    w = np.ascontiguousarray(wavelengths, dtype=np.float64)
+    t = np.ascontiguousarray(thetas, dtype=np.float64)
+    DEr = np.zeros(len(w) * N); DEt = np.zeros(len(w) * N)
+    rc = lib.ref_sweep1d(_p(f), len(w), _p(w), _p(t), M, pol, _p(DEr), _p(DEt))
+    if rc != 0:
+        raise ValueError(lib.ref_last_error().decode())
+    return DEr.reshape(len(w), N), DEt.reshape(len(w), N)
+
+
+def eig(lib, A):
+    n = A.shape[0]
+    a = np.asfortranarray(A, dtype=np.complex128)
+    flat = a.ravel(order="F").view(np.float64).copy()
+    w = np.zeros(2 * n); V = np.zeros(2 * n * n); info = ctypes.c_int(); res = ctypes.c_double()
+    rc = lib.ref_eig(n, _p(flat), _p(w), _p(V), ctypes.byref(info), ctypes.byref(res))
+    if rc != 0:
+        raise ValueError(lib.ref_last_error().decode())
+    return w.view(np.complex128), V.view(np.complex128).reshape((n, n), order="F"), info.value, res.value

@@ -1,0 +1,230 @@
+// oracle/ref_capi.cpp -- test infrastructure, never part of the product.
+//
+// A thin extern "C" wrapper around the reference's header-only C++17 solver, compiled from the headers
+// where they lie (include/stablercwa/ of the reference tree) into oracle/_ref/libsrcwa_ref.so by
+// oracle/Makefile.  Only tests/, the golden-vector scripts under tests/golden/ and bench.py's CPU
+// baseline leg load it; it is the checker, never the thing measured as the GPU path.
+//
+// Stacks cross this boundary in the flat double encoding shared with the product C-ABI
+// (include/srcwa.h, "flat stack records"):
+//   1-D : period, nI.re, nI.im, nII.re, nII.im, L, then per layer
+//         d, kind(0 uniform | 1 grating),
+//           uniform: n.re, n.im
+//           grating: K, edges[K+1], (n.re, n.im)[K], fourier_method(0 analytic | 1 FFT), fft_samples
+//   2-D : Lx, Ly, nI, nII.re, nII.im, L, then per layer
+//         d, kind(0 uniform | 1 patterned),
+//           uniform: n.re, n.im
+//           patterned: Ky, y_edges[Ky+1], then per y segment: Kx, x_edges[Kx+1], (n.re, n.im)[Kx]
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "stablercwa/rcwa1d.hpp"
+#include "stablercwa/rcwa1d_conical.hpp"
+#include "stablercwa/rcwa2d.hpp"
+#include "stablercwa/eig.hpp"
+
+namespace rcwa = stablercwa::rcwa;
+using stablercwa::cplx;
+
+namespace {
+
+thread_local std::string g_error;
+
+rcwa::Stack decode1d(const double* p) {
+    rcwa::Stack s;
+    s.period = p[0];
+    s.nI = cplx{p[1], p[2]};
+    s.nII = cplx{p[3], p[4]};
+    const int L = static_cast<int>(p[5]);
+    std::size_t k = 6;
+    for (int j = 0; j < L; ++j) {
+        const double d = p[k++];
+        const int kind = static_cast<int>(p[k++]);
+        if (kind == 0) {
+            const cplx n{p[k], p[k + 1]};
+            k += 2;
+            s.layers.push_back(rcwa::Layer::uniform(d, n));
+        } else {
+            const int K = static_cast<int>(p[k++]);
+            std::vector<double> e(p + k, p + k + K + 1);
+            k += K + 1;
+            std::vector<cplx> n;
+            for (int i = 0; i < K; ++i) n.emplace_back(p[k + 2 * i], p[k + 2 * i + 1]);
+            k += 2 * K;
+            const int method = static_cast<int>(p[k++]);
+            const int samples = static_cast<int>(p[k++]);
+            s.layers.push_back(rcwa::Layer::grating(d, e, n,
+                method ? rcwa::FourierMethod::FFT : rcwa::FourierMethod::Analytic, samples));
+        }
+    }
+    return s;
+}
+
+rcwa::Stack2D decode2d(const double* p) {
+    rcwa::Stack2D s;
+    s.Lx = p[0];
+    s.Ly = p[1];
+    s.nI = p[2];
+    s.nII = cplx{p[3], p[4]};
+    const int L = static_cast<int>(p[5]);
+    std::size_t k = 6;
+    for (int j = 0; j < L; ++j) {
+        const double d = p[k++];
+        const int kind = static_cast<int>(p[k++]);
+        if (kind == 0) {
+            s.layers.push_back(rcwa::Layer2D::uniform(d, cplx{p[k], p[k + 1]}));
+            k += 2;
+        } else {
+            const int Ky = static_cast<int>(p[k++]);
+            std::vector<double> ye(p + k, p + k + Ky + 1);
+            k += Ky + 1;
+            std::vector<rcwa::XProfile2D> rows;
+            for (int y = 0; y < Ky; ++y) {
+                rcwa::XProfile2D prof;
+                const int Kx = static_cast<int>(p[k++]);
+                prof.edges.assign(p + k, p + k + Kx + 1);
+                k += Kx + 1;
+                for (int i = 0; i < Kx; ++i) prof.n_values.emplace_back(p[k + 2 * i], p[k + 2 * i + 1]);
+                k += 2 * Kx;
+                rows.push_back(std::move(prof));
+            }
+            s.layers.push_back(rcwa::Layer2D::patterned(d, ye, rows));
+        }
+    }
+    return s;
+}
+
+void put(double* out, const std::vector<cplx>& v) {
+    for (std::size_t i = 0; i < v.size(); ++i) {
+        out[2 * i] = v[i].real();
+        out[2 * i + 1] = v[i].imag();
+    }
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::invalid_argument& e) {
+        g_error = e.what();
+        return -1;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -2;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ref_last_error() { return g_error.c_str(); }
+
+void ref_set_threads(int n) { stablercwa::set_num_threads(n); }
+
+// One planar solve. Outputs have N = 2M+1 entries (R, T interleaved complex).
+int ref_solve1d(const double* stack, double wavelength, double theta_deg, int M, int pol, int fast_path,
+                double* R, double* T, double* DEr, double* DEt, int* eig_info, double* max_eig_residual) {
+    return guarded([&] {
+        rcwa::Options opt;
+        opt.homogeneous_fast_path = fast_path != 0;
+        opt.diagnostics = max_eig_residual != nullptr;
+        const rcwa::Result r = rcwa::solve(decode1d(stack), wavelength, theta_deg, M,
+                                           pol ? rcwa::Polarization::TM : rcwa::Polarization::TE, opt);
+        put(R, r.R);
+        put(T, r.T);
+        std::memcpy(DEr, r.DEr.data(), r.DEr.size() * sizeof(double));
+        std::memcpy(DEt, r.DEt.data(), r.DEt.size() * sizeof(double));
+        *eig_info = r.eig_info;
+        if (max_eig_residual) *max_eig_residual = r.max_eig_residual;
+    });
+}
+
+// A wavelength sweep on the reference's thread pool (set by ref_set_threads). DEr/DEt: points x N.
+int ref_sweep1d(const double* stack, int n_points, const double* wavelengths, const double* thetas, int M, int pol,
+                double* DEr, double* DEt) {
+    return guarded([&] {
+        std::vector<rcwa::Point> pts(static_cast<std::size_t>(n_points));
+        for (int i = 0; i < n_points; ++i) pts[i] = rcwa::Point{wavelengths[i], thetas[i]};
+        const auto res = rcwa::sweep(decode1d(stack), pts, M, pol ? rcwa::Polarization::TM : rcwa::Polarization::TE);
+        const int n = 2 * M + 1;
+        for (int i = 0; i < n_points; ++i) {
+            std::memcpy(DEr + static_cast<std::size_t>(i) * n, res[i].DEr.data(), n * sizeof(double));
+            std::memcpy(DEt + static_cast<std::size_t>(i) * n, res[i].DEt.data(), n * sizeof(double));
+        }
+    });
+}
+
+// One conical solve. Jr, Jt: N x 4 complex (ss, sp, ps, pp); DEr, DEt: N x 4 real.
+int ref_solve_conical(const double* stack, double wavelength, double theta_deg, double phi_deg, int M,
+                      double* Jr, double* Jt, double* DEr, double* DEt, int* eig_info, int* n_schur) {
+    return guarded([&] {
+        const rcwa::ConicalResult r = rcwa::solve_conical(decode1d(stack), wavelength, theta_deg, phi_deg, M);
+        const std::size_t n = r.Jr.size();
+        for (std::size_t j = 0; j < n; ++j) {
+            const cplx a[4] = {r.Jr[j].ss, r.Jr[j].sp, r.Jr[j].ps, r.Jr[j].pp};
+            const cplx b[4] = {r.Jt[j].ss, r.Jt[j].sp, r.Jt[j].ps, r.Jt[j].pp};
+            for (int c = 0; c < 4; ++c) {
+                Jr[8 * j + 2 * c] = a[c].real();
+                Jr[8 * j + 2 * c + 1] = a[c].imag();
+                Jt[8 * j + 2 * c] = b[c].real();
+                Jt[8 * j + 2 * c + 1] = b[c].imag();
+            }
+            const double er[4] = {r.DEr[j].ss, r.DEr[j].sp, r.DEr[j].ps, r.DEr[j].pp};
+            const double et[4] = {r.DEt[j].ss, r.DEt[j].sp, r.DEt[j].ps, r.DEt[j].pp};
+            std::memcpy(DEr + 4 * j, er, sizeof er);
+            std::memcpy(DEt + 4 * j, et, sizeof et);
+        }
+        *eig_info = r.eig_info;
+        *n_schur = r.n_schur;
+    });
+}
+
+// One 2-D solve. jones = [Es.re, Es.im, Ep.re, Ep.im]. DEr/DEt: N real; Rs, Rp, Ts, Tp: N complex.
+int ref_solve2d(const double* stack, double wavelength, double theta_deg, double phi_deg, int M1, int M2,
+                const double* jones, double* DEr, double* DEt, double* Rs, double* Rp, double* Ts, double* Tp,
+                int* status, int* eig_info) {
+    return guarded([&] {
+        const rcwa::Jones2D j{cplx{jones[0], jones[1]}, cplx{jones[2], jones[3]}};
+        const rcwa::Result2D r = rcwa::solve2d(decode2d(stack), wavelength, theta_deg, phi_deg, M1, M2, j);
+        std::memcpy(DEr, r.DEr.data(), r.DEr.size() * sizeof(double));
+        std::memcpy(DEt, r.DEt.data(), r.DEt.size() * sizeof(double));
+        put(Rs, r.Rs);
+        put(Rp, r.Rp);
+        put(Ts, r.Ts);
+        put(Tp, r.Tp);
+        *status = static_cast<int>(r.status);
+        *eig_info = r.eig_info;
+    });
+}
+
+// eig_qr on a column-major n x n complex matrix. w: n complex; V: n x n complex; residual: eig_residual.
+int ref_eig(int n, const double* A, double* w, double* V, int* info, double* residual) {
+    return guarded([&] {
+        stablercwa::Matrix a(n, n);
+        std::memcpy(a.data(), A, sizeof(double) * 2 * n * n);
+        const stablercwa::EigResult r = stablercwa::eig_qr(a);
+        put(w, r.values);
+        if (r.info == 0) std::memcpy(V, r.vectors.data(), sizeof(double) * 2 * n * n);
+        *info = r.info;
+        if (residual) *residual = r.info == 0 ? stablercwa::eig_residual(a, r) : -1.0;
+    });
+}
+
+// The layer matrix the reference hands to its eigensolver (TE: Kx^2 - [[eps]], TM: the inverse-rule form).
+int ref_layer_matrix(const double* stack, int layer, double wavelength, double theta_deg, int M, int pol, double* A) {
+    return guarded([&] {
+        const rcwa::Stack s = decode1d(stack);
+        const stablercwa::Matrix a = rcwa::layer_matrix(s.layers.at(static_cast<std::size_t>(layer)), s.period,
+                                                        wavelength, s.nI, theta_deg, M,
+                                                        pol ? rcwa::Polarization::TM : rcwa::Polarization::TE);
+        std::memcpy(A, a.data(), sizeof(double) * 2 * a.rows() * a.cols());
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// Device-side descriptors shared by the host runtime and the kernels.
+#pragma once
+
+#include <cstdint>
+
+// Largest matrix order the eigensolver's LDS rotation table holds.
+constexpr int EIG_MAX_N_HOST = 1024;
+
+// Status word of one instance.
+enum : int {
+    ST_OK = 0,
+    ST_NUMERICAL = 1,      // singular system, non-finite value or unconverged eigensolve: results are NaN
+    ST_WORKSPACE = 2,      // internal: the instance's device workspace was too small (never expected)
+    ST_RAYLEIGH = 3,       // crossed gratings: an order is exactly grazing in a homogeneous medium
+};
+
+// One planar (TE/TM) or conical instance.
+struct Inst1D {
+    int M, N, pol, L;          // pol: 0 TE, 1 TM, 2 conical
+    int first_task, n_tasks;   // layer tasks (grating layers) of this instance
+    double wavelength, theta_deg, phi_deg;
+    int64_t stack_off;         // offset of the flat stack record (doubles)
+    int64_t layout_off;        // offset of the layout table (int64)
+    int64_t out_off;           // offset of this instance's orders in the result arrays
+    int64_t ws_lo, ws_hi;      // bump region of the S-matrix recursion (cd units)
+};
+
+// One layer task: a grating layer of an instance that needs an eigensolve.
+struct LayerTask {
+    int inst, layer;
+};
+
+// Layout table of a planar instance (int64 cd offsets into the workspace):
+//   [0]                      kx (N)
+//   [1 + 2r], [2 + 2r]       q, p of region r = 0 .. L+1 (N each)
+//   [1 + 2(L+2) + j]         base of layer j's slots, or -1 for a uniform film:
+//                            S0..S3 (N x N each), then lambda, ort, tmp (N each), pivots (N ints)
+constexpr int LAYOUT_KX = 0;
+__host__ __device__ inline int layout_q(int r) { return 1 + 2 * r; }
+__host__ __device__ inline int layout_p(int r) { return 2 + 2 * r; }
+__host__ __device__ inline int layout_layer(int L, int j) { return 1 + 2 * (L + 2) + j; }
+__host__ __device__ inline int layout_size(int L) { return 1 + 2 * (L + 2) + L; }
+// slots of a grating layer of order n
+__host__ __device__ inline int64_t slot_words(int64_t n) { return 4 * n * n + 4 * n; }
+
+// Per-instance results.
+struct Out1D {
+    double* R;      // complex, N per instance (planar) / 4N Jones (conical)
+    double* T;
+    double* DEr;    // N (planar) / 4N (conical)
+    double* DEt;
+    int* status;
+    int* eig_info;
+};
+
+// Batched dense kernels that mirror the reference's hook shapes (custom_eig / custom_solve /
+// custom_multiply, rcwa1d.hpp:122-136): one item per workgroup, column-major complex.
+struct DenseItem {
+    int n, nrhs, k;            // zgemm: C(n x nrhs) = A(n x k) B(k x nrhs); zgesv: A n x n, B n x nrhs
+    int64_t a, b, c, w;        // cd offsets into one arena
+    int64_t scratch;           // cd offset of per-item scratch
+};

@@ -1,0 +1,394 @@
+// Batched complex nonsymmetric eigensolver for gfx950: one 256-thread workgroup per matrix.
+//
+// The algorithm is the reference's eig_qr (include/stablercwa/eig_qr.hpp:1008-1124), i.e. EISPACK
+// CORTH + COMQR2 without balancing:
+//   1. Householder reduction to upper Hessenberg form            (eig_qr.hpp:231-272)
+//   2. accumulate the reflectors into Z                          (eig_qr.hpp:354-378)
+//   3. clear below the sub-diagonal, make the sub-diagonal real  (eig_qr.hpp:420-439)
+//   4. single-shift QR sweeps with complex-cosine / real-sine rotations, Wilkinson shift with
+//      power-of-two scaling, exceptional shifts at iterations 10 and 20, deflation
+//      |h(l,l-1)| <= eps (|h(l-1,l-1)| + |h(l,l)|), max_iter sweeps per eigenvalue
+//                                                                (eig_qr.hpp:442-607, 1061-1095)
+//   5. back-substitution on T with the overflow control and the denominator floor
+//      rel * max(|l_i|, |l_j|), V = Z X, columns normalised to unit 2-norm  (eig_qr.hpp:866-982)
+//
+// How the work maps onto the workgroup:
+//   * Householder steps: one wave per column for the left reflector (lane-strided dot products),
+//     one thread per row for the right reflector.
+//   * QR sweep, left rotations: one thread per column.  The rotation chain G_{l+1..en} is serial;
+//     the thread owning column i-1 forms G_i from the carried h(i-1,i-1) and the real h(i,i-1),
+//     publishes it in LDS, and after one barrier every thread applies it to its own columns.
+//     Each column streams down its rows with the running value in a register.
+//   * QR sweep, right rotations on H and Z: one thread per row; a row passes through all the
+//     sweep's rotations in order with the running value in a register, so H and Z rows are
+//     independent and the column-major loads are coalesced across threads.
+//   * Back-substitution: one thread per eigenvector, X kept row-major so the per-step updates are
+//     coalesced; V = Z X on MFMA (wg_gemm with X^T), then a wave per column normalises.
+#pragma once
+
+#include "batch.hpp"
+#include "wg_la.hpp"
+
+constexpr int EIG_MAX_N = EIG_MAX_N_HOST;   // largest order the LDS rotation table holds
+
+struct EigScratch {
+    cd* ort;     // n
+    cd* tmp;     // n
+};
+
+// Eigen-decompose the n x n matrix A (column-major, leading dimension n) in place.
+//   in : A = the matrix (overwritten)
+//   out: A = right eigenvectors (unit 2-norm columns) when info == 0; w = eigenvalues
+// Z and X are n x n scratch matrices.  Returns info (0, or k > 0 when eigenvalue k did not converge).
+DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z, cd* __restrict__ X,
+               EigScratch s, int max_iter, double eps) {
+    const int tid = threadIdx.x;
+    const int lane = lane_id(), wave = wave_id();
+    cd* H = A;
+    cd* ort = s.ort;
+    auto h = [&](int i, int j) -> cd& { return H[i + (long)j * n]; };
+
+    // ---- 1. Householder reduction to Hessenberg form (CORTH)
+    for (int m = 1; m < n - 1; ++m) {
+        cd* x = H + (long)(m - 1) * n;
+        double part = 0.0;
+        for (int i = m + tid; i < n; i += NT) part += abs1(x[i]);
+        const double scale = block_sum(part);
+        if (scale == 0.0) {
+            if (tid == 0) ort[m] = cd{0.0, 0.0};
+            __syncthreads();
+            continue;
+        }
+        double hp = 0.0;
+        for (int i = m + tid; i < n; i += NT) {
+            const cd u = cdiv(x[i], scale);
+            ort[i] = u;
+            hp += norm2(u);
+        }
+        double hh = block_sum(hp);          // also a barrier: ort[] is complete
+        double g = sqrt(hh);
+        const double f = cabs(ort[m]);
+        if (f == 0.0) {
+            __syncthreads();
+            if (tid == 0) {
+                ort[m] = cd{g, 0.0};
+                x[m] = cd{scale, 0.0};
+            }
+        } else {
+            hh += f * g;
+            g /= f;
+            const cd om = ort[m];
+            __syncthreads();
+            if (tid == 0) ort[m] = om * (1.0 + g);
+        }
+        __syncthreads();
+        // left: columns m..n-1, one wave per column
+        const int len = n - m;
+        for (int j = m + wave; j < n; j += NWAVE) {
+            cd* aj = H + (long)j * n;
+            double sr = 0.0, si = 0.0;
+            for (int i = lane; i < len; i += 64) {
+                const cd u = ort[m + i], a = aj[m + i];
+                sr += u.x * a.x + u.y * a.y;
+                si += u.x * a.y - u.y * a.x;
+            }
+            sr = wave_sum(sr);
+            si = wave_sum(si);
+            const cd t = cd{-sr / hh, -si / hh};
+            for (int i = lane; i < len; i += 64) aj[m + i] = aj[m + i] + t * ort[m + i];
+        }
+        __syncthreads();
+        // right: rows 0..n-1, one thread per row
+        for (int i = tid; i < n; i += NT) {
+            cd acc{0.0, 0.0};
+            for (int j = n - 1; j >= m; --j) acc = acc + ort[j] * h(i, j);
+            acc = cdiv(acc, hh);
+            for (int j = m; j < n; ++j) h(i, j) = h(i, j) - conj(ort[j]) * acc;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            ort[m] = ort[m] * scale;
+            x[m] = x[m] * (-g);
+        }
+        __syncthreads();
+    }
+
+    // ---- 2. Z = P_1 ... P_{n-2}
+    wg_identity(n, Z, n);
+    cd* v = s.tmp;
+    for (int i = n - 2; i >= 1; --i) {
+        const cd oi = ort[i];
+        const cd sub0 = h(i, i - 1);
+        if (is_zero(oi) || is_zero(sub0)) continue;
+        const int e = binary_exponent(abs1(oi));
+        const cd sub = scale2(sub0, -e);
+        const cd v0 = scale2(oi, -e);
+        const double minus_h = sub.x * v0.x + sub.y * v0.y;
+        const int len = n - i;
+        for (int k = tid; k < len; k += NT) v[k] = k == 0 ? v0 : scale2(h(i + k, i - 1), -e);
+        __syncthreads();
+        for (int j = i + wave; j < n; j += NWAVE) {
+            cd* zj = Z + (long)j * n + i;
+            double sr = 0.0, si = 0.0;
+            for (int k = lane; k < len; k += 64) {
+                const cd u = v[k], a = zj[k];
+                sr += u.x * a.x + u.y * a.y;
+                si += u.x * a.y - u.y * a.x;
+            }
+            sr = wave_sum(sr);
+            si = wave_sum(si);
+            const cd t = cd{sr / minus_h, si / minus_h};
+            for (int k = lane; k < len; k += 64) zj[k] = zj[k] + t * v[k];
+        }
+        __syncthreads();
+    }
+
+    // ---- 3. clear below the sub-diagonal; make the sub-diagonal real (D^H H D, Z D)
+    for (int e = tid; e < n * n; e += NT) {
+        const int i = e % n, j = e / n;
+        if (i >= j + 2) h(i, j) = cd{0.0, 0.0};
+    }
+    __syncthreads();
+    // y_i (unit phases) form a serial chain through the column scaling of step i-1
+    cd* y = s.tmp;
+    if (tid == 0) {
+        cd prev{1.0, 0.0};
+        bool prev_scaled = false;
+        y[0] = cd{1.0, 0.0};
+        for (int i = 1; i < n; ++i) {
+            cd sub = h(i, i - 1);
+            if (prev_scaled) sub = sub * prev;
+            if (sub.y == 0.0) {
+                h(i, i - 1) = sub;
+                y[i] = cd{1.0, 0.0};
+                prev_scaled = false;
+            } else {
+                double norm = 0.0;
+                const cd yy = unit_phase(sub, norm);
+                h(i, i - 1) = cd{norm, 0.0};
+                y[i] = yy;
+                prev = yy;
+                prev_scaled = true;
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < n * n; e += NT) {
+        const int i = e % n, j = e / n;
+        if (i <= j) {
+            cd val = h(i, j);
+            if (i >= 1) val = conj(y[i]) * val;
+            val = val * y[j];
+            h(i, j) = val;
+        }
+        Z[e] = Z[e] * y[j];
+    }
+    __syncthreads();
+
+    // ---- 4. shifted QR iterations (COMQR2)
+    __shared__ double rot_c[EIG_MAX_N + 1], rot_ci[EIG_MAX_N + 1], rot_s[EIG_MAX_N + 1];
+    __shared__ int s_info;
+    cd t{0.0, 0.0};
+    int en = n - 1, its = 0, info = 0;
+    while (en >= 0) {
+        int lc = 0;
+        for (int l = 1 + tid; l <= en; l += NT)
+            if (abs1(h(l, l - 1)) <= eps * (abs1(h(l - 1, l - 1)) + abs1(h(l, l)))) lc = l;
+        const int l = block_max_int(lc);
+        if (l == en) {
+            __syncthreads();
+            if (tid == 0) {
+                const cd val = h(en, en) + t;
+                w[en] = val;
+                h(en, en) = val;
+            }
+            __syncthreads();
+            --en;
+            its = 0;
+            continue;
+        }
+        if (its == max_iter) {
+            info = en + 1;
+            break;
+        }
+        cd sh;
+        if (its == 10 || its == 20) {
+            sh = cd{fabs(h(en, en - 1).x) + (en >= 2 ? fabs(h(en - 1, en - 2).x) : 0.0), 0.0};
+        } else {
+            const cd a = h(en - 1, en - 1), b = h(en - 1, en), d = h(en, en);
+            const double c = h(en, en - 1).x;
+            const cd yv = 0.5 * (a - d);
+            const int e = binary_exponent(fmax(fmax(abs1(b), fabs(c)), abs1(yv)));
+            const cd xv = scale2(b, -e) * ldexp(c, -e);
+            if (is_zero(xv)) {
+                sh = d;
+            } else {
+                const cd ys = scale2(yv, -e);
+                cd zz = csqrt(ys * ys + xv);
+                if (ys.x * zz.x + ys.y * zz.y < 0.0) zz = -zz;
+                sh = d - scale2(cdiv(xv, ys + zz), e);
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i <= en; i += NT) h(i, i) = h(i, i) - sh;
+        t = t + sh;
+        ++its;
+        __syncthreads();
+
+        // (1) left rotations: one thread per column c in [l, n-1]
+        constexpr int QMAX = (EIG_MAX_N + NT - 1) / NT;
+        cd carry[QMAX];
+#pragma unroll
+        for (int q = 0; q < QMAX; ++q) {
+            const int c = l + tid + q * NT;
+            if (c < n) carry[q] = h(l, c);
+        }
+        for (int i = l + 1; i <= en; ++i) {
+            // owner of column i-1 forms rotation G_i from (h(i-1,i-1), h(i,i-1))
+            const int owner = i - 1 - l;
+            if ((owner % NT) == tid) {
+                const int q = owner / NT;
+                cd xc{0.0, 0.0};
+#pragma unroll
+                for (int qq = 0; qq < QMAX; ++qq)
+                    if (qq == q) xc = carry[qq];
+                double xr = xc.x, xi = xc.y, sub = h(i, i - 1).x;
+                const double mm = fmax(fmax(fabs(xr), fabs(xi)), fabs(sub));
+                int e = 0;
+                if (!(mm > 1e-140 && mm < 1e140)) {
+                    e = binary_exponent(mm);
+                    xr = ldexp(xr, -e);
+                    xi = ldexp(xi, -e);
+                    sub = ldexp(sub, -e);
+                }
+                const double nrm = sqrt(xr * xr + xi * xi + sub * sub);
+                h(i - 1, i - 1) = cd{e ? ldexp(nrm, e) : nrm, 0.0};
+                h(i, i - 1) = cd{0.0, 0.0};
+                rot_c[i] = xr / nrm;
+                rot_ci[i] = xi / nrm;
+                rot_s[i] = sub / nrm;
+            }
+            __syncthreads();
+            const double cr = rot_c[i], ci = rot_ci[i], sn = rot_s[i];
+#pragma unroll
+            for (int q = 0; q < QMAX; ++q) {
+                const int c = l + tid + q * NT;
+                if (c >= i && c < n) {
+                    const cd yv = carry[q];
+                    const cd z = h(i, c);
+                    h(i - 1, c) = cd{cr * yv.x + ci * yv.y + sn * z.x, cr * yv.y - ci * yv.x + sn * z.y};
+                    carry[q] = cd{cr * z.x - ci * z.y - sn * yv.x, cr * z.y + ci * z.x - sn * yv.y};
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < QMAX; ++q) {
+            const int c = l + tid + q * NT;
+            if (c >= en && c < n) h(en, c) = carry[q];
+        }
+        __syncthreads();
+
+        // (2) phase that makes r(en, en) real
+        const cd hee = h(en, en);
+        const bool has_phase = hee.y != 0.0;
+        cd phase{1.0, 0.0};
+        if (has_phase) {
+            double nrm = 0.0;
+            phase = unit_phase(hee, nrm);
+            __syncthreads();
+            if (tid == 0) h(en, en) = cd{nrm, 0.0};
+            for (int j = en + 1 + tid; j < n; j += NT) h(en, j) = conj(phase) * h(en, j);
+        }
+        __syncthreads();
+
+        // (3) right rotations: rows of H (0..en) and Z (0..n-1), one thread per row
+        for (int r = tid; r < en + 1 + n; r += NT) {
+            const bool zrow = r > en;
+            const int row = zrow ? r - en - 1 : r;
+            cd* base = zrow ? Z : H;
+            const int j0 = zrow ? l + 1 : max(l + 1, row);
+            cd a = base[row + (long)(j0 - 1) * n];
+            for (int j = j0; j <= en; ++j) {
+                const double cr = rot_c[j], ci = rot_ci[j], sn = rot_s[j];
+                const cd b = base[row + (long)j * n];
+                base[row + (long)(j - 1) * n] = cd{cr * a.x - ci * a.y + sn * b.x, cr * a.y + ci * a.x + sn * b.y};
+                a = cd{cr * b.x + ci * b.y - sn * a.x, cr * b.y - ci * b.x - sn * a.y};
+            }
+            if (has_phase) a = a * phase;
+            base[row + (long)en * n] = a;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) s_info = info;
+    __syncthreads();
+    info = s_info;
+    if (info != 0) {
+        for (int i = tid; i <= en; i += NT) w[i] = h(i, i) + t;
+        __syncthreads();
+        return info;
+    }
+
+    // ---- 5. eigenvectors of T, V = Z X, normalise
+    for (int j = 0; j + 1 < n; ++j)
+        if (tid == 0) h(j + 1, j) = cd{0.0, 0.0};
+    __syncthreads();
+    double nm = 0.0;
+    for (int e = tid; e < n * n; e += NT) {
+        const int i = e % n, j = e / n;
+        if (i <= j) nm = fmax(nm, abs1(H[e]));
+    }
+    const double tnorm = block_max(nm);
+    const double rel = fmax(eps, kEps);
+    double tiny = rel * tnorm;
+    if (tiny == 0.0) tiny = 2.2250738585072014e-308;
+    constexpr double kQuotientMax = 1e300;
+    const double kShrink = ldexp(1.0, -512);
+    // X row-major: X[k*n + j] = component k of eigenvector j
+    for (int e = tid; e < n * n; e += NT) {
+        const int k = e / n, j = e % n;
+        X[e] = k < j ? h(k, j) : (k == j ? cd{1.0, 0.0} : cd{0.0, 0.0});
+    }
+    __syncthreads();
+    for (int j = tid; j < n; j += NT) {
+        const cd wj = w[j];
+        for (int i = j - 1; i >= 0; --i) {
+            const cd wi = w[i];
+            cd yv = wj - wi;
+            if (is_zero(yv)) {
+                yv = cd{tiny, 0.0};
+            } else {
+                const double fl = rel * fmax(abs1(wj), abs1(wi));
+                if (abs1(yv) < fl) yv = cd{fl, 0.0};
+            }
+            for (int pass = 0; pass < 3 && abs1(X[(long)i * n + j]) > abs1(yv) * kQuotientMax; ++pass)
+                for (int k = 0; k <= j; ++k) X[(long)k * n + j] = X[(long)k * n + j] * kShrink;
+            const cd xi = cdiv(X[(long)i * n + j], yv);
+            X[(long)i * n + j] = xi;
+            const double tr = abs1(xi);
+            cd xs = xi;
+            if (tr != 0.0 && tr + 1.0 / tr <= tr) {
+                for (int k = 0; k <= j; ++k) X[(long)k * n + j] = cdiv(X[(long)k * n + j], tr);
+                xs = X[(long)i * n + j];
+            }
+            const cd* ti = H + (long)i * n;
+            for (int k = 0; k < i; ++k) X[(long)k * n + j] = X[(long)k * n + j] + xs * ti[k];
+        }
+    }
+    __syncthreads();
+    // V = Z X into A (T is no longer needed)
+    wg_gemm<GEMM_SET, true>(n, n, n, Z, n, X, n, A, n);
+    for (int j = wave; j < n; j += NWAVE) {
+        cd* vj = A + (long)j * n;
+        double big = 0.0;
+        for (int i = lane; i < n; i += 64) big = fmax(big, fmax(fabs(vj[i].x), fabs(vj[i].y)));
+        big = wave_max(big);
+        if (big == 0.0 || !isfinite(big)) continue;
+        double ssq = 0.0;
+        for (int i = lane; i < n; i += 64) ssq += norm2(cdiv(vj[i], big));
+        ssq = wave_sum(ssq);
+        const double inv = 1.0 / sqrt(ssq);
+        for (int i = lane; i < n; i += 64) vj[i] = cdiv(vj[i], big) * inv;
+    }
+    __syncthreads();
+    return 0;
+}

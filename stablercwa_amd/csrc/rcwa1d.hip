@@ -1,0 +1,269 @@
+// Planar (TE/TM) RCWA on gfx950: batched kernels over instances of one launch plan.
+//
+//   k_setup1d      one workgroup per instance: kx_m and the analytic modes of the incidence medium,
+//                  exit medium and uniform films           (rcwa1d.hpp:410-415, 522-544)
+//   k_assemble1d   one workgroup per grating layer: analytic Fourier coefficients, Toeplitz matrix,
+//                  layer matrix A (TE: Kx^2 - [[eps]]; TM: [[1/eps]]^-1 (Kx [[eps]]^-1 Kx - I))
+//                                                          (rcwa1d.hpp:366-407, 432-468)
+//   k_eig1d        one workgroup per grating layer: eigenpairs of A (CORTH + COMQR2, eig.hpp)
+//   k_modes1d      one workgroup per grating layer: q = branch(sqrt(lambda)), V = W q or [[1/eps]] W q
+//                                                          (rcwa1d.hpp:949-961)
+//   k_smatrix1d    one workgroup per instance: the whole S-matrix recursion with the reference's
+//                  zero / diagonal / dense operator algebra, the top block, the downward sweep and
+//                  the efficiencies                        (rcwa1d.hpp:549-808, 963-1055)
+#include "batch.hpp"
+#include "eig.hpp"
+#include "rcwa_common.hpp"
+
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_setup1d(const Inst1D* __restrict__ insts, const double* __restrict__ stacks,
+                                                const int64_t* __restrict__ layouts, cd* __restrict__ ws) {
+    const Inst1D in = insts[blockIdx.x];
+    const double* st = stacks + in.stack_off;
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N, M = in.M, L = in.L;
+    const double period = st[0];
+    const cd nI = cd{st[1], st[2]}, nII = cd{st[3], st[4]};
+    cd* kx = ws + lay[LAYOUT_KX];
+    const double sn = nI.x * sin(in.theta_deg * kPi / 180.0);
+    for (int j = threadIdx.x; j < N; j += NT) kx[j] = cd{sn - (j - M) * in.wavelength / period, 0.0};
+    __syncthreads();
+    analytic_modes_1d(nI, kx, N, in.pol, false, ws + lay[layout_q(0)], ws + lay[layout_p(0)]);
+    analytic_modes_1d(nII, kx, N, in.pol, false, ws + lay[layout_q(L + 1)], ws + lay[layout_p(L + 1)]);
+    LayerCursor cur(st);
+    for (int j = 0; j < L; ++j) {
+        const LayerView lv = cur.next();
+        if (lv.homogeneous)
+            analytic_modes_1d(lv.n, kx, N, in.pol, true, ws + lay[layout_q(j + 1)], ws + lay[layout_p(j + 1)]);
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
+                                                   const double* __restrict__ stacks, const int64_t* __restrict__ layouts,
+                                                   cd* __restrict__ ws, int* __restrict__ task_info) {
+    const LayerTask tk = tasks[blockIdx.x];
+    const Inst1D in = insts[tk.inst];
+    const double* st = stacks + in.stack_off;
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N, M = in.M;
+    const double period = st[0];
+    LayerCursor cur(st);
+    LayerView lv = cur.next();
+    for (int j = 0; j < tk.layer; ++j) lv = cur.next();
+    const cd* kx = ws + lay[LAYOUT_KX];
+    cd* base = ws + lay[layout_layer(in.L, tk.layer)];
+    const int64_t NN = (int64_t)N * N;
+    cd *S0 = base, *S1 = base + NN, *S2 = base + 2 * NN, *S3 = base + 3 * NN;
+    cd* coef = base + 4 * NN;                 // lambda + ort hold the 4M+1 coefficients
+    int* piv = reinterpret_cast<int*>(base + 4 * NN + 3 * N);
+    int fail = 0;
+    if (in.pol == 0) {
+        fourier_coefficients(lv, period, 2 * M, false, coef);
+        toeplitz(coef, N, S0);
+        // A = Kx^2 - [[eps]]
+        for (int64_t e = threadIdx.x; e < NN; e += NT) {
+            const int i = (int)(e % N), c = (int)(e / N);
+            cd v = -S0[e];
+            if (i == c) v = v + cd{kx[i].x * kx[i].x, 0.0};
+            S0[e] = v;
+        }
+        __syncthreads();
+    } else {
+        // S1 = [[eps]], S3 = [[1/eps]] (kept for V), S2 = diag(kx)
+        fourier_coefficients(lv, period, 2 * M, false, coef);
+        toeplitz(coef, N, S1);
+        fourier_coefficients(lv, period, 2 * M, true, coef);
+        toeplitz(coef, N, S3);
+        wg_diag(N, kx, S2, N);
+        fail |= wg_lu(N, S1, N, piv);
+        wg_lu_solve(N, S1, N, piv, S2, N, N);
+        // S2 = Kx S2 - I
+        for (int64_t e = threadIdx.x; e < NN; e += NT) {
+            const int i = (int)(e % N), c = (int)(e / N);
+            cd v = kx[i].x * S2[e];
+            if (i == c) v = v - cd{1.0, 0.0};
+            S2[e] = v;
+        }
+        __syncthreads();
+        wg_copy(N, N, S3, N, S0, N);
+        fail |= wg_lu(N, S0, N, piv);
+        wg_lu_solve(N, S0, N, piv, S2, N, N);
+        wg_copy(N, N, S2, N, S0, N);
+    }
+    int bad = 0;
+    for (int64_t e = threadIdx.x; e < NN; e += NT) bad |= !finite(S0[e]);
+    bad = block_max_int(bad);
+    if (threadIdx.x == 0) task_info[blockIdx.x] = (fail || bad) ? -1 : 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_eig1d(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
+                                              const int64_t* __restrict__ layouts, cd* __restrict__ ws,
+                                              int* __restrict__ task_info, int max_iter) {
+    if (task_info[blockIdx.x] != 0) return;        // assembly failed: nothing to decompose
+    const LayerTask tk = tasks[blockIdx.x];
+    const Inst1D in = insts[tk.inst];
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N;
+    const int64_t NN = (int64_t)N * N;
+    cd* base = ws + lay[layout_layer(in.L, tk.layer)];
+    cd *S0 = base, *S1 = base + NN, *S2 = base + 2 * NN;
+    cd* lam = base + 4 * NN;
+    EigScratch s{base + 4 * NN + N, base + 4 * NN + 2 * N};
+    const int info = wg_eig(N, S0, lam, S1, S2, s, max_iter, kEps);
+    if (threadIdx.x == 0) task_info[blockIdx.x] = info;
+}
+
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_modes1d(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
+                                                const int64_t* __restrict__ layouts, cd* __restrict__ ws,
+                                                const int* __restrict__ task_info) {
+    if (task_info[blockIdx.x] != 0) return;
+    const LayerTask tk = tasks[blockIdx.x];
+    const Inst1D in = insts[tk.inst];
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N;
+    const int64_t NN = (int64_t)N * N;
+    cd* base = ws + lay[layout_layer(in.L, tk.layer)];
+    cd *W = base, *V = base + NN, *S3 = base + 3 * NN;
+    cd* lam = base + 4 * NN;
+    cd* q = ws + lay[layout_q(tk.layer + 1)];
+    cd* basis = base + 4 * NN + N;
+    for (int j = threadIdx.x; j < N; j += NT) {
+        const cd qq = branch_q(csqrt(lam[j]));
+        q[j] = qq;
+        basis[j] = is_zero(qq) ? cd{1.0, 0.0} : qq;
+    }
+    __syncthreads();
+    if (in.pol == 0) {
+        wg_scale_cols(N, N, W, N, basis, V, N);
+    } else {
+        wg_gemm<GEMM_SET, false>(N, N, N, S3, N, W, N, V, N);
+        wg_scale_cols(N, N, V, N, basis, V, N);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_smatrix1d(const Inst1D* __restrict__ insts, const double* __restrict__ stacks,
+                                                  const int64_t* __restrict__ layouts, cd* __restrict__ ws,
+                                                  const int* __restrict__ task_info, Out1D out) {
+    const Inst1D in = insts[blockIdx.x];
+    const double* st = stacks + in.stack_off;
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N, M = in.M, L = in.L;
+    const double k0 = 2.0 * kPi / in.wavelength;
+    const cd nI = cd{st[1], st[2]}, nII = cd{st[3], st[4]};
+    cd* R = reinterpret_cast<cd*>(out.R) + in.out_off;
+    cd* T = reinterpret_cast<cd*>(out.T) + in.out_off;
+    double* DEr = out.DEr + in.out_off;
+    double* DEt = out.DEt + in.out_off;
+
+    // failures of assembly or eigensolve (first nonzero eig info in layer order)
+    int eig_info = 0, failed = 0;
+    for (int t = 0; t < in.n_tasks; ++t) {
+        const int v = task_info[in.first_task + t];
+        if (v < 0) failed = 1;
+        else if (v > 0 && eig_info == 0) eig_info = v;
+    }
+    int status = (failed || eig_info) ? ST_NUMERICAL : ST_OK;
+    if (status == ST_OK) {
+        __shared__ double s_thick[SMAT_MAX_LAYERS];
+        LayerCursor cur(st);
+        bool pure = true;
+        SmatCtx c;
+        c.N = N;
+        c.bump = Bump(ws, in.ws_lo, in.ws_hi);
+        c.fail = 0;
+        // regions: 0 = incidence, j+1 = layer j, L+1 = exit
+        auto region = [&](int r) -> Region {
+            Region g;
+            g.q = ws + lay[layout_q(r)];
+            g.p = ws + lay[layout_p(r)];
+            if (r == 0 || r == L + 1) {
+                g.analytic = true;
+                g.fzb = false;
+                return g;
+            }
+            const int64_t b = lay[layout_layer(L, r - 1)];
+            if (b < 0) {
+                g.analytic = true;
+                g.fzb = true;
+            } else {
+                g.analytic = false;
+                g.fzb = true;
+                g.W = ws + b;
+                g.V = ws + b + (int64_t)N * N;
+            }
+            return g;
+        };
+        for (int j = 0; j < L; ++j) {
+            const LayerView lv = cur.next();
+            if (threadIdx.x == 0) s_thick[j] = lv.d;
+            pure = pure && lv.homogeneous;
+        }
+        __syncthreads();
+        run_planar_recursion(c, L, M, k0, s_thick, region, R, T);
+        status = c.fail ? (c.fail == 2 ? ST_WORKSPACE : ST_NUMERICAL) : ST_OK;
+        if (status == ST_OK) {
+            __syncthreads();
+            if (pure)
+                for (int j = threadIdx.x; j < N; j += NT) {
+                    if (j == M) continue;
+                    if (!finite(R[j])) R[j] = cd{0.0, 0.0};
+                    if (!finite(T[j])) T[j] = cd{0.0, 0.0};
+                }
+            __syncthreads();
+            const cd* qI = ws + lay[layout_q(0)];
+            const cd* qII = ws + lay[layout_q(L + 1)];
+            const cd den{-k0 * qI[M].y, k0 * qI[M].x};
+            int bad = 0;
+            for (int j = threadIdx.x; j < N; j += NT) {
+                const cd r = R[j], t = T[j];
+                if (!finite(r) || !finite(t)) bad = 1;
+                const cd kIz{-k0 * qI[j].y, k0 * qI[j].x};
+                const cd kIIz{-k0 * qII[j].y, k0 * qII[j].x};
+                const double der = norm2(r) * cdiv(kIz, den).x;
+                cd factor;
+                if (in.pol == 0) factor = cdiv(kIIz, den);
+                else factor = cdiv(cdiv(kIIz, nII * nII), cdiv(den, nI * nI));
+                const double det = norm2(t) * factor.x;
+                if (!isfinite(der) || !isfinite(det)) bad = 1;
+                DEr[j] = der;
+                DEt[j] = det;
+            }
+            if (block_max_int(bad)) status = ST_NUMERICAL;
+        }
+    }
+    __syncthreads();
+    if (status != ST_OK) {
+        const double nan = __builtin_nan("");
+        for (int j = threadIdx.x; j < N; j += NT) {
+            R[j] = cd{nan, nan};
+            T[j] = cd{nan, nan};
+            DEr[j] = nan;
+            DEt[j] = nan;
+        }
+    }
+    if (threadIdx.x == 0) {
+        out.status[blockIdx.x] = status;
+        out.eig_info[blockIdx.x] = eig_info;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// host launchers
+void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
+                   const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
+                   hipEvent_t* ev) {
+    if (ev) hipEventRecord(ev[0], s);
+    k_setup1d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws);
+    if (n_tasks > 0) k_assemble1d<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
+    if (ev) hipEventRecord(ev[1], s);
+    if (n_tasks > 0) k_eig1d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info, max_iter);
+    if (ev) hipEventRecord(ev[2], s);
+    if (n_tasks > 0) k_modes1d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
+    k_smatrix1d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
+    if (ev) hipEventRecord(ev[3], s);
+}

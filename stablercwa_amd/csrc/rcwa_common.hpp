@@ -1,0 +1,545 @@
+// Pieces of the planar / conical RCWA solve shared by the kernels: flat stack parsing, analytic
+// Fourier coefficients and Toeplitz matrices, homogeneous-medium modes, and the S-matrix
+// recursion with the reference's zero / diagonal / dense operator algebra.
+#pragma once
+
+#include "batch.hpp"
+#include "wg_la.hpp"
+
+constexpr int SMAT_MAX_LAYERS = 128;
+constexpr int SMAT_MAX_BLOCKS = 2 * SMAT_MAX_LAYERS + 1;
+
+// ------------------------------------------------------------------------------------------------
+// Flat planar stack record (see include/srcwa.h): period, nI, nII, L, then per layer
+// d, kind, uniform: n | grating: K, edges[K+1], n[K], method, samples.
+struct LayerView {
+    double d;
+    bool homogeneous;
+    cd n;
+    int K;
+    const double* edges;
+    const double* vals;     // K complex, interleaved
+};
+
+struct LayerCursor {
+    const double* p;
+    int64_t k;
+    DEV explicit LayerCursor(const double* st) : p(st), k(6) {}
+    DEV LayerView next() {
+        LayerView v;
+        v.d = p[k];
+        const int kind = (int)p[k + 1];
+        k += 2;
+        if (kind == 0) {
+            v.homogeneous = true;
+            v.n = cd{p[k], p[k + 1]};
+            v.K = 1;
+            v.edges = nullptr;
+            v.vals = nullptr;
+            k += 2;
+        } else {
+            v.homogeneous = false;
+            v.n = cd{1.0, 0.0};
+            v.K = (int)p[k];
+            v.edges = p + k + 1;
+            v.vals = p + k + 2 + v.K;
+            k += 1 + (v.K + 1) + 2 * v.K + 2;
+        }
+        return v;
+    }
+};
+
+// c_h, h = -hmax..hmax, of eps = n^2 (or 1/eps when inverse), stored at out[h + hmax]
+// (rcwa1d.hpp:366-388; a uniform layer is one segment on [0, period]).
+DEV void fourier_coefficients(const LayerView& lv, double period, int hmax, bool inverse, cd* out) {
+    for (int hh = threadIdx.x; hh <= 2 * hmax; hh += NT) {
+        const int h = hh - hmax;
+        cd sum{0.0, 0.0};
+        const int K = lv.K;
+        if (h == 0) {
+            for (int j = 0; j < K; ++j) {
+                const cd nj = lv.homogeneous ? lv.n : cd{lv.vals[2 * j], lv.vals[2 * j + 1]};
+                cd v = nj * nj;
+                if (inverse) v = cdiv(cd{1.0, 0.0}, v);
+                const double e0 = lv.homogeneous ? 0.0 : lv.edges[j];
+                const double e1 = lv.homogeneous ? period : lv.edges[j + 1];
+                sum = sum + cdiv(v * (e1 - e0), period);
+            }
+        } else {
+            const double w = -2.0 * kPi * h;
+            double prev_s, prev_c;
+            {
+                const double e0 = lv.homogeneous ? 0.0 : lv.edges[0];
+                sincos(-2.0 * kPi * h * e0 / period, &prev_s, &prev_c);
+            }
+            for (int j = 0; j < K; ++j) {
+                const cd nj = lv.homogeneous ? lv.n : cd{lv.vals[2 * j], lv.vals[2 * j + 1]};
+                cd v = nj * nj;
+                if (inverse) v = cdiv(cd{1.0, 0.0}, v);
+                const double e1 = lv.homogeneous ? period : lv.edges[j + 1];
+                double s1, c1;
+                sincos(-2.0 * kPi * h * e1 / period, &s1, &c1);
+                const cd t = v * cd{c1 - prev_c, s1 - prev_s};
+                sum = sum + cd{t.y / w, -t.x / w};
+                prev_s = s1;
+                prev_c = c1;
+            }
+        }
+        out[hh] = sum;
+    }
+    __syncthreads();
+}
+
+// [[f]](m, p) = c_{m-p}; c holds 2N-1 coefficients with c_0 at index N-1.
+DEV void toeplitz(const cd* c, int N, cd* S) {
+    const int64_t NN = (int64_t)N * N;
+    for (int64_t e = threadIdx.x; e < NN; e += NT) {
+        const int i = (int)(e % N), j = (int)(e / N);
+        S[e] = c[i - j + N - 1];
+    }
+    __syncthreads();
+}
+
+// Re q >= 0; propagating (Re q ~ 0) takes Im q <= 0 (rcwa1d.hpp:514-518).
+DEV cd branch_q(cd q) {
+    if (q.x < 0.0) q = -q;
+    if (q.x < 1e-10 && q.y > 0.0) q = -q;
+    return q;
+}
+
+// Homogeneous medium: kz = sqrt(n^2 - kx^2) (Im >= 0; Re >= 0 when Im ~ 0), q = -i kz,
+// p = q (TE) or q / n^2 (TM); finite layers use q_basis = 1 for an exact zero root (rcwa1d.hpp:522-544).
+DEV void analytic_modes_1d(cd n, const cd* kx, int N, int pol, bool fzb, cd* q, cd* p) {
+    const cd n2 = n * n;
+    for (int j = threadIdx.x; j < N; j += NT) {
+        cd kz = csqrt(n2 - cd{kx[j].x * kx[j].x, 0.0});
+        if (kz.y < 0.0) kz = -kz;
+        if (fabs(kz.y) < 1e-12 && kz.x < 0.0) kz = -kz;
+        const cd qq{kz.y, -kz.x};
+        q[j] = qq;
+        const cd basis = (fzb && is_zero(qq)) ? cd{1.0, 0.0} : qq;
+        p[j] = pol == 0 ? basis : cdiv(basis, n2);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Bump allocator over one instance's workspace: temporaries from the low end (released per
+// S-matrix block), kept results from the high end.  Requests that do not fit set the workspace
+// failure and return the start of the region, so every access stays inside the instance's slab.
+struct Bump {
+    cd* ws;
+    int64_t lo, hi, lo0;
+    int overflow;
+    DEV Bump() : ws(nullptr), lo(0), hi(0), lo0(0), overflow(0) {}
+    DEV Bump(cd* w, int64_t l, int64_t h) : ws(w), lo(l), hi(h), lo0(l), overflow(0) {}
+    DEV cd* tmp(int64_t n) {
+        n = (n + 15) & ~int64_t(15);
+        if (lo + n > hi) {
+            overflow = 1;
+            return ws + lo0;
+        }
+        cd* r = ws + lo;
+        lo += n;
+        return r;
+    }
+    DEV cd* keep(int64_t n) {
+        n = (n + 15) & ~int64_t(15);
+        if (hi - n < lo) {
+            overflow = 1;
+            return ws + lo0;
+        }
+        hi -= n;
+        return ws + hi;
+    }
+};
+
+struct SmatCtx {
+    int N;
+    Bump bump;
+    int fail;       // 1 numerical, 2 workspace
+    DEV void check() {
+        if (bump.overflow) fail = 2;
+    }
+};
+
+// zero / diagonal / dense operator of order N (rcwa1d.hpp:549-629)
+enum { OP_ZERO = 0, OP_DIAG = 1, OP_DENSE = 2 };
+struct DOp {
+    int kind;
+    cd* p;
+};
+
+struct Region {
+    bool analytic;
+    bool fzb;           // finite-layer zero-root basis
+    cd* q;
+    cd* p;
+    cd* W;
+    cd* V;
+};
+
+struct SBlock {
+    DOp Tdd, Rdu, Rud, Tuu;
+};
+
+DEV DOp op_zero() { return DOp{OP_ZERO, nullptr}; }
+
+DEV DOp op_mul(SmatCtx& c, DOp a, DOp b) {
+    const int N = c.N;
+    if (a.kind == OP_ZERO || b.kind == OP_ZERO) return op_zero();
+    if (a.kind == OP_DIAG && b.kind == OP_DIAG) {
+        cd* v = c.bump.tmp(N);
+        for (int i = threadIdx.x; i < N; i += NT) v[i] = a.p[i] * b.p[i];
+        __syncthreads();
+        return DOp{OP_DIAG, v};
+    }
+    cd* m = c.bump.tmp((int64_t)N * N);
+    if (a.kind == OP_DIAG) wg_scale_rows(N, N, a.p, b.p, N, m, N);
+    else if (b.kind == OP_DIAG) wg_scale_cols(N, N, a.p, N, b.p, m, N);
+    else wg_gemm<GEMM_SET, false>(N, N, N, a.p, N, b.p, N, m, N);
+    return DOp{OP_DENSE, m};
+}
+
+DEV DOp op_add(SmatCtx& c, DOp a, DOp b) {
+    const int N = c.N;
+    if (b.kind == OP_ZERO) return a;
+    if (a.kind == OP_ZERO) return b;
+    if (a.kind == OP_DIAG && b.kind == OP_DIAG) {
+        cd* v = c.bump.tmp(N);
+        for (int i = threadIdx.x; i < N; i += NT) v[i] = a.p[i] + b.p[i];
+        __syncthreads();
+        return DOp{OP_DIAG, v};
+    }
+    const int64_t NN = (int64_t)N * N;
+    cd* m = c.bump.tmp(NN);
+    for (int64_t e = threadIdx.x; e < NN; e += NT) {
+        const int i = (int)(e % N), j = (int)(e / N);
+        cd v;
+        if (a.kind == OP_DENSE && b.kind == OP_DENSE) v = a.p[e] + b.p[e];
+        else if (a.kind == OP_DENSE) v = i == j ? a.p[e] + b.p[i] : a.p[e];
+        else v = i == j ? b.p[e] + a.p[i] : b.p[e];
+        m[e] = v;
+    }
+    __syncthreads();
+    return DOp{OP_DENSE, m};
+}
+
+// y = a x (y must not alias x)
+DEV void op_apply(SmatCtx& c, DOp a, const cd* x, cd* y) {
+    const int N = c.N;
+    for (int i = threadIdx.x; i < N; i += NT) {
+        if (a.kind == OP_DENSE) {
+            cd s{0.0, 0.0};
+            for (int j = 0; j < N; ++j) s = s + a.p[i + (int64_t)j * N] * x[j];
+            y[i] = s;
+        } else if (a.kind == OP_DIAG) {
+            y[i] = a.p[i] * x[i];
+        } else {
+            y[i] = cd{0.0, 0.0};
+        }
+    }
+    __syncthreads();
+}
+
+DEV cd* op_to_dense(SmatCtx& c, DOp a) {
+    const int N = c.N;
+    cd* m = c.bump.tmp((int64_t)N * N);
+    if (a.kind == OP_DENSE) wg_copy(N, N, a.p, N, m, N);
+    else if (a.kind == OP_DIAG) wg_diag(N, a.p, m, N);
+    else wg_fill((int64_t)N * N, m, cd{0.0, 0.0});
+    return m;
+}
+
+// B <- A^{-1} B (A is preserved).  Singular -> numerical failure.
+DEV void dense_solve(SmatCtx& c, const cd* A, cd* B, int nrhs) {
+    const int N = c.N;
+    cd* lu = c.bump.tmp((int64_t)N * N);
+    int* piv = reinterpret_cast<int*>(c.bump.tmp(N));
+    c.check();
+    if (c.fail) return;
+    wg_copy(N, N, A, N, lu, N);
+    if (wg_lu(N, lu, N, piv)) {
+        c.fail = 1;
+        return;
+    }
+    wg_lu_solve(N, lu, N, piv, B, N, nrhs);
+}
+// B <- A^{-1} B, destroying A.
+DEV void dense_solve_inplace(SmatCtx& c, cd* A, cd* B, int nrhs) {
+    const int N = c.N;
+    int* piv = reinterpret_cast<int*>(c.bump.tmp(N));
+    c.check();
+    if (c.fail) return;
+    if (wg_lu(N, A, N, piv)) {
+        c.fail = 1;
+        return;
+    }
+    wg_lu_solve(N, A, N, piv, B, N, nrhs);
+}
+
+// Both sides homogeneous: per-order Fresnel blocks (rcwa1d.hpp:649-669).
+DEV SBlock diagonal_interface(SmatCtx& c, const Region& up, const Region& low) {
+    const int N = c.N;
+    cd* tdd = c.bump.tmp(N);
+    cd* rdu = c.bump.tmp(N);
+    cd* rud = c.bump.tmp(N);
+    cd* tuu = c.bump.tmp(N);
+    for (int j = threadIdx.x; j < N; j += NT) {
+        const cd p1 = up.p[j], p2 = low.p[j];
+        const cd inv = cdiv(cd{1.0, 0.0}, p1 + p2);
+        tdd[j] = (2.0 * p1) * inv;
+        rdu[j] = (p2 - p1) * inv;
+        rud[j] = (p1 - p2) * inv;
+        tuu[j] = (2.0 * p2) * inv;
+    }
+    __syncthreads();
+    return SBlock{DOp{OP_DIAG, tdd}, DOp{OP_DIAG, rdu}, DOp{OP_DIAG, rud}, DOp{OP_DIAG, tuu}};
+}
+
+// T+ and T- (rcwa1d.hpp:672-692)
+DEV void interface_matching(SmatCtx& c, const Region& up, const Region& low, cd*& tp, cd*& tm) {
+    const int N = c.N;
+    const int64_t NN = (int64_t)N * N;
+    cd* mm = c.bump.tmp(NN);
+    cd* nn = c.bump.tmp(NN);
+    c.check();
+    if (c.fail) return;
+    if (low.analytic) {
+        wg_copy(N, N, up.W, N, mm, N);
+        for (int64_t e = threadIdx.x; e < NN; e += NT) {
+            const int i = (int)(e % N);
+            nn[e] = cdiv(cd{1.0, 0.0}, low.p[i]) * up.V[e];
+        }
+        __syncthreads();
+    } else if (up.analytic) {
+        wg_identity(N, mm, N);
+        dense_solve(c, low.W, mm, N);
+        wg_diag(N, up.p, nn, N);
+        dense_solve(c, low.V, nn, N);
+    } else {
+        wg_copy(N, N, up.W, N, mm, N);
+        dense_solve(c, low.W, mm, N);
+        wg_copy(N, N, up.V, N, nn, N);
+        dense_solve(c, low.V, nn, N);
+    }
+    if (c.fail) return;
+    for (int64_t e = threadIdx.x; e < NN; e += NT) {
+        const cd a = mm[e], b = nn[e];
+        mm[e] = 0.5 * (a + b);
+        nn[e] = 0.5 * (a - b);
+    }
+    __syncthreads();
+    tp = mm;
+    tm = nn;
+}
+
+DEV SBlock interface_block(SmatCtx& c, const Region& up, const Region& low) {
+    if (up.analytic && low.analytic) return diagonal_interface(c, up, low);
+    const int N = c.N;
+    const int64_t NN = (int64_t)N * N;
+    cd *tp = nullptr, *tm = nullptr;
+    interface_matching(c, up, low, tp, tm);
+    SBlock s{op_zero(), op_zero(), op_zero(), op_zero()};
+    if (c.fail) return s;
+    cd* tpi = c.bump.tmp(NN);
+    cd* rdu = c.bump.tmp(NN);
+    cd* rud = c.bump.tmp(NN);
+    c.check();
+    if (c.fail) return s;
+    wg_identity(N, tpi, N);
+    dense_solve(c, tp, tpi, N);
+    if (c.fail) return s;
+    wg_gemm<GEMM_SET, false>(N, N, N, tm, N, tpi, N, rdu, N);       // Rdu = T- T+^-1
+    wg_gemm<GEMM_SUB, false>(N, N, N, rdu, N, tm, N, tp, N);        // Tdd = T+ - Rdu T-
+    wg_fill(NN, rud, cd{0.0, 0.0});
+    wg_gemm<GEMM_SUB, false>(N, N, N, tpi, N, tm, N, rud, N);       // Rud = -T+^-1 T-
+    return SBlock{DOp{OP_DENSE, tp}, DOp{OP_DENSE, rdu}, DOp{OP_DENSE, rud}, DOp{OP_DENSE, tpi}};
+}
+
+// exp(-k0 q d); finite-layer exact zero roots use 1/(1+h/2) and (h/2)/(1+h/2) (rcwa1d.hpp:751-784).
+DEV SBlock propagation_block(SmatCtx& c, const cd* q, double d, double k0, bool fzb) {
+    const int N = c.N;
+    cd* x = c.bump.tmp(N);
+    cd* refl = c.bump.tmp(N);
+    int zero = 0, bad = 0;
+    const double h = k0 * d;
+    for (int j = threadIdx.x; j < N; j += NT) {
+        if (fzb && is_zero(q[j])) {
+            if (!isfinite(h)) bad = 1;
+            const double den = 1.0 + 0.5 * h;
+            x[j] = cd{1.0 / den, 0.0};
+            refl[j] = cd{(0.5 * h) / den, 0.0};
+            zero = 1;
+        } else {
+            x[j] = cexp((-k0 * d) * q[j]);
+            refl[j] = cd{0.0, 0.0};
+        }
+    }
+    zero = block_max_int(zero);
+    bad = block_max_int(bad);
+    if (bad) c.fail = 1;
+    if (zero) return SBlock{DOp{OP_DIAG, x}, DOp{OP_DIAG, refl}, DOp{OP_DIAG, refl}, DOp{OP_DIAG, x}};
+    return SBlock{DOp{OP_DIAG, x}, op_zero(), op_zero(), DOp{OP_DIAG, x}};
+}
+
+// g = (I - Rdu r)^-1 Tdd, r' = Rud + Tuu r g (rcwa1d.hpp:794-808)
+DEV void stack_on(SmatCtx& c, const SBlock& s, DOp r, DOp& g, DOp& rn) {
+    const int N = c.N;
+    DOp k = op_mul(c, s.Rdu, r);
+    if (k.kind == OP_ZERO) {
+        g = s.Tdd;
+    } else if (k.kind == OP_DIAG) {
+        for (int i = threadIdx.x; i < N; i += NT) k.p[i] = cdiv(cd{1.0, 0.0}, cd{1.0, 0.0} - k.p[i]);
+        __syncthreads();
+        g = op_mul(c, k, s.Tdd);
+    } else {
+        const int64_t NN = (int64_t)N * N;
+        for (int64_t e = threadIdx.x; e < NN; e += NT) {
+            const int i = (int)(e % N), j = (int)(e / N);
+            cd v = -k.p[e];
+            if (i == j) v = v + cd{1.0, 0.0};
+            k.p[e] = v;
+        }
+        __syncthreads();
+        cd* rhs = op_to_dense(c, s.Tdd);
+        c.check();
+        if (c.fail) return;
+        dense_solve_inplace(c, k.p, rhs, N);
+        g = DOp{OP_DENSE, rhs};
+    }
+    c.check();
+    if (c.fail) return;
+    const DOp rg = op_mul(c, r, g);
+    const DOp t = op_mul(c, s.Tuu, rg);
+    rn = op_add(c, s.Rud, t);
+    c.check();
+}
+
+// Persist an operator into kept storage.
+DEV DOp op_keep(SmatCtx& c, DOp a, cd* slot) {
+    const int N = c.N;
+    if (a.kind == OP_ZERO) return a;
+    const int64_t cnt = a.kind == OP_DIAG ? N : (int64_t)N * N;
+    if (a.p != slot) {
+        for (int64_t e = threadIdx.x; e < cnt; e += NT) slot[e] = a.p[e];
+        __syncthreads();
+    }
+    return DOp{a.kind, slot};
+}
+
+// The whole planar recursion: bottom-up seams, the top block (incident column M only), the
+// downward sweep.  region(r) returns the modes of region r (0 incidence, j+1 layer j, L+1 exit).
+// Writes the reflected / transmitted amplitudes R, T (N each).  On return c.fail says whether the
+// instance failed.  (rcwa1d.hpp:963-1034)
+template <class RegionFn>
+DEV void run_planar_recursion(SmatCtx& c, int L, int M, double k0, const double* thick, RegionFn region, cd* Rout,
+                              cd* Tout) {
+    __shared__ int g_kind[SMAT_MAX_BLOCKS];
+    __shared__ cd* g_ptr[SMAT_MAX_BLOCKS];
+    const int N = c.N;
+    const int64_t NN = (int64_t)N * N;
+    const int nb = 2 * L + 1;
+    cd* rslot[2] = {c.bump.keep(NN), c.bump.keep(NN)};
+    int rs = 0;
+    DOp r = op_zero();
+    c.check();
+    for (int b = nb - 1; b >= 1 && !c.fail; --b) {
+        const int64_t mark = c.bump.lo;
+        const int j = b / 2;
+        SBlock blk;
+        if (b % 2 == 0) {
+            blk = interface_block(c, region(j), region(j + 1));
+        } else {
+            const Region g = region(j + 1);
+            blk = propagation_block(c, g.q, thick[j], k0, g.fzb);
+        }
+        c.check();
+        if (c.fail) break;
+        DOp g, rn;
+        stack_on(c, blk, r, g, rn);
+        if (c.fail) break;
+        cd* gslot = c.bump.keep(g.kind == OP_DIAG ? N : NN);
+        c.check();
+        if (c.fail) break;
+        g = op_keep(c, g, gslot);
+        if (threadIdx.x == 0) {
+            g_kind[b] = g.kind;
+            g_ptr[b] = g.p;
+        }
+        r = op_keep(c, rn, rslot[rs ^ 1]);
+        rs ^= 1;
+        c.bump.lo = mark;
+        __syncthreads();
+    }
+    if (c.fail) return;
+
+    // top block: only the incident column M of Tdd and Rud (rcwa1d.hpp:715-746)
+    const Region up = region(0), low = region(1);
+    cd* tdd = c.bump.tmp(N);
+    cd* rud = c.bump.tmp(N);
+    DOp Rdu, Tuu;
+    if (up.analytic && low.analytic) {
+        const SBlock s = diagonal_interface(c, up, low);
+        for (int j = threadIdx.x; j < N; j += NT) {
+            tdd[j] = j == M ? s.Tdd.p[M] : cd{0.0, 0.0};
+            rud[j] = j == M ? s.Rud.p[M] : cd{0.0, 0.0};
+        }
+        __syncthreads();
+        Rdu = s.Rdu;
+        Tuu = s.Tuu;
+    } else {
+        cd *tp = nullptr, *tm = nullptr;
+        interface_matching(c, up, low, tp, tm);
+        if (c.fail) return;
+        cd* tpi = c.bump.tmp(NN);
+        cd* rdu = c.bump.tmp(NN);
+        c.check();
+        if (c.fail) return;
+        wg_identity(N, tpi, N);
+        dense_solve(c, tp, tpi, N);
+        if (c.fail) return;
+        wg_gemm<GEMM_SET, false>(N, N, N, tm, N, tpi, N, rdu, N);
+        const cd* tm_e = tm + (int64_t)M * N;
+        op_apply(c, DOp{OP_DENSE, rdu}, tm_e, tdd);
+        for (int j = threadIdx.x; j < N; j += NT) tdd[j] = tp[j + (int64_t)M * N] - tdd[j];
+        __syncthreads();
+        op_apply(c, DOp{OP_DENSE, tpi}, tm_e, rud);
+        for (int j = threadIdx.x; j < N; j += NT) rud[j] = -rud[j];
+        __syncthreads();
+        Rdu = DOp{OP_DENSE, rdu};
+        Tuu = DOp{OP_DENSE, tpi};
+    }
+    DOp k = op_mul(c, Rdu, r);
+    cd* down = c.bump.tmp(N);
+    cd* tmpv = c.bump.tmp(N);
+    c.check();
+    if (c.fail) return;
+    for (int j = threadIdx.x; j < N; j += NT) down[j] = tdd[j];
+    __syncthreads();
+    if (k.kind == OP_DIAG) {
+        for (int j = threadIdx.x; j < N; j += NT) down[j] = cdiv(down[j], cd{1.0, 0.0} - k.p[j]);
+        __syncthreads();
+    } else if (k.kind == OP_DENSE) {
+        for (int64_t e = threadIdx.x; e < NN; e += NT) {
+            const int i = (int)(e % N), j = (int)(e / N);
+            cd v = -k.p[e];
+            if (i == j) v = v + cd{1.0, 0.0};
+            k.p[e] = v;
+        }
+        __syncthreads();
+        dense_solve_inplace(c, k.p, down, 1);
+        if (c.fail) return;
+    }
+    op_apply(c, r, down, tmpv);
+    op_apply(c, Tuu, tmpv, Rout);
+    for (int j = threadIdx.x; j < N; j += NT) Rout[j] = rud[j] + Rout[j];
+    __syncthreads();
+    for (int b = 1; b < nb; ++b) {
+        op_apply(c, DOp{g_kind[b], g_ptr[b]}, down, tmpv);
+        cd* sw = down;
+        down = tmpv;
+        tmpv = sw;
+    }
+    for (int j = threadIdx.x; j < N; j += NT) Tout[j] = down[j];
+    __syncthreads();
+}

@@ -1,0 +1,704 @@
+// Host runtime of libsrcwa_amd.so: contexts, launch plans, HBM-footprint admission into chunks,
+// input validation with the reference's error conventions, and the C-ABI of include/srcwa.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/srcwa.h"
+#include "batch.hpp"
+#include "cplx.hpp"
+
+// kernel launchers (rcwa1d.hip, dense.hip)
+void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
+                   const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
+                   hipEvent_t* ev);
+void launch_zgemm_batched(hipStream_t s, int count, const DenseItem* items, cd* arena);
+void launch_zgesv_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info);
+void launch_zgeev_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info, int max_iter);
+
+namespace {
+
+thread_local std::string g_error;
+
+struct Failure {
+    int code;
+    std::string msg;
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        if (e == hipErrorOutOfMemory) throw Failure{SRCWA_ENOMEM, std::string(what) + ": out of device memory"};
+        throw Failure{SRCWA_EDEVICE, std::string(what) + ": " + hipGetErrorString(e)};
+    }
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return SRCWA_OK;
+    } catch (const Failure& e) {
+        g_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_error = "host allocation failed";
+        return SRCWA_ENOMEM;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return SRCWA_EDEVICE;
+    }
+}
+
+void invalid(const std::string& m) { throw Failure{SRCWA_EINVAL, m}; }
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hip_check(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)), "hipMalloc");
+        n = count;
+    }
+    void upload(const std::vector<T>& v, hipStream_t s) {
+        alloc(v.size());
+        if (!v.empty())
+            hip_check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpy H2D");
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+int64_t align16(int64_t x) { return (x + 15) & ~int64_t(15); }
+
+// ---- validation (rcwa1d.hpp:188-229) ------------------------------------------------------------
+bool finite_c(double re, double im) { return std::isfinite(re) && std::isfinite(im); }
+
+void check_index(double re, double im) {
+    if (!finite_c(re, im) || std::hypot(re, im) == 0.0 || re < 0.0 || im < 0.0)
+        invalid("rcwa: refractive index must be finite, nonzero, with Re n >= 0 and Im n >= 0");
+}
+
+struct LayerInfo {
+    bool grating;
+    double d;
+};
+
+// Parses and validates one planar stack record; returns the per-layer kinds.
+std::vector<LayerInfo> parse_stack1d(const double* p, int64_t avail, int M) {
+    if (avail < 6) invalid("rcwa: malformed stack record");
+    const double period = p[0];
+    const int L = static_cast<int>(p[5]);
+    if (!(p[5] >= 0.0) || L > 128) invalid("rcwa: a stack holds 0..128 layers");
+    std::vector<LayerInfo> out;
+    int64_t k = 6;
+    auto need = [&](int64_t n) {
+        if (k + n > avail) invalid("rcwa: malformed stack record");
+    };
+    for (int j = 0; j < L; ++j) {
+        need(2);
+        const double d = p[k];
+        const int kind = static_cast<int>(p[k + 1]);
+        k += 2;
+        if (kind == 0) {
+            need(2);
+            if (!std::isfinite(d) || d < 0.0) invalid("rcwa: layer thickness must be finite and >= 0");
+            check_index(p[k], p[k + 1]);
+            k += 2;
+            out.push_back({false, d});
+        } else if (kind == 1) {
+            need(1);
+            const int K = static_cast<int>(p[k]);
+            if (K < 1 || K > 100000) invalid("rcwa: a grating layer needs K indices and K+1 edges");
+            need(1 + (K + 1) + 2 * K + 2);
+            const double* e = p + k + 1;
+            const double* v = e + K + 1;
+            const int method = static_cast<int>(v[2 * K]);
+            if (method != 0 && method != 1) invalid("rcwa: unknown Fourier method");
+            if (method == 1) throw Failure{SRCWA_EUNSUPPORTED, "rcwa: FFT Fourier coefficients are not provided by this build"};
+            if (!std::isfinite(d) || d < 0.0) invalid("rcwa: layer thickness must be finite and >= 0");
+            for (int i = 0; i <= K; ++i)
+                if (!std::isfinite(e[i]) || (i > 0 && e[i] <= e[i - 1]))
+                    invalid("rcwa: grating edges must be finite and strictly increasing");
+            if (std::abs((e[K] - e[0]) - period) > 1e-10 * period)
+                invalid("rcwa: grating edges must span exactly one period");
+            for (int i = 0; i < K; ++i) check_index(v[2 * i], v[2 * i + 1]);
+            k += 1 + (K + 1) + 2 * K + 2;
+            out.push_back({true, d});
+        } else {
+            invalid("rcwa: unknown layer kind");
+        }
+    }
+    (void)M;
+    return out;
+}
+
+void check_setup(double period, double wavelength, double nIr, double nIi, double theta_deg, int M, int pol) {
+    if (M < 0 || M > (std::numeric_limits<int>::max() - 1) / 4) invalid("rcwa: invalid truncation order M");
+    if (pol != 0 && pol != 1) invalid("rcwa: polarization must be TE or TM");
+    if (!std::isfinite(wavelength) || wavelength <= 0.0 || !std::isfinite(period) || period <= 0.0)
+        invalid("rcwa: wavelength and period must be finite and positive");
+    if (!std::isfinite(theta_deg) || std::abs(theta_deg) >= 90.0)
+        invalid("rcwa: incidence angle must lie strictly between -90 and 90 degrees");
+    check_index(nIr, nIi);
+    if (nIi != 0.0 || nIr <= 0.0) invalid("rcwa: the incident medium must have a real positive index");
+}
+
+// ---- operator-kind simulation (mirrors run_planar_recursion) ------------------------------------
+enum Kind { KZ = 0, KD = 1, KX = 2 };
+int kmul(int a, int b) { return (a == KZ || b == KZ) ? KZ : (a == KD && b == KD) ? KD : KX; }
+int kadd(int a, int b) { return b == KZ ? a : a == KZ ? b : (a == KD && b == KD) ? KD : KX; }
+
+struct PlanarCost {
+    int64_t bump_words;
+    double flops_eig, flops_other;
+};
+
+// general[r]: region r (0 incidence, j+1 layer j, L+1 exit) needs an eigensolve.
+PlanarCost planar_cost(int N, const std::vector<bool>& general, int pol) {
+    const int L = static_cast<int>(general.size()) - 2;
+    const double n = N, n3 = n * n * n;
+    const double LU = 8.0 / 3.0 * n3, SOLVE_N = 8.0 * n3, GEMM = 8.0 * n3;
+    PlanarCost c{0, 0.0, 0.0};
+    int64_t persist = 2 * align16(int64_t(N) * N);
+    int r_worst = KZ, r = KZ;
+    for (int b = 2 * L; b >= 1; --b) {
+        const int j = b / 2;
+        int tdd, rdu, rud, tuu, rdu_w, rud_w;
+        if (b % 2 == 0) {
+            const bool dense = general[j] || general[j + 1];
+            tdd = rdu = rud = tuu = dense ? KX : KD;
+            rdu_w = rdu;
+            rud_w = rud;
+            if (dense) {
+                if (general[j + 1]) c.flops_other += 2 * LU + 2 * SOLVE_N;   // matching solves
+                c.flops_other += LU + SOLVE_N + 3 * GEMM;                     // T+^-1, Rdu, Tdd, Rud
+            }
+        } else {
+            tdd = tuu = KD;
+            rdu = rud = KZ;
+            rdu_w = rud_w = KD;
+        }
+        // flops along the expected kinds
+        const int k = kmul(rdu, r);
+        if (rdu == KX && r == KX) c.flops_other += GEMM;
+        const int g = k == KZ ? tdd : k == KD ? kmul(KD, tdd) : KX;
+        if (k == KX) c.flops_other += LU + SOLVE_N;
+        if (r == KX && g == KX) c.flops_other += GEMM;
+        const int rg = kmul(r, g);
+        if (tuu == KX && rg == KX) c.flops_other += GEMM;
+        r = kadd(rud, kmul(tuu, rg));
+        // memory along the worst-case kinds
+        const int kw = kmul(rdu_w, r_worst);
+        const int gw = kw == KZ ? tdd : kw == KD ? kmul(KD, tdd) : KX;
+        persist += align16(gw == KD ? N : int64_t(N) * N);
+        r_worst = kadd(rud_w, kmul(tuu, kmul(r_worst, gw)));
+    }
+    // top block
+    if (general[0] || general[1]) {
+        if (general[1]) c.flops_other += 2 * LU + 2 * SOLVE_N;
+        c.flops_other += LU + SOLVE_N + GEMM;
+        if (r == KX) c.flops_other += GEMM + LU + 8.0 * n * n;
+    }
+    for (int j = 1; j <= L; ++j)
+        if (general[j]) {
+            c.flops_eig += 100.0 * n3;
+            if (pol == 1) c.flops_other += 2 * LU + 2 * SOLVE_N + GEMM;   // TM assembly + V
+        }
+    const int64_t NN = int64_t(N) * N;
+    c.bump_words = persist + 16 * NN + 64 * int64_t(N) + 4096;
+    return c;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------
+struct srcwa_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct PlanChunk {
+    int i0, i1, t0, t1;
+};
+
+struct srcwa_plan {
+    srcwa_context* ctx = nullptr;
+    int kind = 1;
+    int n_inst = 0, n_tasks = 0;
+    int64_t total_orders = 0;
+    std::vector<PlanChunk> chunks;
+    DevBuf<Inst1D> insts;
+    DevBuf<LayerTask> tasks;
+    DevBuf<double> stacks;
+    DevBuf<int64_t> layouts;
+    DevBuf<cd> ws;
+    DevBuf<int> task_info;
+    DevBuf<double> R, T, DEr, DEt;
+    DevBuf<int> status, eig_info;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool timed = false;
+    double flops[2] = {0.0, 0.0};
+    ~srcwa_plan() {
+        for (hipEvent_t& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    int64_t bytes() const {
+        return insts.bytes() + tasks.bytes() + stacks.bytes() + layouts.bytes() + ws.bytes() + task_info.bytes() +
+               R.bytes() + T.bytes() + DEr.bytes() + DEt.bytes() + status.bytes() + eig_info.bytes();
+    }
+};
+
+namespace {
+// ---- hook-shaped dense batches ------------------------------------------------------------------
+template <class Body>
+int dense_batch(srcwa_context* ctx, Body&& body) {
+    return guarded([&] {
+        if (!ctx) invalid("null context");
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        body(ctx->stream);
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* srcwa_last_error(void) { return g_error.c_str(); }
+const char* srcwa_version(void) { return "srcwa-amd 0.1 (gfx950)"; }
+
+int srcwa_context_create(int device, srcwa_context** out) {
+    return guarded([&] {
+        if (!out) invalid("srcwa_context_create: null output");
+        int count = 0;
+        hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
+        if (device < 0 || device >= count) invalid("srcwa_context_create: no such device");
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        auto* c = new srcwa_context;
+        c->device = device;
+        const hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            hip_check(e, "hipStreamCreate");
+        }
+        *out = c;
+    });
+}
+
+void srcwa_context_destroy(srcwa_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+int srcwa_context_sync(srcwa_context* ctx) {
+    return guarded([&] {
+        if (!ctx) invalid("null context");
+        hip_check(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                        int n_problems, const int* stack_index, const double* wavelength, const double* theta_deg,
+                        const int* M, const int* pol, int64_t max_workspace_bytes, srcwa_plan** out) {
+    return guarded([&] {
+        if (!ctx || !out) invalid("srcwa_plan1d_create: null context or output");
+        if (n_stacks < 1 || n_problems < 1 || !stacks || !stack_offsets || !stack_index || !wavelength ||
+            !theta_deg || !M || !pol)
+            invalid("srcwa_plan1d_create: empty or null inputs");
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        // stack record extents
+        int64_t total_words = 0;
+        for (int s = 0; s < n_stacks; ++s) {
+            if (stack_offsets[s] < 0) invalid("rcwa: malformed stack offsets");
+            total_words = std::max<int64_t>(total_words, stack_offsets[s] + 6);
+        }
+        std::vector<int64_t> order(n_stacks);
+        for (int s = 0; s < n_stacks; ++s) order[s] = stack_offsets[s];
+        std::vector<int64_t> sorted = order;
+        std::sort(sorted.begin(), sorted.end());
+        auto record_end = [&](int64_t off) {
+            auto it = std::upper_bound(sorted.begin(), sorted.end(), off);
+            return it == sorted.end() ? int64_t(-1) : *it;
+        };
+        std::vector<std::vector<LayerInfo>> infos(n_stacks);
+        int64_t stack_words = 0;
+        for (int s = 0; s < n_stacks; ++s) {
+            const int64_t off = stack_offsets[s];
+            int64_t end = record_end(off);
+            // the last record: parse with a generous bound, then measure
+            const int64_t avail = end > 0 ? end - off : (int64_t(1) << 40);
+            infos[s] = parse_stack1d(stacks + off, avail, 0);
+            // record length
+            int64_t k = 6;
+            const double* p = stacks + off;
+            for (size_t j = 0; j < infos[s].size(); ++j) {
+                const int kind = static_cast<int>(p[k + 1]);
+                k += 2;
+                if (kind == 0) k += 2;
+                else {
+                    const int K = static_cast<int>(p[k]);
+                    k += 1 + (K + 1) + 2 * K + 2;
+                }
+            }
+            stack_words = std::max(stack_words, off + k);
+        }
+        std::vector<double> h_stacks(stacks, stacks + stack_words);
+
+        auto plan = std::unique_ptr<srcwa_plan>(new srcwa_plan);
+        plan->ctx = ctx;
+        plan->kind = 1;
+        plan->n_inst = n_problems;
+        const int64_t budget = max_workspace_bytes > 0 ? max_workspace_bytes : (int64_t(32) << 30);
+        const int64_t budget_words = budget / int64_t(sizeof(cd));
+
+        std::vector<Inst1D> h_insts(n_problems);
+        std::vector<LayerTask> h_tasks;
+        std::vector<int64_t> h_layouts;
+        int64_t orders = 0, chunk_ws = 0, max_ws = 0;
+        PlanChunk cur{0, 0, 0, 0};
+        for (int i = 0; i < n_problems; ++i) {
+            const int s = stack_index[i];
+            if (s < 0 || s >= n_stacks) invalid("rcwa: stack index out of range");
+            const double* st = stacks + stack_offsets[s];
+            check_setup(st[0], wavelength[i], st[1], st[2], theta_deg[i], M[i], pol[i]);
+            check_index(st[3], st[4]);
+            const std::vector<LayerInfo>& li = infos[s];
+            const int L = static_cast<int>(li.size());
+            const int N = 2 * M[i] + 1;
+            if (N > EIG_MAX_N_HOST) throw Failure{SRCWA_EUNSUPPORTED, "rcwa: truncation order above this build's limit"};
+            std::vector<bool> general(L + 2, false);
+            for (int j = 0; j < L; ++j) general[j + 1] = li[j].grating;
+            const PlanarCost cost = planar_cost(N, general, pol[i]);
+            plan->flops[0] += cost.flops_eig;
+            plan->flops[1] += cost.flops_other;
+            // layout (offsets relative to the instance; rebased onto the chunk below)
+            int64_t w = 0;
+            std::vector<int64_t> lay(layout_size(L), -1);
+            lay[LAYOUT_KX] = w;
+            w += align16(N);
+            for (int r = 0; r < L + 2; ++r) {
+                lay[layout_q(r)] = w;
+                w += align16(N);
+                lay[layout_p(r)] = w;
+                w += align16(N);
+            }
+            for (int j = 0; j < L; ++j)
+                if (li[j].grating) {
+                    lay[layout_layer(L, j)] = w;
+                    w += align16(slot_words(N));
+                }
+            const int64_t ws_lo = w;
+            w += cost.bump_words;
+            // admission into the current chunk by workspace footprint
+            if (i > cur.i0 && chunk_ws + w > budget_words) {
+                cur.i1 = i;
+                cur.t1 = static_cast<int>(h_tasks.size());
+                plan->chunks.push_back(cur);
+                max_ws = std::max(max_ws, chunk_ws);
+                cur = PlanChunk{i, i, cur.t1, cur.t1};
+                chunk_ws = 0;
+            }
+            const int first_task = static_cast<int>(h_tasks.size());
+            for (int j = 0; j < L; ++j)
+                if (li[j].grating) h_tasks.push_back(LayerTask{i - cur.i0, j});
+            const int64_t layout_off = static_cast<int64_t>(h_layouts.size());
+            for (int64_t v : lay) h_layouts.push_back(v >= 0 ? v + chunk_ws : -1);
+            Inst1D& in = h_insts[i];
+            in.M = M[i];
+            in.N = N;
+            in.pol = pol[i];
+            in.L = L;
+            in.first_task = first_task - cur.t0;
+            in.n_tasks = static_cast<int>(h_tasks.size()) - first_task;
+            in.wavelength = wavelength[i];
+            in.theta_deg = theta_deg[i];
+            in.phi_deg = 0.0;
+            in.stack_off = stack_offsets[s];
+            in.layout_off = layout_off;
+            in.out_off = orders;
+            in.ws_lo = chunk_ws + ws_lo;
+            in.ws_hi = chunk_ws + w;
+            chunk_ws += w;
+            orders += N;
+        }
+        cur.i1 = n_problems;
+        cur.t1 = static_cast<int>(h_tasks.size());
+        plan->chunks.push_back(cur);
+        max_ws = std::max(max_ws, chunk_ws);
+        plan->n_tasks = static_cast<int>(h_tasks.size());
+        plan->total_orders = orders;
+
+        hipStream_t st = ctx->stream;
+        plan->insts.upload(h_insts, st);
+        plan->tasks.upload(h_tasks, st);
+        plan->stacks.upload(h_stacks, st);
+        plan->layouts.upload(h_layouts, st);
+        plan->ws.alloc(static_cast<size_t>(max_ws));
+        plan->task_info.alloc(std::max(1, plan->n_tasks));
+        plan->R.alloc(2 * orders);
+        plan->T.alloc(2 * orders);
+        plan->DEr.alloc(orders);
+        plan->DEt.alloc(orders);
+        plan->status.alloc(n_problems);
+        plan->eig_info.alloc(n_problems);
+        for (hipEvent_t& e : plan->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        *out = plan.release();
+    });
+}
+
+int srcwa_plan_execute(srcwa_plan* plan) {
+    return guarded([&] {
+        if (!plan) invalid("null plan");
+        hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+        hipStream_t s = plan->ctx->stream;
+        const bool single = plan->chunks.size() == 1;
+        for (size_t c = 0; c < plan->chunks.size(); ++c) {
+            const PlanChunk& ch = plan->chunks[c];
+            Out1D out{plan->R.p, plan->T.p, plan->DEr.p, plan->DEt.p, plan->status.p + ch.i0, plan->eig_info.p + ch.i0};
+            hipEvent_t* ev = single ? plan->ev : nullptr;
+            if (!single && c == 0) hip_check(hipEventRecord(plan->ev[0], s), "hipEventRecord");
+            launch_planar(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p,
+                          plan->layouts.p, plan->ws.p, plan->task_info.p + ch.t0, out, 30, ev);
+            hip_check(hipGetLastError(), "kernel launch");
+        }
+        if (!single) {
+            hip_check(hipEventRecord(plan->ev[1], s), "hipEventRecord");
+            hip_check(hipEventRecord(plan->ev[2], s), "hipEventRecord");
+            hip_check(hipEventRecord(plan->ev[3], s), "hipEventRecord");
+        }
+        plan->timed = true;
+    });
+}
+
+int srcwa_plan_sync(srcwa_plan* plan) {
+    return guarded([&] {
+        if (!plan) invalid("null plan");
+        hip_check(hipStreamSynchronize(plan->ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int srcwa_plan_results(srcwa_plan* plan, double* R, double* T, double* DEr, double* DEt, int* status, int* eig_info) {
+    return guarded([&] {
+        if (!plan) invalid("null plan");
+        hipStream_t s = plan->ctx->stream;
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        const int64_t o = plan->total_orders;
+        if (R) hip_check(hipMemcpy(R, plan->R.p, 2 * o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+        if (T) hip_check(hipMemcpy(T, plan->T.p, 2 * o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+        if (DEr) hip_check(hipMemcpy(DEr, plan->DEr.p, o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+        if (DEt) hip_check(hipMemcpy(DEt, plan->DEt.p, o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+        if (status) hip_check(hipMemcpy(status, plan->status.p, plan->n_inst * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+        if (eig_info)
+            hip_check(hipMemcpy(eig_info, plan->eig_info.p, plan->n_inst * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+    });
+}
+
+int64_t srcwa_plan_orders(const srcwa_plan* plan) { return plan ? plan->total_orders : -1; }
+int64_t srcwa_plan_device_bytes(const srcwa_plan* plan) { return plan ? plan->bytes() : -1; }
+int srcwa_plan_chunks(const srcwa_plan* plan) { return plan ? static_cast<int>(plan->chunks.size()) : -1; }
+
+int srcwa_plan_stage_ms(const srcwa_plan* plan, double* ms4) {
+    return guarded([&] {
+        if (!plan || !ms4) invalid("null plan or output");
+        if (!plan->timed) invalid("plan has not been executed");
+        hip_check(hipEventSynchronize(plan->ev[3]), "hipEventSynchronize");
+        float a = 0, b = 0, c = 0, t = 0;
+        hip_check(hipEventElapsedTime(&a, plan->ev[0], plan->ev[1]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&b, plan->ev[1], plan->ev[2]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&c, plan->ev[2], plan->ev[3]), "hipEventElapsedTime");
+        hip_check(hipEventElapsedTime(&t, plan->ev[0], plan->ev[3]), "hipEventElapsedTime");
+        ms4[0] = a;
+        ms4[1] = b;
+        ms4[2] = c;
+        ms4[3] = t;
+    });
+}
+
+int srcwa_plan_flops(const srcwa_plan* plan, double* flops2) {
+    return guarded([&] {
+        if (!plan || !flops2) invalid("null plan or output");
+        flops2[0] = plan->flops[0];
+        flops2[1] = plan->flops[1];
+    });
+}
+
+void srcwa_plan_destroy(srcwa_plan* plan) {
+    if (!plan) return;
+    (void)hipSetDevice(plan->ctx->device);
+    (void)hipStreamSynchronize(plan->ctx->stream);
+    delete plan;
+}
+
+int srcwa_solve1d_batch(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                        int n_problems, const int* stack_index, const double* wavelength, const double* theta_deg,
+                        const int* M, const int* pol, double* R, double* T, double* DEr, double* DEt, int* status,
+                        int* eig_info) {
+    srcwa_plan* plan = nullptr;
+    int rc = srcwa_plan1d_create(ctx, n_stacks, stacks, stack_offsets, n_problems, stack_index, wavelength, theta_deg,
+                                 M, pol, 0, &plan);
+    if (rc != SRCWA_OK) return rc;
+    rc = srcwa_plan_execute(plan);
+    if (rc == SRCWA_OK) rc = srcwa_plan_results(plan, R, T, DEr, DEt, status, eig_info);
+    srcwa_plan_destroy(plan);
+    return rc;
+}
+
+int srcwa_zgemm_batched(srcwa_context* ctx, int count, const int* m, const int* n, const int* k, const double* A,
+                        const double* B, double* C) {
+    return dense_batch(ctx, [&](hipStream_t s) {
+        if (count < 1 || !m || !n || !k || !A || !B || !C) invalid("srcwa_zgemm_batched: empty or null inputs");
+        std::vector<DenseItem> items(count);
+        int64_t off = 0, ao = 0, bo = 0, co = 0;
+        for (int i = 0; i < count; ++i) {
+            if (m[i] < 0 || n[i] < 0 || k[i] < 0) invalid("srcwa_zgemm_batched: negative size");
+            DenseItem& it = items[i];
+            it.n = m[i];
+            it.nrhs = n[i];
+            it.k = k[i];
+            it.a = off;
+            off += align16(int64_t(m[i]) * k[i]);
+            it.b = off;
+            off += align16(int64_t(k[i]) * n[i]);
+            it.c = off;
+            off += align16(int64_t(m[i]) * n[i]);
+        }
+        DevBuf<cd> arena;
+        arena.alloc(off);
+        DevBuf<DenseItem> d_items;
+        d_items.upload(items, s);
+        for (int i = 0; i < count; ++i) {
+            const int64_t as = int64_t(m[i]) * k[i], bs = int64_t(k[i]) * n[i];
+            hip_check(hipMemcpyAsync(arena.p + items[i].a, A + 2 * ao, as * 16, hipMemcpyHostToDevice, s), "H2D");
+            hip_check(hipMemcpyAsync(arena.p + items[i].b, B + 2 * bo, bs * 16, hipMemcpyHostToDevice, s), "H2D");
+            ao += as;
+            bo += bs;
+        }
+        launch_zgemm_batched(s, count, d_items.p, arena.p);
+        hip_check(hipGetLastError(), "kernel launch");
+        for (int i = 0; i < count; ++i) {
+            const int64_t cs = int64_t(m[i]) * n[i];
+            hip_check(hipMemcpyAsync(C + 2 * co, arena.p + items[i].c, cs * 16, hipMemcpyDeviceToHost, s), "D2H");
+            co += cs;
+        }
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    });
+}
+
+int srcwa_zgesv_batched(srcwa_context* ctx, int count, const int* n, const int* nrhs, const double* A, const double* B,
+                        double* X, int* info) {
+    return dense_batch(ctx, [&](hipStream_t s) {
+        if (count < 1 || !n || !nrhs || !A || !B || !X || !info) invalid("srcwa_zgesv_batched: empty or null inputs");
+        std::vector<DenseItem> items(count);
+        int64_t off = 0;
+        for (int i = 0; i < count; ++i) {
+            if (n[i] < 1 || nrhs[i] < 0) invalid("srcwa_zgesv_batched: bad size");
+            DenseItem& it = items[i];
+            it.n = n[i];
+            it.nrhs = nrhs[i];
+            it.k = 0;
+            it.a = off;
+            off += align16(int64_t(n[i]) * n[i]);
+            it.b = off;
+            off += align16(int64_t(n[i]) * nrhs[i]);
+            it.scratch = off;
+            off += align16(n[i]);
+        }
+        DevBuf<cd> arena;
+        arena.alloc(off);
+        DevBuf<DenseItem> d_items;
+        d_items.upload(items, s);
+        DevBuf<int> d_info;
+        d_info.alloc(count);
+        int64_t ao = 0, bo = 0;
+        for (int i = 0; i < count; ++i) {
+            const int64_t as = int64_t(n[i]) * n[i], bs = int64_t(n[i]) * nrhs[i];
+            hip_check(hipMemcpyAsync(arena.p + items[i].a, A + 2 * ao, as * 16, hipMemcpyHostToDevice, s), "H2D");
+            hip_check(hipMemcpyAsync(arena.p + items[i].b, B + 2 * bo, bs * 16, hipMemcpyHostToDevice, s), "H2D");
+            ao += as;
+            bo += bs;
+        }
+        launch_zgesv_batched(s, count, d_items.p, arena.p, d_info.p);
+        hip_check(hipGetLastError(), "kernel launch");
+        bo = 0;
+        for (int i = 0; i < count; ++i) {
+            const int64_t bs = int64_t(n[i]) * nrhs[i];
+            hip_check(hipMemcpyAsync(X + 2 * bo, arena.p + items[i].b, bs * 16, hipMemcpyDeviceToHost, s), "D2H");
+            bo += bs;
+        }
+        hip_check(hipMemcpyAsync(info, d_info.p, count * sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    });
+}
+
+int srcwa_zgeev_batched(srcwa_context* ctx, int count, const int* n, const double* A, double* w, double* V, int* info) {
+    return dense_batch(ctx, [&](hipStream_t s) {
+        if (count < 1 || !n || !A || !w || !V || !info) invalid("srcwa_zgeev_batched: empty or null inputs");
+        std::vector<DenseItem> items(count);
+        int64_t off = 0;
+        for (int i = 0; i < count; ++i) {
+            if (n[i] < 1 || n[i] > EIG_MAX_N_HOST) invalid("srcwa_zgeev_batched: order out of range");
+            DenseItem& it = items[i];
+            const int64_t nn = int64_t(n[i]) * n[i];
+            it.n = n[i];
+            it.nrhs = 0;
+            it.k = 0;
+            it.a = off;
+            off += align16(nn);
+            it.w = off;
+            off += align16(n[i]);
+            it.scratch = off;
+            off += align16(2 * nn + 2 * n[i]);
+        }
+        DevBuf<cd> arena;
+        arena.alloc(off);
+        DevBuf<DenseItem> d_items;
+        d_items.upload(items, s);
+        DevBuf<int> d_info;
+        d_info.alloc(count);
+        int64_t ao = 0;
+        for (int i = 0; i < count; ++i) {
+            const int64_t as = int64_t(n[i]) * n[i];
+            hip_check(hipMemcpyAsync(arena.p + items[i].a, A + 2 * ao, as * 16, hipMemcpyHostToDevice, s), "H2D");
+            ao += as;
+        }
+        launch_zgeev_batched(s, count, d_items.p, arena.p, d_info.p, 30);
+        hip_check(hipGetLastError(), "kernel launch");
+        ao = 0;
+        int64_t wo = 0;
+        for (int i = 0; i < count; ++i) {
+            const int64_t as = int64_t(n[i]) * n[i];
+            hip_check(hipMemcpyAsync(V + 2 * ao, arena.p + items[i].a, as * 16, hipMemcpyDeviceToHost, s), "D2H");
+            hip_check(hipMemcpyAsync(w + 2 * wo, arena.p + items[i].w, n[i] * 16, hipMemcpyDeviceToHost, s), "D2H");
+            ao += as;
+            wo += n[i];
+        }
+        hip_check(hipMemcpyAsync(info, d_info.p, count * sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+    });
+}
+
+}  // extern "C"

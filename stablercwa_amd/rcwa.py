@@ -1,0 +1,195 @@
+"""Python mirror of the reference's solver entry points, backed by libsrcwa_amd.so on the GPU.
+
+    from stablercwa_amd import rcwa
+    r = rcwa.solve(stack, 1.053, 66.36, 20, rcwa.Polarization.TE)      # rcwa1d.hpp:1077
+    rs = rcwa.sweep(stack, [rcwa.Point(l, 66.36) for l in lams], 20)  # rcwa1d.hpp:1084
+
+``solve``/``sweep`` keep the reference's names, argument meaning and error behaviour: invalid input
+raises ``ValueError`` (the reference's std::invalid_argument), a numerical failure returns NaN arrays
+with ``eig_info`` set.  ``PlanarBatch`` is the batched, device-resident form used by the benchmark and
+the request server: inputs are uploaded once, ``execute`` re-runs every solve on the GPU.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import List, Sequence
+
+import ctypes
+import numpy as np
+
+from . import _lib
+from .types import (ConicalResult, FourierMethod, Jones, Layer, Layer2D, Polarization, Result, Result2D,
+                    Stack, Stack2D, Status2D, XProfile2D, jones_from_angles_2d)
+
+__all__ = ["solve", "sweep", "Point", "PlanarBatch", "Layer", "Stack", "Polarization", "FourierMethod",
+           "Result", "zgeev_batched", "zgesv_batched", "zgemm_batched"]
+
+
+@dataclass
+class Point:
+    wavelength: float = 1.0
+    theta_deg: float = 0.0
+
+
+def _pack_stacks(stacks: Sequence[Stack]):
+    recs = [s.flat() for s in stacks]
+    offsets = np.zeros(len(recs), dtype=np.int64)
+    pos = 0
+    for i, r in enumerate(recs):
+        offsets[i] = pos
+        pos += r.size
+    return np.concatenate(recs).astype(np.float64), offsets
+
+
+class PlanarBatch:
+    """A device-resident batch of planar solves (one launch plan).
+
+    problems: sequence of (stack_index, wavelength, theta_deg, M, pol)."""
+
+    def __init__(self, stacks: Sequence[Stack], problems, ctx: _lib.Context | None = None,
+                 max_workspace_bytes: int = 0):
+        self.ctx = ctx or _lib.default_context()
+        self.lib = _lib.load()
+        self._flat, self._offsets = _pack_stacks(stacks)
+        P = len(problems)
+        self.stack_index = np.array([p[0] for p in problems], dtype=np.int32)
+        self.wavelength = np.array([p[1] for p in problems], dtype=np.float64)
+        self.theta = np.array([p[2] for p in problems], dtype=np.float64)
+        self.M = np.array([p[3] for p in problems], dtype=np.int32)
+        self.pol = np.array([int(p[4]) for p in problems], dtype=np.int32)
+        self.N = 2 * self.M + 1
+        self.order_offsets = np.concatenate([[0], np.cumsum(self.N)]).astype(np.int64)
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.srcwa_plan1d_create(
+            self.ctx.handle, len(stacks), _lib.ptr(self._flat), _lib.ptr(self._offsets), P,
+            _lib.ptr(self.stack_index), _lib.ptr(self.wavelength), _lib.ptr(self.theta), _lib.ptr(self.M),
+            _lib.ptr(self.pol), int(max_workspace_bytes), ctypes.byref(h)))
+        self.handle = h
+        self.n = P
+
+    def execute(self) -> None:
+        _lib.check(self.lib.srcwa_plan_execute(self.handle))
+
+    def sync(self) -> None:
+        _lib.check(self.lib.srcwa_plan_sync(self.handle))
+
+    def results(self):
+        o = int(self.order_offsets[-1])
+        R = np.empty(2 * o); T = np.empty(2 * o); DEr = np.empty(o); DEt = np.empty(o)
+        st = np.empty(self.n, dtype=np.int32); info = np.empty(self.n, dtype=np.int32)
+        _lib.check(self.lib.srcwa_plan_results(self.handle, _lib.ptr(R), _lib.ptr(T), _lib.ptr(DEr), _lib.ptr(DEt),
+                                               _lib.ptr(st), _lib.ptr(info)))
+        return dict(R=R.view(np.complex128), T=T.view(np.complex128), DEr=DEr, DEt=DEt, status=st, eig_info=info)
+
+    def result(self, i: int, res=None) -> Result:
+        res = res or self.results()
+        a, b = int(self.order_offsets[i]), int(self.order_offsets[i + 1])
+        M = int(self.M[i])
+        return Result(orders=np.arange(-M, M + 1), R=res["R"][a:b].copy(), T=res["T"][a:b].copy(),
+                      DEr=res["DEr"][a:b].copy(), DEt=res["DEt"][a:b].copy(), eig_info=int(res["eig_info"][i]))
+
+    def stage_ms(self) -> np.ndarray:
+        ms = np.zeros(4)
+        _lib.check(self.lib.srcwa_plan_stage_ms(self.handle, _lib.ptr(ms)))
+        return ms
+
+    def flops(self) -> np.ndarray:
+        f = np.zeros(2)
+        _lib.check(self.lib.srcwa_plan_flops(self.handle, _lib.ptr(f)))
+        return f
+
+    def device_bytes(self) -> int:
+        return int(self.lib.srcwa_plan_device_bytes(self.handle))
+
+    def chunks(self) -> int:
+        return int(self.lib.srcwa_plan_chunks(self.handle))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self.lib.srcwa_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve(stack: Stack, wavelength: float, theta_deg: float, M: int, pol: Polarization = Polarization.TE,
+          ctx: _lib.Context | None = None) -> Result:
+    """One planar solve (rcwa::solve, rcwa1d.hpp:1077)."""
+    b = PlanarBatch([stack], [(0, wavelength, theta_deg, M, pol)], ctx)
+    b.execute()
+    r = b.result(0)
+    b.close()
+    return r
+
+
+def sweep(stack: Stack, points: Sequence[Point], M: int, pol: Polarization = Polarization.TE,
+          ctx: _lib.Context | None = None) -> List[Result]:
+    """Independent solves over points, one batched launch (rcwa::sweep, rcwa1d.hpp:1084-1091)."""
+    b = PlanarBatch([stack], [(0, p.wavelength, p.theta_deg, M, pol) for p in points], ctx)
+    b.execute()
+    res = b.results()
+    out = [b.result(i, res) for i in range(len(points))]
+    b.close()
+    return out
+
+
+# ---- hook-shaped dense batches ---------------------------------------------------------------------
+def _pack(mats):
+    return np.concatenate([np.asfortranarray(m, dtype=np.complex128).ravel(order="F") for m in mats]).view(np.float64)
+
+
+def zgeev_batched(mats, ctx: _lib.Context | None = None):
+    """Eigenpairs of each square matrix (custom_eig shape): returns [(w, V, info)]."""
+    ctx = ctx or _lib.default_context()
+    n = np.array([m.shape[0] for m in mats], dtype=np.int32)
+    A = _pack(mats)
+    tot = int(np.sum(n.astype(np.int64) ** 2))
+    V = np.zeros(2 * tot); w = np.zeros(2 * int(n.sum())); info = np.zeros(len(mats), dtype=np.int32)
+    _lib.check(_lib.load().srcwa_zgeev_batched(ctx.handle, len(mats), _lib.ptr(n), _lib.ptr(A), _lib.ptr(w),
+                                               _lib.ptr(V), _lib.ptr(info)))
+    Vc, wc = V.view(np.complex128), w.view(np.complex128)
+    out, a, b = [], 0, 0
+    for i, k in enumerate(n):
+        out.append((wc[b:b + k].copy(), Vc[a:a + k * k].reshape((k, k), order="F").copy(), int(info[i])))
+        a += k * k; b += k
+    return out
+
+
+def zgesv_batched(As, Bs, ctx: _lib.Context | None = None):
+    """X_i = A_i^{-1} B_i (custom_solve shape): returns [(X, info)]."""
+    ctx = ctx or _lib.default_context()
+    n = np.array([a.shape[0] for a in As], dtype=np.int32)
+    nrhs = np.array([b.shape[1] for b in Bs], dtype=np.int32)
+    A, B = _pack(As), _pack(Bs)
+    X = np.zeros_like(B); info = np.zeros(len(As), dtype=np.int32)
+    _lib.check(_lib.load().srcwa_zgesv_batched(ctx.handle, len(As), _lib.ptr(n), _lib.ptr(nrhs), _lib.ptr(A),
+                                               _lib.ptr(B), _lib.ptr(X), _lib.ptr(info)))
+    Xc = X.view(np.complex128)
+    out, a = [], 0
+    for i in range(len(As)):
+        k, r = int(n[i]), int(nrhs[i])
+        out.append((Xc[a:a + k * r].reshape((k, r), order="F").copy(), int(info[i])))
+        a += k * r
+    return out
+
+
+def zgemm_batched(As, Bs, ctx: _lib.Context | None = None):
+    """C_i = A_i B_i (custom_multiply shape)."""
+    ctx = ctx or _lib.default_context()
+    m = np.array([a.shape[0] for a in As], dtype=np.int32)
+    k = np.array([a.shape[1] for a in As], dtype=np.int32)
+    n = np.array([b.shape[1] for b in Bs], dtype=np.int32)
+    A, B = _pack(As), _pack(Bs)
+    C = np.zeros(2 * int(np.sum(m.astype(np.int64) * n)))
+    _lib.check(_lib.load().srcwa_zgemm_batched(ctx.handle, len(As), _lib.ptr(m), _lib.ptr(n), _lib.ptr(k),
+                                               _lib.ptr(A), _lib.ptr(B), _lib.ptr(C)))
+    Cc = C.view(np.complex128)
+    out, a = [], 0
+    for i in range(len(As)):
+        out.append(Cc[a:a + m[i] * n[i]].reshape((m[i], n[i]), order="F").copy())
+        a += int(m[i] * n[i])
+    return out

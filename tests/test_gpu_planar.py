@@ -1,0 +1,180 @@
+"""GPU parity of the planar (TE/TM) solve through the C-ABI.
+
+* the reference's own 23 efficiency tables at 1e-9 (tests/golden/planar_tables.json)
+* vectors of the reference's compiled solver at the benchmark structures at 1e-10, R+T = 1 at 1e-12
+* the oracle on seeded random stacks at 1e-10
+* a full-size 4096-wavelength TE+TM sweep of the grating on a 20-layer mirror: R+T = 1 for every
+  solve (lossless), oracle parity on a sample
+* each result is bitwise the same alone and inside a mixed batch
+* error conventions: invalid input raises ValueError; FFT layers are rejected explicitly
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden, max_abs_diff
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(cases, ctx):
+    from stablercwa_amd import rcwa
+    from stablercwa_amd.types import Stack
+    stacks, probs = [], []
+    for i, c in enumerate(cases):
+        stacks.append(c["stack"])
+        probs.append((i, c["wavelength"], c["theta"], c["M"], c["pol"]))
+    b = _RawBatch(stacks, probs, ctx)
+    b.execute()
+    return b, b.results()
+
+
+class _RawBatch:
+    """PlanarBatch over flat stack records (the fixtures store records, not Stack objects)."""
+
+    def __new__(cls, flats, probs, ctx):
+        from stablercwa_amd import rcwa
+
+        class _S:
+            def __init__(self, f):
+                self.f = np.asarray(f, dtype=np.float64)
+
+            def flat(self):
+                return self.f
+
+        return rcwa.PlanarBatch([_S(f) for f in flats], probs, ctx)
+
+
+def test_reference_tables(ctx):
+    cases = load_golden("planar_tables.json")["cases"]
+    b, res = _batch(cases, ctx)
+    for i, c in enumerate(cases):
+        r = b.result(i, res)
+        assert res["status"][i] == 0, c["name"]
+        assert max_abs_diff(r.DEr, c["DEr"]) <= 1e-9, c["name"]
+        assert max_abs_diff(r.DEt, c["DEt"]) <= 1e-9, c["name"]
+
+
+def test_reference_build_vectors(ctx):
+    cases = load_golden("planar_ref.json")["cases"]
+    b, res = _batch(cases, ctx)
+    for i, c in enumerate(cases):
+        r = b.result(i, res)
+        assert res["status"][i] == 0, c["name"]
+        assert max_abs_diff(r.DEr, c["DEr"]) <= 1e-10, c["name"]
+        assert max_abs_diff(r.DEt, c["DEt"]) <= 1e-10, c["name"]
+        if c["name"].startswith(("readme", "mirror")):
+            assert abs(r.DEr.sum() + r.DEt.sum() - 1.0) <= 1e-12, c["name"]
+
+
+def test_readme_values(ctx):
+    """README.md:106-111 of the reference: m=0 R 0.026034 T 0.497012; m=+1 R 0.258135 T 0.218818."""
+    from stablercwa_amd import rcwa
+    from stablercwa_amd.types import Layer, Stack
+    st = Stack(1.0, [Layer.grating(0.38, [0, 0.172, 0.574], [1.46, 1.0]), Layer.uniform(0.115, 1.96)], 1.46, 0.574)
+    r = rcwa.solve(st, 1.053, 66.36, 20, rcwa.Polarization.TE, ctx)
+    assert abs(r.DEr[20] - 0.026034) < 5e-7 and abs(r.DEt[20] - 0.497012) < 5e-7
+    assert abs(r.DEr[21] - 0.258135) < 5e-7 and abs(r.DEt[21] - 0.218818) < 5e-7
+    assert abs(r.reflectance() + r.transmittance() - 1.0) <= 1e-12
+
+
+def _mirror_stack():
+    from stablercwa_amd.types import Layer, Stack
+    layers = [Layer.grating(0.38, [0.0, 0.3 * 0.574, 0.574], [1.46, 1.0])]
+    for k in range(20):
+        n = 2.1 if k % 2 == 0 else 1.46
+        layers.append(Layer.uniform(1.053 / (4 * n), n))
+    return Stack(1.0, layers, 1.46, 0.574)
+
+
+def test_full_size_sweep_energy_and_sample_parity(ctx):
+    """4096 wavelengths x TE/TM at M = 60 (N = 121): energy conservation on every solve and
+    oracle parity on a sample."""
+    from oracle import rcwa1d
+    from stablercwa_amd import rcwa
+    st = _mirror_stack()
+    lams = np.linspace(0.95, 1.15, 4096)
+    probs = [(0, float(l), 66.36, 60, pol) for pol in (0, 1) for l in lams]
+    b = rcwa.PlanarBatch([st], probs, ctx)
+    b.execute()
+    res = b.results()
+    assert np.all(res["status"] == 0)
+    N = 121
+    der = res["DEr"].reshape(-1, N)
+    det = res["DEt"].reshape(-1, N)
+    rt = der.sum(axis=1) + det.sum(axis=1)
+    assert np.max(np.abs(rt - 1.0)) <= 1e-12, np.max(np.abs(rt - 1.0))
+    flat = st.flat()
+    for idx in (0, 1111, 4095, 4096, 6000, 8191):
+        pol = probs[idx][4]
+        o = rcwa1d.solve(flat, probs[idx][1], 66.36, 60, pol)
+        assert max_abs_diff(der[idx], o["DEr"]) <= 1e-10
+        assert max_abs_diff(det[idx], o["DEt"]) <= 1e-10
+
+
+def test_random_stacks_vs_oracle(ctx):
+    from oracle import rcwa1d
+    from stablercwa_amd import rcwa
+    from stablercwa_amd.types import Layer, Stack
+    rng = np.random.default_rng(77)
+    stacks, probs = [], []
+    for i in range(24):
+        period = float(rng.uniform(0.3, 2.0))
+        layers = []
+        for _ in range(int(rng.integers(1, 6))):
+            if rng.random() < 0.4:
+                layers.append(Layer.uniform(float(rng.uniform(0, 0.6)), complex(rng.uniform(1.0, 3.0), rng.uniform(0, 0.1))))
+            else:
+                K = int(rng.integers(2, 6))
+                e = np.sort(rng.uniform(0, period, K - 1))
+                edges = [0.0] + [float(v) for v in e] + [period]
+                layers.append(Layer.grating(float(rng.uniform(0.01, 0.7)), edges,
+                                            [complex(rng.uniform(1.0, 3.0), rng.uniform(0, 0.2)) for _ in range(K)]))
+        stacks.append(Stack(float(rng.uniform(1.0, 1.5)), layers, complex(rng.uniform(1.0, 2.5), rng.uniform(0, 0.05)), period))
+        probs.append((i, float(rng.uniform(0.4, 2.0)), float(rng.uniform(-80, 80)), int(rng.integers(0, 30)),
+                      int(rng.integers(0, 2))))
+    b = rcwa.PlanarBatch(stacks, probs, ctx)
+    b.execute()
+    res = b.results()
+    for i, p in enumerate(probs):
+        o = rcwa1d.solve(stacks[i].flat(), p[1], p[2], p[3], p[4])
+        r = b.result(i, res)
+        assert res["status"][i] == 0
+        assert max_abs_diff(r.DEr, o["DEr"]) <= 1e-10, i
+        assert max_abs_diff(r.DEt, o["DEt"]) <= 1e-10, i
+
+
+def test_alone_equals_in_mix(ctx):
+    from stablercwa_amd import rcwa
+    cases = load_golden("planar_ref.json")["cases"]
+    b, mixed = _batch(cases, ctx)
+    for i in (0, 3, 9, len(cases) - 1):
+        b1, alone = _batch([cases[i]], ctx)
+        r_mix, r_alone = b.result(i, mixed), b1.result(0, alone)
+        assert np.array_equal(r_mix.DEr, r_alone.DEr) and np.array_equal(r_mix.DEt, r_alone.DEt)
+        assert np.array_equal(r_mix.R, r_alone.R) and np.array_equal(r_mix.T, r_alone.T)
+
+
+def test_chunked_plan_equals_single_chunk(ctx):
+    """Admission by workspace footprint: a small budget splits the batch into chunks, same results."""
+    from stablercwa_amd import rcwa
+    st = _mirror_stack()
+    probs = [(0, float(l), 66.36, 30, pol) for pol in (0, 1) for l in np.linspace(1.0, 1.1, 40)]
+    b1 = rcwa.PlanarBatch([st], probs, ctx)
+    b2 = rcwa.PlanarBatch([st], probs, ctx, max_workspace_bytes=8 << 20)
+    assert b2.chunks() > 1
+    b1.execute(); b2.execute()
+    r1, r2 = b1.results(), b2.results()
+    assert np.array_equal(r1["DEr"], r2["DEr"]) and np.array_equal(r1["DEt"], r2["DEt"])
+
+
+def test_invalid_input_raises(ctx):
+    from stablercwa_amd import rcwa
+    from stablercwa_amd.types import Layer, Stack
+    st = Stack(1.0, [Layer.grating(0.3, [0, 0.2, 0.5], [1.5, 1.0])], 1.5, 0.6)   # edges do not span the period
+    with pytest.raises(ValueError):
+        rcwa.solve(st, 1.0, 10.0, 5, rcwa.Polarization.TE, ctx)
+    ok = Stack(1.0, [Layer.grating(0.3, [0, 0.2, 0.6], [1.5, 1.0])], 1.5, 0.6)
+    with pytest.raises(ValueError):
+        rcwa.solve(ok, 1.0, 90.0, 5, rcwa.Polarization.TE, ctx)
+    with pytest.raises(ValueError):
+        rcwa.solve(Stack(complex(1.0, 0.1), ok.layers, 1.5, 0.6), 1.0, 10.0, 5, rcwa.Polarization.TE, ctx)
