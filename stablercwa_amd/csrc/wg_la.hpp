@@ -96,7 +96,7 @@ DEV int block_argmax(double v, int idx) {
 // GEMM.  Each wave owns 32x32 output tiles (2x2 blocks of 16x16).  A complex product is four real
 // MFMAs per block: Cre += Ar*Br, Cre += (-Ai)*Bi, Cim += Ar*Bi, Cim += Ai*Br.
 // v_mfma_f64_16x16x4_f64 operand layout (per lane l): A[l%16][l/16], B[l/16][l%16];
-// result element r of lane l is D[4*(l/16) + r][l%16].
+// result element r of lane l is D[l/16 + 4*r][l%16] (checked on gfx950 against numpy).
 enum { GEMM_SET = 0, GEMM_SUB = 1, GEMM_ADD = 2 };
 
 template <int MODE, bool BT>
@@ -145,7 +145,7 @@ DEV void wg_gemm(int m, int n, int k, const cd* __restrict__ A, int lda, const c
                 if (col >= n) continue;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const int row = r0 + 16 * i + 4 * lk + q;
+                    const int row = r0 + 16 * i + lk + 4 * q;
                     if (row >= m) continue;
                     cd* p = C + row + (long)col * ldc;
                     const cd v{re[i][j][q], im[i][j][q]};

@@ -2,6 +2,7 @@
 
 * the reference's own 23 efficiency tables at 1e-9 (tests/golden/planar_tables.json)
 * vectors of the reference's compiled solver at the benchmark structures at 1e-10, R+T = 1 at 1e-12
+  (README structure) and 3e-12 (M = 60 mirror, where the reference itself reaches 1.06e-12)
 * the oracle on seeded random stacks at 1e-10
 * a full-size 4096-wavelength TE+TM sweep of the grating on a 20-layer mirror: R+T = 1 for every
   solve (lossless), oracle parity on a sample
@@ -62,8 +63,12 @@ def test_reference_build_vectors(ctx):
         assert res["status"][i] == 0, c["name"]
         assert max_abs_diff(r.DEr, c["DEr"]) <= 1e-10, c["name"]
         assert max_abs_diff(r.DEt, c["DEt"]) <= 1e-10, c["name"]
-        if c["name"].startswith(("readme", "mirror")):
+        # lossless stacks: the reference itself reaches |R+T-1| = 1.06e-12 on the M = 60 mirror
+        # (planar_ref.json, mirror_m60_te_1.1); the README structure stays at 1e-12.
+        if c["name"].startswith("readme"):
             assert abs(r.DEr.sum() + r.DEt.sum() - 1.0) <= 1e-12, c["name"]
+        if c["name"].startswith("mirror"):
+            assert abs(r.DEr.sum() + r.DEt.sum() - 1.0) <= 3e-12, c["name"]
 
 
 def test_readme_values(ctx):
@@ -102,7 +107,14 @@ def test_full_size_sweep_energy_and_sample_parity(ctx):
     der = res["DEr"].reshape(-1, N)
     det = res["DEt"].reshape(-1, N)
     rt = der.sum(axis=1) + det.sum(axis=1)
-    assert np.max(np.abs(rt - 1.0)) <= 1e-12, np.max(np.abs(rt - 1.0))
+    # lossless: |R+T-1| is rounding only.  Held to the reference solver's own defect on this sweep
+    # (planar_energy_ref.json: max 3.7e-12, median 5.3e-13 over 1024 wavelengths per polarization).
+    e = np.abs(rt - 1.0)
+    ref = load_golden("planar_energy_ref.json")
+    ref_max = max(ref["TE"]["max"], ref["TM"]["max"])
+    ref_med = max(ref["TE"]["median"], ref["TM"]["median"])
+    assert np.max(e) <= 1.5 * ref_max, (np.max(e), ref_max)
+    assert np.median(e) <= 2.0 * ref_med, (np.median(e), ref_med)
     flat = st.flat()
     for idx in (0, 1111, 4095, 4096, 6000, 8191):
         pol = probs[idx][4]
