@@ -55,7 +55,7 @@ int srcwa_context_sync(srcwa_context* ctx);
  * Results of problem i start at order offset sum_{k<i} (2 M_k + 1).
  *
  * stacks: n_stacks flat records concatenated; stack_offsets[s] = start (in doubles) of record s.
- * max_workspace_bytes bounds the device workspace of one launch chunk (0 = 32 GiB); problems are
+ * max_workspace_bytes bounds the device workspace of one launch chunk (0 = 64 GiB); problems are
  * admitted into chunks in order until the bound is reached.
  */
 int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
@@ -73,7 +73,8 @@ int64_t srcwa_plan_orders(const srcwa_plan* plan);
 int64_t srcwa_plan_device_bytes(const srcwa_plan* plan);
 int srcwa_plan_chunks(const srcwa_plan* plan);
 /* Per-stage GPU milliseconds of the last execute, from HIP events on the plan's stream:
- * [0] setup + assembly, [1] eigensolves, [2] modes + S-matrix + efficiencies, [3] total. */
+ * [0] setup + assembly, [1] eigensolves, [2] modes + S-matrix + efficiencies (each summed over
+ * chunks), [3] wall time from the first chunk's start to the last chunk's end. */
 int srcwa_plan_stage_ms(const srcwa_plan* plan, double* ms4);
 /* Algorithmic FP64 flops of one execute of the plan, by the accounting convention
  * (complex GEMM m k n: 8mkn; LU n: 8n^3/3; triangular solves with k RHS: 8 n^2 k; eig with vectors: 100 n^3):

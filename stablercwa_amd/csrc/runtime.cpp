@@ -253,12 +253,16 @@ struct srcwa_plan {
     DevBuf<int> task_info;
     DevBuf<double> R, T, DEr, DEt;
     DevBuf<int> status, eig_info;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<hipEvent_t> ev;      // 4 per chunk: start, after assembly, after eigensolves, end
     bool timed = false;
     double flops[2] = {0.0, 0.0};
     ~srcwa_plan() {
         for (hipEvent_t& e : ev)
             if (e) (void)hipEventDestroy(e);
+    }
+    void make_events() {
+        ev.assign(4 * chunks.size(), nullptr);
+        for (hipEvent_t& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
     }
     int64_t bytes() const {
         return insts.bytes() + tasks.bytes() + stacks.bytes() + layouts.bytes() + ws.bytes() + task_info.bytes() +
@@ -371,7 +375,7 @@ int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, 
         plan->ctx = ctx;
         plan->kind = 1;
         plan->n_inst = n_problems;
-        const int64_t budget = max_workspace_bytes > 0 ? max_workspace_bytes : (int64_t(32) << 30);
+        const int64_t budget = max_workspace_bytes > 0 ? max_workspace_bytes : (int64_t(64) << 30);
         const int64_t budget_words = budget / int64_t(sizeof(cd));
 
         std::vector<Inst1D> h_insts(n_problems);
@@ -464,7 +468,7 @@ int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, 
         plan->DEt.alloc(orders);
         plan->status.alloc(n_problems);
         plan->eig_info.alloc(n_problems);
-        for (hipEvent_t& e : plan->ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        plan->make_events();
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
         *out = plan.release();
     });
@@ -475,20 +479,12 @@ int srcwa_plan_execute(srcwa_plan* plan) {
         if (!plan) invalid("null plan");
         hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
         hipStream_t s = plan->ctx->stream;
-        const bool single = plan->chunks.size() == 1;
         for (size_t c = 0; c < plan->chunks.size(); ++c) {
             const PlanChunk& ch = plan->chunks[c];
             Out1D out{plan->R.p, plan->T.p, plan->DEr.p, plan->DEt.p, plan->status.p + ch.i0, plan->eig_info.p + ch.i0};
-            hipEvent_t* ev = single ? plan->ev : nullptr;
-            if (!single && c == 0) hip_check(hipEventRecord(plan->ev[0], s), "hipEventRecord");
             launch_planar(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p,
-                          plan->layouts.p, plan->ws.p, plan->task_info.p + ch.t0, out, 30, ev);
+                          plan->layouts.p, plan->ws.p, plan->task_info.p + ch.t0, out, 30, plan->ev.data() + 4 * c);
             hip_check(hipGetLastError(), "kernel launch");
-        }
-        if (!single) {
-            hip_check(hipEventRecord(plan->ev[1], s), "hipEventRecord");
-            hip_check(hipEventRecord(plan->ev[2], s), "hipEventRecord");
-            hip_check(hipEventRecord(plan->ev[3], s), "hipEventRecord");
         }
         plan->timed = true;
     });
@@ -525,12 +521,21 @@ int srcwa_plan_stage_ms(const srcwa_plan* plan, double* ms4) {
     return guarded([&] {
         if (!plan || !ms4) invalid("null plan or output");
         if (!plan->timed) invalid("plan has not been executed");
-        hip_check(hipEventSynchronize(plan->ev[3]), "hipEventSynchronize");
-        float a = 0, b = 0, c = 0, t = 0;
-        hip_check(hipEventElapsedTime(&a, plan->ev[0], plan->ev[1]), "hipEventElapsedTime");
-        hip_check(hipEventElapsedTime(&b, plan->ev[1], plan->ev[2]), "hipEventElapsedTime");
-        hip_check(hipEventElapsedTime(&c, plan->ev[2], plan->ev[3]), "hipEventElapsedTime");
-        hip_check(hipEventElapsedTime(&t, plan->ev[0], plan->ev[3]), "hipEventElapsedTime");
+        hip_check(hipEventSynchronize(plan->ev.back()), "hipEventSynchronize");
+        double a = 0, b = 0, c = 0, t = 0;
+        for (size_t k = 0; k < plan->chunks.size(); ++k) {
+            const hipEvent_t* e = plan->ev.data() + 4 * k;
+            float x = 0, y = 0, z = 0;
+            hip_check(hipEventElapsedTime(&x, e[0], e[1]), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&y, e[1], e[2]), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&z, e[2], e[3]), "hipEventElapsedTime");
+            a += x;
+            b += y;
+            c += z;
+        }
+        float tt = 0;
+        hip_check(hipEventElapsedTime(&tt, plan->ev.front(), plan->ev.back()), "hipEventElapsedTime");
+        t = tt;
         ms4[0] = a;
         ms4[1] = b;
         ms4[2] = c;
