@@ -18,7 +18,8 @@
  *         patterned: Ky, y_edges[Ky+1], then per y segment Kx, x_edges[Kx+1], (n.re, n.im)[Kx]
  *
  * Status words per solve: 0 ok, 1 numerical failure (results NaN; the reference's NumericalFailure,
- * rcwa1d.hpp:37-38), 3 Rayleigh anomaly (crossed gratings, rcwa2d_types.hpp:32-37).
+ * rcwa1d.hpp:37-38), 3 Rayleigh anomaly (crossed gratings, rcwa2d_types.hpp:32-37), 4 conical layer
+ * needing Schur-basis propagation.
  */
 #ifndef SRCWA_H
 #define SRCWA_H
@@ -61,6 +62,16 @@ int srcwa_context_sync(srcwa_context* ctx);
 int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
                         int n_problems, const int* stack_index, const double* wavelength, const double* theta_deg,
                         const int* M, const int* pol, int64_t max_workspace_bytes, srcwa_plan** out);
+
+/* Batched conical solves.  Replaces rcwa::solve_conical / sweep_conical (rcwa1d_conical.hpp:985-999).
+ * Per problem N = 2M+1 orders; R and T receive the Jones matrices Jr, Jt as N x 4 complex
+ * (ss, sp, ps, pp per order) and DEr, DEt the N x 4 channel efficiencies flux * |J|^2
+ * (ConicalResult, rcwa1d_conical.hpp:105-183).  Result offsets are 4 * the planar order offsets.
+ * Status 4 marks a degenerate layer that needs the reference's Schur-basis propagation. */
+int srcwa_plan_conical_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                              int n_problems, const int* stack_index, const double* wavelength,
+                              const double* theta_deg, const double* phi_deg, const int* M,
+                              int64_t max_workspace_bytes, srcwa_plan** out);
 
 /* Launch all chunks of the plan on the context stream (asynchronous). */
 int srcwa_plan_execute(srcwa_plan* plan);

@@ -12,6 +12,7 @@ enum : int {
     ST_NUMERICAL = 1,      // singular system, non-finite value or unconverged eigensolve: results are NaN
     ST_WORKSPACE = 2,      // internal: the instance's device workspace was too small (never expected)
     ST_RAYLEIGH = 3,       // crossed gratings: an order is exactly grazing in a homogeneous medium
+    ST_SCHUR = 4,          // conical: a degenerate layer needs Schur-basis propagation (not provided)
 };
 
 // One planar (TE/TM) or conical instance.
@@ -42,6 +43,19 @@ __host__ __device__ inline int layout_layer(int L, int j) { return 1 + 2 * (L + 
 __host__ __device__ inline int layout_size(int L) { return 1 + 2 * (L + 2) + L; }
 // slots of a grating layer of order n
 __host__ __device__ inline int64_t slot_words(int64_t n) { return 4 * n * n + 4 * n; }
+
+// Layout table of a conical instance (L = layers with d > 0):
+//   [0]                          geometry: kx, cos phi_m, sin phi_m, kzI, kzII, flux (N each), ky (1)
+//   [1 + 3r], [2 + 3r], [3 + 3r] q, w, v of region r = 0 .. L+1 (2N each; w, v = -1 for grating layers)
+//   [1 + 3(L+2) + j]             slot base of layer j (-1 for a uniform layer)
+//   [1 + 3(L+2) + L + j]         index of layer j in the stack record
+__host__ __device__ inline int conical_q(int r) { return 1 + 3 * r; }
+__host__ __device__ inline int conical_w(int r) { return 2 + 3 * r; }
+__host__ __device__ inline int conical_v(int r) { return 3 + 3 * r; }
+__host__ __device__ inline int conical_slot(int L, int j) { return 1 + 3 * (L + 2) + j; }
+__host__ __device__ inline int conical_orig(int L, int j) { return 1 + 3 * (L + 2) + L + j; }
+__host__ __device__ inline int conical_layout_size(int L) { return 1 + 3 * (L + 2) + 2 * L; }
+__host__ __device__ inline int64_t conical_slot_words(int64_t N) { return 16 * N * N + 7 * N; }
 
 // Per-instance results.
 struct Out1D {

@@ -1,0 +1,647 @@
+// Conical-incidence RCWA on gfx950 (the reference's solve_conical, rcwa1d_conical.hpp:678-982).
+//
+//   k_setup_conical     one workgroup per instance: kx_m, ky, the per-order azimuths (cos, sin),
+//                       kz and flux factors of the outer media, and the analytic (s, t)-basis modes
+//                       of homogeneous regions                   (rcwa1d_conical.hpp:195-283, 724-747)
+//   k_assemble_conical  one workgroup per grating layer: [[eps]]^-1, C = [[1/eps]]^-1, B C and
+//                       A = Kx^2 - [[eps]]                       (rcwa1d_conical.hpp:302-336)
+//   k_eig_conical       one workgroup per N x N eigensolve: A and B C of every grating layer
+//   k_modes_conical     one workgroup per grating layer: the degeneracy test, then the 2N modes
+//                       W, V in the (s, t) basis                 (rcwa1d_conical.hpp:341-411, 826-848)
+//   k_smatrix_conical   one workgroup per instance: the recursion at order 2N with the s and p
+//                       incident columns, Jones matrices and channel efficiencies
+//                                                                (rcwa1d_conical.hpp:852-956)
+// Zero-thickness layers do not take part (the host lists only layers with d > 0).  A layer whose
+// reduced eigenvalues come within 1e-2 |ky|^1.5 of zero takes the reference's Schur-basis
+// propagation (rcwa1d_conical.hpp:413-502): Schur form of the 2N-order P Q, triangular square
+// root, V = Q Z G^-1, and a dense exp(-k0 d G) propagation block.
+#include "batch.hpp"
+#include "eig.hpp"
+#include "rcwa_common.hpp"
+
+// Conical layout table (int64 cd offsets), see conical_layout_* in batch.hpp.
+
+// kz = sqrt(n^2 - kx^2 - ky^2) with Im >= 0 (Re >= 0 when Im ~ 0); exact sign rules for real media
+// (rcwa1d_conical.hpp:193-203).
+DEV cd longitudinal(cd n, double kx, double ky) {
+    const cd z = n * n - cd{kx * kx + 0.0, 0.0} - cd{ky * ky, 0.0};
+    if (z.y == 0.0) return z.x >= 0.0 ? cd{sqrt(z.x), 0.0} : cd{0.0, sqrt(-z.x)};
+    cd kz = csqrt(z);
+    if (kz.y < 0.0) kz = -kz;
+    if (fabs(kz.y) < 1e-12 && kz.x < 0.0) kz = -kz;
+    return kz;
+}
+// zero order: (n - nI)(n + nI) + (nI cos theta)^2 (rcwa1d_conical.hpp:264-272)
+DEV cd longitudinal0(cd n, double nI, double cosine) {
+    const double t = nI * cosine;
+    const cd z = (n - cd{nI, 0.0}) * (n + cd{nI, 0.0}) + cd{t * t, 0.0};
+    if (z.y == 0.0) return z.x >= 0.0 ? cd{sqrt(z.x), 0.0} : cd{0.0, sqrt(-z.x)};
+    cd kz = csqrt(z);
+    if (kz.y < 0.0) kz = -kz;
+    if (fabs(kz.y) < 1e-12 && kz.x < 0.0) kz = -kz;
+    return kz;
+}
+
+// s branch (w, v) = (1, q), t branch (w, v) = (kz / n, -i n), q = -i kz (rcwa1d_conical.hpp:273-291)
+DEV void analytic_region(cd n, const cd* kx, double ky, int N, int M, double nI, double cosine, cd* q, cd* w, cd* v) {
+    for (int j = threadIdx.x; j < N; j += NT) {
+        const cd kz = j == M ? longitudinal0(n, nI, cosine) : longitudinal(n, kx[j].x, ky);
+        const cd qq{kz.y, -kz.x};
+        q[j] = qq;
+        q[N + j] = qq;
+        w[j] = cd{1.0, 0.0};
+        v[j] = qq;
+        w[N + j] = cdiv(kz, n);
+        v[N + j] = cd{n.y, -n.x};
+    }
+}
+
+struct ConicalGeo {
+    cd *kx, *cs, *sn, *kzI, *kzII, *flux;   // cs, sn, flux (r in .x, t in .y) real
+};
+DEV ConicalGeo geo_of(cd* ws, const int64_t* lay, int N) {
+    cd* g = ws + lay[0];
+    return ConicalGeo{g, g + N, g + 2 * N, g + 3 * N, g + 4 * N, g + 5 * N};
+}
+
+// a grating layer's slots (order N): A, BC, IE, C, Z1, X1, Z2, X2 (N x N); W, V (2N x 2N);
+// lambda1, lambda2, ort1, tmp1, ort2, tmp2, pivots (N)
+struct ConicalSlots {
+    cd *A, *BC, *IE, *C, *Z1, *X1, *Z2, *X2, *W, *V, *lam1, *lam2, *ort1, *tmp1, *ort2, *tmp2;
+    int* piv;
+};
+DEV ConicalSlots slots_of(cd* base, int N) {
+    const int64_t NN = (int64_t)N * N;
+    ConicalSlots s;
+    s.A = base;
+    s.BC = base + NN;
+    s.IE = base + 2 * NN;
+    s.C = base + 3 * NN;
+    s.Z1 = base + 4 * NN;
+    s.X1 = base + 5 * NN;
+    s.Z2 = base + 6 * NN;
+    s.X2 = base + 7 * NN;
+    s.W = base + 8 * NN;
+    s.V = base + 12 * NN;
+    cd* v = base + 16 * NN;
+    s.lam1 = v;
+    s.lam2 = v + N;
+    s.ort1 = v + 2 * N;
+    s.tmp1 = v + 3 * N;
+    s.ort2 = v + 4 * N;
+    s.tmp2 = v + 5 * N;
+    s.piv = reinterpret_cast<int*>(v + 6 * N);
+    return s;
+}
+
+DEV LayerView layer_at(const double* st, int index) {
+    LayerCursor cur(st);
+    LayerView lv = cur.next();
+    for (int j = 0; j < index; ++j) lv = cur.next();
+    return lv;
+}
+
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_setup_conical(const Inst1D* __restrict__ insts, const double* __restrict__ stacks,
+                                                      const int64_t* __restrict__ layouts, cd* __restrict__ ws) {
+    const Inst1D in = insts[blockIdx.x];
+    const double* st = stacks + in.stack_off;
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N, M = in.M, L = in.L;
+    const double period = st[0];
+    const cd nI = cd{st[1], st[2]}, nII = cd{st[3], st[4]};
+    const double theta = in.theta_deg * kPi / 180.0, phi = in.phi_deg * kPi / 180.0;
+    const double nIr = nI.x;
+    const double sine = nIr * sin(theta);
+    const double ky = sine * sin(phi);
+    const double cth = cos(theta);
+    const double flux_unit = nIr * cth;
+    ConicalGeo g = geo_of(ws, lay, N);
+    for (int j = threadIdx.x; j < N; j += NT) {
+        const double kxj = sine * cos(phi) - (j - M) * in.wavelength / period;
+        const double kt = hypot(kxj, ky);
+        g.kx[j] = cd{kxj, 0.0};
+        if (kt <= 1e-14) {
+            g.cs[j] = cd{cos(phi), 0.0};
+            g.sn[j] = cd{sin(phi), 0.0};
+        } else {
+            g.cs[j] = cd{kxj / kt, 0.0};
+            g.sn[j] = cd{ky / kt, 0.0};
+        }
+        const cd kzI = j == M ? longitudinal0(nI, nIr, cth) : longitudinal(nI, kxj, ky);
+        const cd kzII = j == M ? longitudinal0(nII, nIr, cth) : longitudinal(nII, kxj, ky);
+        g.kzI[j] = kzI;
+        g.kzII[j] = kzII;
+        g.flux[j] = cd{kzI.x / flux_unit, kzII.x / flux_unit};
+    }
+    if (threadIdx.x == 0) g.kx[6 * N] = cd{ky, 0.0};
+    __syncthreads();
+    analytic_region(nI, g.kx, ky, N, M, nIr, cth, ws + lay[conical_q(0)], ws + lay[conical_w(0)], ws + lay[conical_v(0)]);
+    analytic_region(nII, g.kx, ky, N, M, nIr, cth, ws + lay[conical_q(L + 1)], ws + lay[conical_w(L + 1)],
+                    ws + lay[conical_v(L + 1)]);
+    for (int j = 0; j < L; ++j) {
+        if (lay[conical_slot(L, j)] >= 0) continue;
+        const LayerView lv = layer_at(st, (int)lay[conical_orig(L, j)]);
+        const cd n = lv.homogeneous ? lv.n : cd{lv.vals[0], lv.vals[1]};
+        analytic_region(n, g.kx, ky, N, M, nIr, cth, ws + lay[conical_q(j + 1)], ws + lay[conical_w(j + 1)],
+                        ws + lay[conical_v(j + 1)]);
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_assemble_conical(const Inst1D* __restrict__ insts,
+                                                         const LayerTask* __restrict__ tasks,
+                                                         const double* __restrict__ stacks,
+                                                         const int64_t* __restrict__ layouts, cd* __restrict__ ws,
+                                                         int* __restrict__ task_info) {
+    const LayerTask tk = tasks[blockIdx.x];
+    const Inst1D in = insts[tk.inst];
+    const double* st = stacks + in.stack_off;
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N, M = in.M, L = in.L;
+    const int64_t NN = (int64_t)N * N;
+    const LayerView lv = layer_at(st, (int)lay[conical_orig(L, tk.layer)]);
+    const ConicalGeo g = geo_of(ws, lay, N);
+    const ConicalSlots s = slots_of(ws + lay[conical_slot(L, tk.layer)], N);
+    cd* coef = s.lam1;                         // lambda1 .. tmp2 hold the 4M+1 coefficients
+    int fail = 0;
+    fourier_coefficients(lv, st[0], 2 * M, false, coef);
+    toeplitz(coef, N, s.A);                    // [[eps]]
+    wg_copy(N, N, s.A, N, s.Z1, N);
+    fail |= wg_lu(N, s.Z1, N, s.piv);
+    wg_identity(N, s.IE, N);
+    if (!fail) wg_lu_solve(N, s.Z1, N, s.piv, s.IE, N, N);          // [[eps]]^-1
+    fourier_coefficients(lv, st[0], 2 * M, true, coef);
+    toeplitz(coef, N, s.Z2);                   // [[1/eps]]
+    fail |= wg_lu(N, s.Z2, N, s.piv);
+    wg_identity(N, s.C, N);
+    if (!fail) wg_lu_solve(N, s.Z2, N, s.piv, s.C, N, N);           // C = [[1/eps]]^-1
+    // B = Kx [[eps]]^-1 Kx - I, then B C
+    for (int64_t e = threadIdx.x; e < NN; e += NT) {
+        const int i = (int)(e % N), j = (int)(e / N);
+        cd v = (g.kx[i].x * s.IE[e]) * g.kx[j].x;
+        if (i == j) v = v - cd{1.0, 0.0};
+        s.X1[e] = v;
+    }
+    __syncthreads();
+    wg_gemm<GEMM_SET, false>(N, N, N, s.X1, N, s.C, N, s.BC, N);
+    // A = Kx^2 - [[eps]]
+    for (int64_t e = threadIdx.x; e < NN; e += NT) {
+        const int i = (int)(e % N), j = (int)(e / N);
+        cd v = -s.A[e];
+        if (i == j) v = v + cd{g.kx[i].x * g.kx[i].x, 0.0};
+        s.A[e] = v;
+    }
+    __syncthreads();
+    int bad = 0;
+    for (int64_t e = threadIdx.x; e < NN; e += NT) bad |= !finite(s.A[e]) || !finite(s.BC[e]);
+    bad = block_max_int(bad);
+    if (threadIdx.x == 0) task_info[4 * blockIdx.x] = (fail || bad) ? -1 : 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_eig_conical(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
+                                                    const int64_t* __restrict__ layouts, cd* __restrict__ ws,
+                                                    int* __restrict__ task_info, int max_iter) {
+    const int t = blockIdx.x >> 1, which = blockIdx.x & 1;
+    if (task_info[4 * t] != 0) return;
+    const LayerTask tk = tasks[t];
+    const Inst1D in = insts[tk.inst];
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N;
+    const ConicalSlots s = slots_of(ws + lay[conical_slot(in.L, tk.layer)], N);
+    int info;
+    if (which == 0) info = wg_eig(N, s.A, s.lam1, s.Z1, s.X1, EigScratch{s.ort1, s.tmp1}, max_iter, kEps);
+    else info = wg_eig(N, s.BC, s.lam2, s.Z2, s.X2, EigScratch{s.ort2, s.tmp2}, max_iter, kEps);
+    if (threadIdx.x == 0) task_info[4 * t + 1 + which] = info;
+}
+
+
+// ------------------------------------------------------------------------------------------------
+// Schur-basis modes of a degenerate layer (rcwa1d_conical.hpp:413-502), order n = 2N.
+// Scratch E1..E4 (n x n each) come from the instance's S-matrix workspace, which is idle until the
+// S-matrix kernel runs; each grating layer of an instance owns its own quarter-sized share.
+
+// max_j sum_i |a(i, j)|
+DEV double wg_norm1(int n, const cd* a) {
+    double best = 0.0;
+    for (int j = wave_id(); j < n; j += NWAVE) {
+        double sum = 0.0;
+        for (int i = lane_id(); i < n; i += 64) sum += cabs(a[i + (int64_t)j * n]);
+        sum = wave_sum(sum);
+        best = fmax(best, sum);
+    }
+    return block_max(best);
+}
+
+// G = sqrt(T) for upper triangular T, in place; fails on a zero or non-finite root or denominator.
+DEV int wg_triangular_root(int n, cd* g) {
+    int bad = 0;
+    for (int i = threadIdx.x; i < n; i += NT) {
+        const cd r = branch_q(csqrt(g[i + (int64_t)i * n]));
+        if (is_zero(r) || !finite(r)) bad = 1;
+        g[i + (int64_t)i * n] = r;
+    }
+    if (block_max_int(bad)) return 1;
+    for (int gap = 1; gap < n; ++gap) {
+        for (int i = threadIdx.x; i + gap < n; i += NT) {
+            const int j = i + gap;
+            cd v = g[i + (int64_t)j * n];
+            for (int k = i + 1; k < j; ++k) v = v - g[i + (int64_t)k * n] * g[k + (int64_t)j * n];
+            const cd den = g[i + (int64_t)i * n] + g[j + (int64_t)j * n];
+            if (is_zero(den)) bad = 1;
+            g[i + (int64_t)j * n] = cdiv(v, den);
+        }
+        __syncthreads();
+    }
+    if (block_max_int(bad)) return 1;
+    int nf = 0;
+    for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) nf |= !finite(g[e]);
+    return block_max_int(nf);
+}
+
+// exp(a) of an upper triangular a by scaled Taylor series and squaring, with the diagonal replaced
+// by exact exponentials after every stage (rcwa1d_conical.hpp:441-462).  a, e, t1, t2: n x n;
+// diag_a[i] = a(i, i).  Returns 1 on failure; e holds the result.
+DEV int wg_triangular_exp(int n, cd* a, const cd* diag_a, cd* e, cd* t1, cd* t2) {
+    const int64_t nn = (int64_t)n * n;
+    const double size = wg_norm1(n, a);
+    if (!isfinite(size)) return 1;
+    const int scaling = size > 0.25 ? max(0, (int)ceil(log2(size) + 2.0)) : 0;
+    if (scaling > 1020) return 1;
+    const double sc = ldexp(1.0, -scaling);
+    for (int64_t x = threadIdx.x; x < nn; x += NT) {
+        a[x] = a[x] * sc;
+        const int i = (int)(x % n), j = (int)(x / n);
+        e[x] = cd{i == j ? 1.0 : 0.0, 0.0};
+        t1[x] = e[x];
+    }
+    __syncthreads();
+    bool converged = n == 0;
+    for (int k = 1; k <= 40 && !converged; ++k) {
+        wg_gemm<GEMM_SET, false>(n, n, n, t1, n, a, n, t2, n);
+        const double inv = 1.0 / k;
+        int bad = 0;
+        for (int64_t x = threadIdx.x; x < nn; x += NT) {
+            const cd term = t2[x] * cd{inv, 0.0};
+            t1[x] = term;
+            e[x] = e[x] + term;
+            bad |= !finite(e[x]) || !finite(term);
+        }
+        if (block_max_int(bad)) return 1;
+        const double nt = wg_norm1(n, t1), ne = wg_norm1(n, e);
+        converged = nt <= kEps * fmax(1.0, ne);
+    }
+    if (!converged) return 1;
+    for (int i = threadIdx.x; i < n; i += NT) e[i + (int64_t)i * n] = cexp(diag_a[i] * sc);
+    __syncthreads();
+    for (int k = 0; k < scaling; ++k) {
+        wg_gemm<GEMM_SET, false>(n, n, n, e, n, e, n, t2, n);
+        const double f = ldexp(1.0, k + 1 - scaling);
+        int bad = 0;
+        for (int64_t x = threadIdx.x; x < nn; x += NT) {
+            const int i = (int)(x % n), j = (int)(x / n);
+            e[x] = i == j ? cexp(diag_a[i] * f) : t2[x];
+            bad |= !finite(e[x]);
+        }
+        if (block_max_int(bad)) return 1;
+    }
+    return 0;
+}
+
+// The degenerate layer's Schur-basis modes: returns 1 on numerical failure, else writes q (2N),
+// W (= Z), V (= Q Z G^-1) and the propagation X = exp(-k0 d G) (into the Z1..X2 slots).
+DEV int schur_modes(const Inst1D& in, const double* st, const int64_t* lay, cd* ws, int layer, int ordinal,
+                    const ConicalGeo& g, const ConicalSlots& s, cd* q, double ky, int max_iter) {
+    const int N = in.N, M = in.M, n = 2 * N;
+    const int64_t NN = (int64_t)N * N, nn = (int64_t)n * n;
+    cd* E1 = ws + in.ws_lo + (int64_t)ordinal * 4 * nn;
+    cd* E2 = E1 + nn;
+    cd* E3 = E2 + nn;
+    cd* E4 = E3 + nn;
+    // A = Kx^2 - [[eps]] again (the eigensolve overwrote it): into the Z1 slot
+    const LayerView lv = layer_at(st, (int)lay[conical_orig(in.L, layer)]);
+    fourier_coefficients(lv, st[0], 2 * M, false, s.lam1);
+    toeplitz(s.lam1, N, s.Z1);
+    for (int64_t e = threadIdx.x; e < NN; e += NT) {
+        const int i = (int)(e % N), j = (int)(e / N);
+        cd v = -s.Z1[e];
+        if (i == j) v = v + cd{g.kx[i].x * g.kx[i].x, 0.0};
+        s.Z1[e] = v;
+    }
+    __syncthreads();
+    // P and Q (rcwa1d_conical.hpp:470-481) into E1, E2
+    for (int64_t e = threadIdx.x; e < NN; e += NT) {
+        const int i = (int)(e % N), j = (int)(e / N);
+        const double kxi = g.kx[i].x, kxj = g.kx[j].x;
+        const cd id{i == j ? 1.0 : 0.0, 0.0};
+        const cd ie = s.IE[e];
+        E1[i + (int64_t)j * n] = (ky * kxi) * ie;
+        E1[i + (int64_t)(N + j) * n] = id - (kxi * ie) * kxj;
+        E1[N + i + (int64_t)j * n] = (ky * ky) * ie - id;
+        E1[N + i + (int64_t)(N + j) * n] = ((-ky) * ie) * kxj;
+        E2[i + (int64_t)j * n] = cd{ky * kxi, 0.0} * id;
+        E2[i + (int64_t)(N + j) * n] = -s.Z1[e];
+        E2[N + i + (int64_t)j * n] = cd{ky * ky, 0.0} * id - s.C[e];
+        E2[N + i + (int64_t)(N + j) * n] = cd{-ky * kxi, 0.0} * id;
+    }
+    __syncthreads();
+    // P <- re P rv, Q <- rv Q re with the per-order 2 x 2 rotations
+    //   re = [[-sn, cs], [cs, sn]], rv = [[-cs, -sn], [-sn, cs]] (blocks over orders i, N+i)
+    for (int64_t e = threadIdx.x; e < nn; e += NT) {              // E3 = re P, E4 = rv Q
+        const int r = (int)(e % n), c = (int)(e / n);
+        const int i = r % N;
+        const double cs = g.cs[i].x, sn = g.sn[i].x;
+        const cd p0 = E1[i + (int64_t)c * n], p1 = E1[N + i + (int64_t)c * n];
+        const cd q0 = E2[i + (int64_t)c * n], q1 = E2[N + i + (int64_t)c * n];
+        if (r < N) {
+            E3[e] = (-sn) * p0 + cs * p1;
+            E4[e] = (-cs) * q0 + (-sn) * q1;
+        } else {
+            E3[e] = cs * p0 + sn * p1;
+            E4[e] = (-sn) * q0 + cs * q1;
+        }
+    }
+    __syncthreads();
+    for (int64_t e = threadIdx.x; e < nn; e += NT) {              // E1 = (re P) rv, E2 = (rv Q) re
+        const int r = (int)(e % n), c = (int)(e / n);
+        const int j = c % N;
+        const double cs = g.cs[j].x, sn = g.sn[j].x;
+        const cd x0 = E3[r + (int64_t)j * n], x1 = E3[r + (int64_t)(N + j) * n];
+        const cd y0 = E4[r + (int64_t)j * n], y1 = E4[r + (int64_t)(N + j) * n];
+        if (c < N) {
+            E1[e] = x0 * (-cs) + x1 * (-sn);
+            E2[e] = y0 * (-sn) + y1 * cs;
+        } else {
+            E1[e] = x0 * (-sn) + x1 * cs;
+            E2[e] = y0 * cs + y1 * sn;
+        }
+    }
+    __syncthreads();
+    // H = P Q, Schur form H = Z T Z^H (max_iter 100, rcwa1d_conical.hpp:487-489): T in E3, Z in W
+    wg_gemm<GEMM_SET, false>(n, n, n, E1, n, E2, n, E3, n);
+    int bad = 0;
+    for (int64_t e = threadIdx.x; e < nn; e += NT) bad |= !finite(E3[e]);
+    if (block_max_int(bad)) return 1;
+    cd* wv = s.lam1;              // eigenvalues and the two scratch vectors: lambda1 .. tmp2 (6N = 3n entries)
+    if (wg_eig(n, E3, wv, s.W, E4, EigScratch{wv + n, wv + 2 * n}, max_iter, kEps, false) != 0) return 1;
+    // G = sqrt(T) with a residual check ||G G - T||_1 <= 1e-10 ||T||_1
+    wg_copy(n, n, E3, n, E1, n);
+    if (wg_triangular_root(n, E3)) return 1;
+    wg_gemm<GEMM_SET, false>(n, n, n, E3, n, E3, n, E4, n);
+    for (int64_t e = threadIdx.x; e < nn; e += NT) E4[e] = E4[e] - E1[e];
+    __syncthreads();
+    const double scale = wg_norm1(n, E1), err = wg_norm1(n, E4);
+    if (!isfinite(scale) || !isfinite(err) || err > 1e-10 * fmax(scale, 2.2250738585072014e-308)) return 1;
+    for (int i = threadIdx.x; i < n; i += NT) q[i] = E3[i + (int64_t)i * n];
+    __syncthreads();
+    // V = Q Z G^-1 (right triangular solve, one thread per row)
+    wg_gemm<GEMM_SET, false>(n, n, n, E2, n, s.W, n, s.V, n);
+    for (int i = threadIdx.x; i < n; i += NT) {
+        for (int j = 0; j < n; ++j) {
+            cd v = s.V[i + (int64_t)j * n];
+            for (int k = 0; k < j; ++k) v = v - s.V[i + (int64_t)k * n] * E3[k + (int64_t)j * n];
+            s.V[i + (int64_t)j * n] = cdiv(v, E3[j + (int64_t)j * n]);
+        }
+    }
+    __syncthreads();
+    bad = 0;
+    for (int64_t e = threadIdx.x; e < nn; e += NT) bad |= !finite(s.V[e]);
+    if (block_max_int(bad)) return 1;
+    // X = exp(-k0 d G) into the Z1..X2 slots (4 N^2 = n^2, contiguous)
+    const double k0 = 2.0 * kPi / in.wavelength;
+    const double dk = -k0 * lv.d;
+    for (int64_t e = threadIdx.x; e < nn; e += NT) E1[e] = E3[e] * cd{dk, 0.0};
+    for (int i = threadIdx.x; i < n; i += NT) E2[i] = E3[i + (int64_t)i * n] * cd{dk, 0.0};
+    __syncthreads();
+    cd* diag_a = s.lam1;          // n entries: lambda1 .. tmp1 are free now
+    for (int i = threadIdx.x; i < n; i += NT) diag_a[i] = E2[i];
+    __syncthreads();
+    return wg_triangular_exp(n, E1, diag_a, s.Z1, E2, E4);
+}
+
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_modes_conical(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
+                                                      const double* __restrict__ stacks,
+                                                      const int64_t* __restrict__ layouts, cd* __restrict__ ws,
+                                                      int* __restrict__ task_info) {
+    const int t = blockIdx.x;
+    if (task_info[4 * t] != 0 || task_info[4 * t + 1] != 0 || task_info[4 * t + 2] != 0) return;
+    const LayerTask tk = tasks[t];
+    const Inst1D in = insts[tk.inst];
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N, L = in.L, n = 2 * N;
+    const int64_t NN = (int64_t)N * N;
+    const ConicalGeo g = geo_of(ws, lay, N);
+    const ConicalSlots s = slots_of(ws + lay[conical_slot(L, tk.layer)], N);
+    cd* q = ws + lay[conical_q(tk.layer + 1)];
+    // ---- degeneracy test (rcwa1d_conical.hpp:826-848)
+    double lm = INFINITY;
+    for (int j = threadIdx.x; j < N; j += NT) lm = fmin(lm, fmin(cabs(s.lam1[j]), cabs(s.lam2[j])));
+    const double local_min = -block_max(-lm);
+    const double kyy = ws[lay[0] + 6 * N].x;
+    const double ky_scale = fabs(kyy) * sqrt(fabs(kyy));
+    if (kyy != 0.0 && local_min < 1e-2 * ky_scale) {
+        const int fail = schur_modes(in, stacks + in.stack_off, lay, ws, tk.layer, t - in.first_task, g, s, q, kyy, 100);
+        if (threadIdx.x == 0) task_info[4 * t + 3] = fail ? -1 : 3;
+        return;
+    }
+    // ---- general region (rcwa1d_conical.hpp:341-411)
+    const cd ky2{kyy * kyy, 0.0};
+    for (int j = threadIdx.x; j < N; j += NT) {
+        q[j] = branch_q(csqrt(s.lam1[j] + ky2));
+        q[N + j] = branch_q(csqrt(s.lam2[j] + ky2));
+    }
+    // cw = C W2 -> X1; coupling = [[eps]]^-1 (ky Kx C W2) -> X2 (W2 lives in BC, W1 in A)
+    wg_gemm<GEMM_SET, false>(N, N, N, s.C, N, s.BC, N, s.X1, N);
+    if (kyy != 0.0) {
+        for (int64_t e = threadIdx.x; e < NN; e += NT) {
+            const int m = (int)(e % N);
+            s.Z1[e] = (kyy * g.kx[m].x) * s.X1[e];
+        }
+        __syncthreads();
+        wg_gemm<GEMM_SET, false>(N, N, N, s.IE, N, s.Z1, N, s.X2, N);
+    }
+    const int64_t nn = (int64_t)n;
+    for (int64_t e = threadIdx.x; e < (int64_t)N * N; e += NT) {
+        const int m = (int)(e % N), j = (int)(e / N);
+        const double csm = g.cs[m].x, snm = g.sn[m].x;
+        {   // first group: Ex = 0, Ey = W1 q1^2, hx = -W1 l1 q1, hy = -ky Kx W1 q1
+            const cd q1 = q[j];
+            const cd fy = q1 * q1, gx = -(s.lam1[j] * q1), gy = (-kyy) * q1;
+            const cd w1 = s.A[m + (int64_t)j * N];
+            const cd ey = w1 * fy, hx = w1 * gx, hy = (g.kx[m].x * w1) * gy;
+            s.W[m + j * nn] = csm * ey;
+            s.W[N + m + j * nn] = snm * ey;
+            s.V[m + j * nn] = -(csm * hx) - snm * hy;
+            s.V[N + m + j * nn] = csm * hy - snm * hx;
+        }
+        {   // second group: Ex = W2 l2, Ey = ky [[eps]]^-1 Kx C W2, hx = 0, hy = -C W2 q2
+            const cd q2 = q[N + j], l2 = s.lam2[j];
+            const cd ex = s.BC[m + (int64_t)j * N] * l2;
+            const cd ey = kyy != 0.0 ? s.X2[m + (int64_t)j * N] : cd{0.0, 0.0};
+            const cd hy = -(s.X1[m + (int64_t)j * N] * q2);
+            const int64_t col = (int64_t)(N + j) * nn;
+            s.W[m + col] = csm * ey - snm * ex;
+            s.W[N + m + col] = csm * ex + snm * ey;
+            s.V[m + col] = -(snm * hy);
+            s.V[N + m + col] = csm * hy;
+        }
+    }
+    __syncthreads();
+    // normalise every column of [W; V] to unit 2-norm; a zero or non-finite column fails the layer
+    int bad = 0;
+    for (int col = wave_id(); col < n; col += NWAVE) {
+        cd* wc = s.W + col * nn;
+        cd* vc = s.V + col * nn;
+        double sum = 0.0;
+        for (int i = lane_id(); i < n; i += 64) sum += norm2(wc[i]) + norm2(vc[i]);
+        sum = wave_sum(sum);
+        const double scale = sqrt(sum);
+        if (!(scale > 0.0) || !isfinite(scale)) {
+            bad = 1;
+            continue;
+        }
+        for (int i = lane_id(); i < n; i += 64) {
+            wc[i] = cdiv(wc[i], scale);
+            vc[i] = cdiv(vc[i], scale);
+        }
+    }
+    bad = block_max_int(bad);
+    if (threadIdx.x == 0) task_info[4 * t + 3] = bad ? -1 : 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void k_smatrix_conical(const Inst1D* __restrict__ insts,
+                                                        const double* __restrict__ stacks,
+                                                        const int64_t* __restrict__ layouts, cd* __restrict__ ws,
+                                                        const int* __restrict__ task_info, Out1D out) {
+    const Inst1D in = insts[blockIdx.x];
+    const double* st = stacks + in.stack_off;
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N, M = in.M, L = in.L, n = 2 * N;
+    const double k0 = 2.0 * kPi / in.wavelength;
+    const ConicalGeo geo = geo_of(ws, lay, N);
+    cd* Jr = reinterpret_cast<cd*>(out.R) + 4 * in.out_off;
+    cd* Jt = reinterpret_cast<cd*>(out.T) + 4 * in.out_off;
+    double* DEr = out.DEr + 4 * in.out_off;
+    double* DEt = out.DEt + 4 * in.out_off;
+
+    int eig_info = 0, failed = 0;
+    for (int t = 0; t < in.n_tasks; ++t) {
+        const int* ti = task_info + 4 * (in.first_task + t);
+        if (ti[0] < 0 || ti[3] < 0) failed = 1;
+        for (int w = 1; w <= 2; ++w)
+            if (ti[w] != 0 && eig_info == 0) eig_info = ti[w];
+    }
+    int status = (failed || eig_info) ? ST_NUMERICAL : ST_OK;
+    // a uniform layer with an exactly grazing order fails (pure film stacks: unexcited orders excepted)
+    bool pure = true;
+    for (int j = 0; j < L; ++j) pure = pure && lay[conical_slot(L, j)] < 0;
+    if (status == ST_OK) {
+        int zero = 0;
+        for (int j = 0; j < L; ++j) {
+            if (lay[conical_slot(L, j)] >= 0) continue;
+            const cd* q = ws + lay[conical_q(j + 1)];
+            for (int k = threadIdx.x; k < N; k += NT)
+                if (is_zero(q[k]) && (!pure || k == M)) zero = 1;
+        }
+        if (block_max_int(zero)) status = ST_NUMERICAL;
+    }
+    __shared__ double s_thick[SMAT_MAX_LAYERS];
+    if (status == ST_OK) {
+        for (int j = 0; j < L; ++j) {
+            const LayerView lv = layer_at(st, (int)lay[conical_orig(L, j)]);
+            if (threadIdx.x == 0) s_thick[j] = lv.d;
+        }
+        __syncthreads();
+        SmatCtx c;
+        c.N = n;
+        c.bump = Bump(ws, in.ws_lo, in.ws_hi);
+        c.fail = 0;
+        auto region = [&](int r) -> Region {
+            Region g;
+            g.fzb = false;
+            g.X = nullptr;
+            g.q = ws + lay[conical_q(r)];
+            if (r == 0 || r == L + 1 || lay[conical_slot(L, r - 1)] < 0) {
+                g.analytic = true;
+                g.w = ws + lay[conical_w(r)];
+                g.p = ws + lay[conical_v(r)];
+                g.W = g.V = nullptr;
+            } else {
+                const ConicalSlots sl = slots_of(ws + lay[conical_slot(L, r - 1)], N);
+                g.analytic = false;
+                g.w = nullptr;
+                g.p = nullptr;
+                g.W = sl.W;
+                g.V = sl.V;
+                // a Schur-basis layer propagates with its dense exp(-k0 d G) (task status 3)
+                int ord = 0;
+                for (int jj = 0; jj < r - 1; ++jj) ord += lay[conical_slot(L, jj)] >= 0;
+                if (task_info[4 * (in.first_task + ord) + 3] == 3) g.X = sl.Z1;
+            }
+            return g;
+        };
+        cd* upm = c.bump.keep(2 * (int64_t)n);
+        cd* downm = c.bump.keep(2 * (int64_t)n);
+        c.check();
+        const int cols[2] = {M, N + M};
+        if (!c.fail) run_recursion(c, L, k0, s_thick, region, 2, cols, upm, downm);
+        status = c.fail ? (c.fail == 2 ? ST_WORKSPACE : ST_NUMERICAL) : ST_OK;
+        if (status == ST_OK) {
+            int bad = 0;
+            for (int j = threadIdx.x; j < N; j += NT) {
+                // outer media: the s amplitude is Es; the t amplitude is Ep downwards and -Ep upwards
+                cd jr[4] = {upm[j], upm[n + j], -upm[N + j], -upm[n + N + j]};
+                cd jt[4] = {downm[j], downm[n + j], downm[N + j], downm[n + N + j]};
+                if (pure && j != M)
+                    for (int k = 0; k < 4; ++k) {
+                        if (!finite(jr[k])) jr[k] = cd{0.0, 0.0};
+                        if (!finite(jt[k])) jt[k] = cd{0.0, 0.0};
+                    }
+                const double fr = geo.flux[j].x, ft = geo.flux[j].y;
+                for (int k = 0; k < 4; ++k) {
+                    if (!finite(jr[k]) || !finite(jt[k])) bad = 1;
+                    Jr[4 * j + k] = jr[k];
+                    Jt[4 * j + k] = jt[k];
+                    const double er = fr * norm2(jr[k]), et = ft * norm2(jt[k]);
+                    if (!isfinite(er) || !isfinite(et)) bad = 1;
+                    DEr[4 * j + k] = er;
+                    DEt[4 * j + k] = et;
+                }
+            }
+            if (block_max_int(bad)) status = ST_NUMERICAL;
+        }
+    }
+    __syncthreads();
+    if (status != ST_OK) {
+        const double nan = __builtin_nan("");
+        for (int j = threadIdx.x; j < 4 * N; j += NT) {
+            Jr[j] = cd{nan, nan};
+            Jt[j] = cd{nan, nan};
+            DEr[j] = nan;
+            DEt[j] = nan;
+        }
+    }
+    if (threadIdx.x == 0) {
+        out.status[blockIdx.x] = status;
+        out.eig_info[blockIdx.x] = eig_info;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
+                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
+                    hipEvent_t* ev) {
+    if (ev) hipEventRecord(ev[0], s);
+    k_setup_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws);
+    if (n_tasks > 0) k_assemble_conical<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
+    if (ev) hipEventRecord(ev[1], s);
+    if (n_tasks > 0) k_eig_conical<<<2 * n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info, max_iter);
+    if (ev) hipEventRecord(ev[2], s);
+    if (n_tasks > 0) k_modes_conical<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
+    k_smatrix_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
+    if (ev) hipEventRecord(ev[3], s);
+}

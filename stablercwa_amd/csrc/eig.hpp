@@ -40,8 +40,11 @@ struct EigScratch {
 //   in : A = the matrix (overwritten)
 //   out: A = right eigenvectors (unit 2-norm columns) when info == 0; w = eigenvalues
 // Z and X are n x n scratch matrices.  Returns info (0, or k > 0 when eigenvalue k did not converge).
+// With vectors == false the Schur form is returned instead (the reference's keep_transform without
+// vectors): A = T (upper triangular, eigenvalues on the diagonal), Z = the unitary transform,
+// Z^H A_in Z = T.
 DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z, cd* __restrict__ X,
-               EigScratch s, int max_iter, double eps) {
+               EigScratch s, int max_iter, double eps, bool vectors = true) {
     const int tid = threadIdx.x;
     const int lane = lane_id(), wave = wave_id();
     cd* H = A;
@@ -332,6 +335,7 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
     for (int j = 0; j + 1 < n; ++j)
         if (tid == 0) h(j + 1, j) = cd{0.0, 0.0};
     __syncthreads();
+    if (!vectors) return 0;
     double nm = 0.0;
     for (int e = tid; e < n * n; e += NT) {
         const int i = e % n, j = e / n;

@@ -181,6 +181,8 @@ __global__ __launch_bounds__(NT) void k_smatrix1d(const Inst1D* __restrict__ ins
             Region g;
             g.q = ws + lay[layout_q(r)];
             g.p = ws + lay[layout_p(r)];
+            g.w = nullptr;
+            g.W = g.V = g.X = nullptr;
             if (r == 0 || r == L + 1) {
                 g.analytic = true;
                 g.fzb = false;
@@ -204,7 +206,8 @@ __global__ __launch_bounds__(NT) void k_smatrix1d(const Inst1D* __restrict__ ins
             pure = pure && lv.homogeneous;
         }
         __syncthreads();
-        run_planar_recursion(c, L, M, k0, s_thick, region, R, T);
+        const int cols[1] = {M};
+        run_recursion(c, L, k0, s_thick, region, 1, cols, R, T);
         status = c.fail ? (c.fail == 2 ? ST_WORKSPACE : ST_NUMERICAL) : ST_OK;
         if (status == ST_OK) {
             __syncthreads();

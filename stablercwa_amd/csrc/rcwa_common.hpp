@@ -169,13 +169,19 @@ struct DOp {
     cd* p;
 };
 
+// Modes of one region.  Homogeneous regions are diagonal: planar ones carry p (V = diag(p), W = I,
+// w == nullptr); conical ones carry w and v in the per-order (s, t) basis (p holds v).  General
+// regions carry dense W, V.  A layer propagated in its Schur basis carries the dense
+// exp(-k0 d G) in X (conical degenerate layers, rcwa1d_conical.hpp:413-502).
 struct Region {
     bool analytic;
-    bool fzb;           // finite-layer zero-root basis
+    bool fzb;           // finite-layer zero-root basis (planar only)
     cd* q;
     cd* p;
+    cd* w;
     cd* W;
     cd* V;
+    cd* X;
 };
 
 struct SBlock {
@@ -241,6 +247,31 @@ DEV void op_apply(SmatCtx& c, DOp a, const cd* x, cd* y) {
     __syncthreads();
 }
 
+// Y = a X for nrhs columns (Y must not alias X)
+DEV void op_apply_cols(SmatCtx& c, DOp a, const cd* x, cd* y, int nrhs) {
+    const int N = c.N;
+    if (a.kind == OP_DENSE && nrhs > 1) {
+        wg_gemm<GEMM_SET, false>(N, nrhs, N, a.p, N, x, N, y, N);
+        return;
+    }
+    for (int k = 0; k < nrhs; ++k) {
+        const cd* xk = x + (int64_t)k * N;
+        cd* yk = y + (int64_t)k * N;
+        for (int i = threadIdx.x; i < N; i += NT) {
+            if (a.kind == OP_DENSE) {
+                cd s{0.0, 0.0};
+                for (int j = 0; j < N; ++j) s = s + a.p[i + (int64_t)j * N] * xk[j];
+                yk[i] = s;
+            } else if (a.kind == OP_DIAG) {
+                yk[i] = a.p[i] * xk[i];
+            } else {
+                yk[i] = cd{0.0, 0.0};
+            }
+        }
+    }
+    __syncthreads();
+}
+
 DEV cd* op_to_dense(SmatCtx& c, DOp a) {
     const int N = c.N;
     cd* m = c.bump.tmp((int64_t)N * N);
@@ -284,13 +315,26 @@ DEV SBlock diagonal_interface(SmatCtx& c, const Region& up, const Region& low) {
     cd* rdu = c.bump.tmp(N);
     cd* rud = c.bump.tmp(N);
     cd* tuu = c.bump.tmp(N);
-    for (int j = threadIdx.x; j < N; j += NT) {
-        const cd p1 = up.p[j], p2 = low.p[j];
-        const cd inv = cdiv(cd{1.0, 0.0}, p1 + p2);
-        tdd[j] = (2.0 * p1) * inv;
-        rdu[j] = (p2 - p1) * inv;
-        rud[j] = (p1 - p2) * inv;
-        tuu[j] = (2.0 * p2) * inv;
+    if (up.w == nullptr) {
+        for (int j = threadIdx.x; j < N; j += NT) {
+            const cd p1 = up.p[j], p2 = low.p[j];
+            const cd inv = cdiv(cd{1.0, 0.0}, p1 + p2);
+            tdd[j] = (2.0 * p1) * inv;
+            rdu[j] = (p2 - p1) * inv;
+            rud[j] = (p1 - p2) * inv;
+            tuu[j] = (2.0 * p2) * inv;
+        }
+    } else {
+        // (w, v) form, finite when one side grazes (rcwa1d_conical.hpp:507-527)
+        for (int j = threadIdx.x; j < N; j += NT) {
+            const cd w1 = up.w[j], v1 = up.p[j], w2 = low.w[j], v2 = low.p[j];
+            const cd a = w1 * v2, b = v1 * w2;
+            const cd inv = cdiv(cd{1.0, 0.0}, a + b);
+            tdd[j] = (2.0 * (w1 * v1)) * inv;
+            rdu[j] = (a - b) * inv;
+            rud[j] = (b - a) * inv;
+            tuu[j] = (2.0 * (w2 * v2)) * inv;
+        }
     }
     __syncthreads();
     return SBlock{DOp{OP_DIAG, tdd}, DOp{OP_DIAG, rdu}, DOp{OP_DIAG, rud}, DOp{OP_DIAG, tuu}};
@@ -304,7 +348,21 @@ DEV void interface_matching(SmatCtx& c, const Region& up, const Region& low, cd*
     cd* nn = c.bump.tmp(NN);
     c.check();
     if (c.fail) return;
-    if (low.analytic) {
+    if (low.analytic && low.w != nullptr) {
+        // W2 = diag(w2), V2 = diag(v2): row scaling; a zero (grazing order) makes the interface singular
+        int zero = 0;
+        for (int j = threadIdx.x; j < N; j += NT) zero |= is_zero(low.w[j]) || is_zero(low.p[j]);
+        if (block_max_int(zero)) {
+            c.fail = 1;
+            return;
+        }
+        for (int64_t e = threadIdx.x; e < NN; e += NT) {
+            const int i = (int)(e % N);
+            mm[e] = cdiv(cd{1.0, 0.0}, low.w[i]) * up.W[e];
+            nn[e] = cdiv(cd{1.0, 0.0}, low.p[i]) * up.V[e];
+        }
+        __syncthreads();
+    } else if (low.analytic) {
         wg_copy(N, N, up.W, N, mm, N);
         for (int64_t e = threadIdx.x; e < NN; e += NT) {
             const int i = (int)(e % N);
@@ -312,7 +370,8 @@ DEV void interface_matching(SmatCtx& c, const Region& up, const Region& low, cd*
         }
         __syncthreads();
     } else if (up.analytic) {
-        wg_identity(N, mm, N);
+        if (up.w != nullptr) wg_diag(N, up.w, mm, N);
+        else wg_identity(N, mm, N);
         dense_solve(c, low.W, mm, N);
         wg_diag(N, up.p, nn, N);
         dense_solve(c, low.V, nn, N);
@@ -427,13 +486,14 @@ DEV DOp op_keep(SmatCtx& c, DOp a, cd* slot) {
     return DOp{a.kind, slot};
 }
 
-// The whole planar recursion: bottom-up seams, the top block (incident column M only), the
-// downward sweep.  region(r) returns the modes of region r (0 incidence, j+1 layer j, L+1 exit).
-// Writes the reflected / transmitted amplitudes R, T (N each).  On return c.fail says whether the
-// instance failed.  (rcwa1d.hpp:963-1034)
+// The whole recursion (rcwa1d.hpp:963-1034; conical rcwa1d_conical.hpp:859-919): bottom-up seams,
+// the top block restricted to the incident columns cols[0..nrhs), the downward sweep.
+// region(r) returns the modes of region r (0 incidence, j+1 layer j, L+1 exit).  Writes the
+// reflected (up) and transmitted (down) amplitudes, N x nrhs each.  On return c.fail says whether
+// the instance failed.
 template <class RegionFn>
-DEV void run_planar_recursion(SmatCtx& c, int L, int M, double k0, const double* thick, RegionFn region, cd* Rout,
-                              cd* Tout) {
+DEV void run_recursion(SmatCtx& c, int L, double k0, const double* thick, RegionFn region, int nrhs, const int* cols,
+                       cd* up_out, cd* down_out) {
     __shared__ int g_kind[SMAT_MAX_BLOCKS];
     __shared__ cd* g_ptr[SMAT_MAX_BLOCKS];
     const int N = c.N;
@@ -451,7 +511,8 @@ DEV void run_planar_recursion(SmatCtx& c, int L, int M, double k0, const double*
             blk = interface_block(c, region(j), region(j + 1));
         } else {
             const Region g = region(j + 1);
-            blk = propagation_block(c, g.q, thick[j], k0, g.fzb);
+            if (g.X != nullptr) blk = SBlock{DOp{OP_DENSE, g.X}, op_zero(), op_zero(), DOp{OP_DENSE, g.X}};
+            else blk = propagation_block(c, g.q, thick[j], k0, g.fzb);
         }
         c.check();
         if (c.fail) break;
@@ -473,16 +534,18 @@ DEV void run_planar_recursion(SmatCtx& c, int L, int M, double k0, const double*
     }
     if (c.fail) return;
 
-    // top block: only the incident column M of Tdd and Rud (rcwa1d.hpp:715-746)
+    // top block: only the incident columns of Tdd and Rud (rcwa1d.hpp:715-746, rcwa1d_conical.hpp:765-795)
     const Region up = region(0), low = region(1);
-    cd* tdd = c.bump.tmp(N);
-    cd* rud = c.bump.tmp(N);
+    const int64_t NR = (int64_t)N * nrhs;
+    cd* tdd = c.bump.tmp(NR);
+    cd* rud = c.bump.tmp(NR);
     DOp Rdu, Tuu;
     if (up.analytic && low.analytic) {
         const SBlock s = diagonal_interface(c, up, low);
-        for (int j = threadIdx.x; j < N; j += NT) {
-            tdd[j] = j == M ? s.Tdd.p[M] : cd{0.0, 0.0};
-            rud[j] = j == M ? s.Rud.p[M] : cd{0.0, 0.0};
+        for (int64_t e = threadIdx.x; e < NR; e += NT) {
+            const int j = (int)(e % N), k = (int)(e / N);
+            tdd[e] = j == cols[k] ? s.Tdd.p[j] : cd{0.0, 0.0};
+            rud[e] = j == cols[k] ? s.Rud.p[j] : cd{0.0, 0.0};
         }
         __syncthreads();
         Rdu = s.Rdu;
@@ -493,31 +556,37 @@ DEV void run_planar_recursion(SmatCtx& c, int L, int M, double k0, const double*
         if (c.fail) return;
         cd* tpi = c.bump.tmp(NN);
         cd* rdu = c.bump.tmp(NN);
+        cd* tme = c.bump.tmp(NR);
         c.check();
         if (c.fail) return;
         wg_identity(N, tpi, N);
         dense_solve(c, tp, tpi, N);
         if (c.fail) return;
         wg_gemm<GEMM_SET, false>(N, N, N, tm, N, tpi, N, rdu, N);
-        const cd* tm_e = tm + (int64_t)M * N;
-        op_apply(c, DOp{OP_DENSE, rdu}, tm_e, tdd);
-        for (int j = threadIdx.x; j < N; j += NT) tdd[j] = tp[j + (int64_t)M * N] - tdd[j];
+        for (int64_t e = threadIdx.x; e < NR; e += NT) {
+            const int j = (int)(e % N), k = (int)(e / N);
+            tme[e] = tm[j + (int64_t)cols[k] * N];
+            tdd[e] = tp[j + (int64_t)cols[k] * N];
+        }
         __syncthreads();
-        op_apply(c, DOp{OP_DENSE, tpi}, tm_e, rud);
-        for (int j = threadIdx.x; j < N; j += NT) rud[j] = -rud[j];
-        __syncthreads();
+        wg_gemm<GEMM_SUB, false>(N, nrhs, N, rdu, N, tme, N, tdd, N);    // Tdd e = T+ e - Rdu (T- e)
+        wg_fill(NR, rud, cd{0.0, 0.0});
+        wg_gemm<GEMM_SUB, false>(N, nrhs, N, tpi, N, tme, N, rud, N);    // Rud e = -T+^-1 (T- e)
         Rdu = DOp{OP_DENSE, rdu};
         Tuu = DOp{OP_DENSE, tpi};
     }
     DOp k = op_mul(c, Rdu, r);
-    cd* down = c.bump.tmp(N);
-    cd* tmpv = c.bump.tmp(N);
+    cd* down = c.bump.tmp(NR);
+    cd* tmpv = c.bump.tmp(NR);
     c.check();
     if (c.fail) return;
-    for (int j = threadIdx.x; j < N; j += NT) down[j] = tdd[j];
+    for (int64_t e = threadIdx.x; e < NR; e += NT) down[e] = tdd[e];
     __syncthreads();
     if (k.kind == OP_DIAG) {
-        for (int j = threadIdx.x; j < N; j += NT) down[j] = cdiv(down[j], cd{1.0, 0.0} - k.p[j]);
+        for (int64_t e = threadIdx.x; e < NR; e += NT) {
+            const int j = (int)(e % N);
+            down[e] = cdiv(down[e], cd{1.0, 0.0} - k.p[j]);
+        }
         __syncthreads();
     } else if (k.kind == OP_DENSE) {
         for (int64_t e = threadIdx.x; e < NN; e += NT) {
@@ -527,19 +596,19 @@ DEV void run_planar_recursion(SmatCtx& c, int L, int M, double k0, const double*
             k.p[e] = v;
         }
         __syncthreads();
-        dense_solve_inplace(c, k.p, down, 1);
+        dense_solve_inplace(c, k.p, down, nrhs);
         if (c.fail) return;
     }
-    op_apply(c, r, down, tmpv);
-    op_apply(c, Tuu, tmpv, Rout);
-    for (int j = threadIdx.x; j < N; j += NT) Rout[j] = rud[j] + Rout[j];
+    op_apply_cols(c, r, down, tmpv, nrhs);
+    op_apply_cols(c, Tuu, tmpv, up_out, nrhs);
+    for (int64_t e = threadIdx.x; e < NR; e += NT) up_out[e] = rud[e] + up_out[e];
     __syncthreads();
     for (int b = 1; b < nb; ++b) {
-        op_apply(c, DOp{g_kind[b], g_ptr[b]}, down, tmpv);
+        op_apply_cols(c, DOp{g_kind[b], g_ptr[b]}, down, tmpv, nrhs);
         cd* sw = down;
         down = tmpv;
         tmpv = sw;
     }
-    for (int j = threadIdx.x; j < N; j += NT) Tout[j] = down[j];
+    for (int64_t e = threadIdx.x; e < NR; e += NT) down_out[e] = down[e];
     __syncthreads();
 }

@@ -20,6 +20,9 @@
 void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
                    hipEvent_t* ev);
+void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
+                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
+                    hipEvent_t* ev);
 void launch_zgemm_batched(hipStream_t s, int count, const DenseItem* items, cd* arena);
 void launch_zgesv_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info);
 void launch_zgeev_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info, int max_iter);
@@ -98,6 +101,7 @@ void check_index(double re, double im) {
 
 struct LayerInfo {
     bool grating;
+    bool uniform_values;   // a grating whose segment indices are all exactly equal
     double d;
 };
 
@@ -122,7 +126,7 @@ std::vector<LayerInfo> parse_stack1d(const double* p, int64_t avail, int M) {
             if (!std::isfinite(d) || d < 0.0) invalid("rcwa: layer thickness must be finite and >= 0");
             check_index(p[k], p[k + 1]);
             k += 2;
-            out.push_back({false, d});
+            out.push_back({false, true, d});
         } else if (kind == 1) {
             need(1);
             const int K = static_cast<int>(p[k]);
@@ -139,9 +143,13 @@ std::vector<LayerInfo> parse_stack1d(const double* p, int64_t avail, int M) {
                     invalid("rcwa: grating edges must be finite and strictly increasing");
             if (std::abs((e[K] - e[0]) - period) > 1e-10 * period)
                 invalid("rcwa: grating edges must span exactly one period");
-            for (int i = 0; i < K; ++i) check_index(v[2 * i], v[2 * i + 1]);
+            bool same = true;
+            for (int i = 0; i < K; ++i) {
+                check_index(v[2 * i], v[2 * i + 1]);
+                same = same && v[2 * i] == v[0] && v[2 * i + 1] == v[1];
+            }
             k += 1 + (K + 1) + 2 * K + 2;
-            out.push_back({true, d});
+            out.push_back({true, same, d});
         } else {
             invalid("rcwa: unknown layer kind");
         }
@@ -161,6 +169,46 @@ void check_setup(double period, double wavelength, double nIr, double nIi, doubl
     if (nIi != 0.0 || nIr <= 0.0) invalid("rcwa: the incident medium must have a real positive index");
 }
 
+struct ParsedStacks {
+    std::vector<std::vector<LayerInfo>> infos;
+    std::vector<double> words;
+};
+
+// Validates every flat stack record and copies the records' extent.
+ParsedStacks parse_all(int n_stacks, const double* stacks, const int64_t* stack_offsets) {
+    ParsedStacks out;
+    std::vector<int64_t> sorted(stack_offsets, stack_offsets + n_stacks);
+    for (int64_t o : sorted)
+        if (o < 0) invalid("rcwa: malformed stack offsets");
+    std::sort(sorted.begin(), sorted.end());
+    auto record_end = [&](int64_t off) {
+        auto it = std::upper_bound(sorted.begin(), sorted.end(), off);
+        return it == sorted.end() ? int64_t(-1) : *it;
+    };
+    out.infos.resize(n_stacks);
+    int64_t stack_words = 0;
+    for (int s = 0; s < n_stacks; ++s) {
+        const int64_t off = stack_offsets[s];
+        const int64_t end = record_end(off);
+        const int64_t avail = end > 0 ? end - off : (int64_t(1) << 40);
+        out.infos[s] = parse_stack1d(stacks + off, avail, 0);
+        int64_t k = 6;
+        const double* p = stacks + off;
+        for (size_t j = 0; j < out.infos[s].size(); ++j) {
+            const int kind = static_cast<int>(p[k + 1]);
+            k += 2;
+            if (kind == 0) k += 2;
+            else {
+                const int K = static_cast<int>(p[k]);
+                k += 1 + (K + 1) + 2 * K + 2;
+            }
+        }
+        stack_words = std::max(stack_words, off + k);
+    }
+    out.words.assign(stacks, stacks + stack_words);
+    return out;
+}
+
 // ---- operator-kind simulation (mirrors run_planar_recursion) ------------------------------------
 enum Kind { KZ = 0, KD = 1, KX = 2 };
 int kmul(int a, int b) { return (a == KZ || b == KZ) ? KZ : (a == KD && b == KD) ? KD : KX; }
@@ -172,7 +220,7 @@ struct PlanarCost {
 };
 
 // general[r]: region r (0 incidence, j+1 layer j, L+1 exit) needs an eigensolve.
-PlanarCost planar_cost(int N, const std::vector<bool>& general, int pol) {
+PlanarCost planar_cost(int N, const std::vector<bool>& general, int pol, bool conical = false) {
     const int L = static_cast<int>(general.size()) - 2;
     const double n = N, n3 = n * n * n;
     const double LU = 8.0 / 3.0 * n3, SOLVE_N = 8.0 * n3, GEMM = 8.0 * n3;
@@ -194,7 +242,7 @@ PlanarCost planar_cost(int N, const std::vector<bool>& general, int pol) {
         } else {
             tdd = tuu = KD;
             rdu = rud = KZ;
-            rdu_w = rud_w = KD;
+            rdu_w = rud_w = conical ? KZ : KD;   // only planar finite layers have the zero-root reflection
         }
         // flops along the expected kinds
         const int k = kmul(rdu, r);
@@ -219,8 +267,15 @@ PlanarCost planar_cost(int N, const std::vector<bool>& general, int pol) {
     }
     for (int j = 1; j <= L; ++j)
         if (general[j]) {
-            c.flops_eig += 100.0 * n3;
-            if (pol == 1) c.flops_other += 2 * LU + 2 * SOLVE_N + GEMM;   // TM assembly + V
+            if (conical) {
+                // two N x N eigensolves (N = half the mode count), 2 LU inverses, B C, C W2, coupling
+                const double h = n / 2, h3 = h * h * h;
+                c.flops_eig += 2 * 100.0 * h3;
+                c.flops_other += 2 * (8.0 / 3.0 * h3) + 2 * 8.0 * h3 + 3 * 8.0 * h3;
+            } else {
+                c.flops_eig += 100.0 * n3;
+                if (pol == 1) c.flops_other += 2 * LU + 2 * SOLVE_N + GEMM;   // TM assembly + V
+            }
         }
     const int64_t NN = int64_t(N) * N;
     c.bump_words = persist + 16 * NN + 64 * int64_t(N) + 4096;
@@ -333,43 +388,9 @@ int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, 
             !theta_deg || !M || !pol)
             invalid("srcwa_plan1d_create: empty or null inputs");
         hip_check(hipSetDevice(ctx->device), "hipSetDevice");
-        // stack record extents
-        int64_t total_words = 0;
-        for (int s = 0; s < n_stacks; ++s) {
-            if (stack_offsets[s] < 0) invalid("rcwa: malformed stack offsets");
-            total_words = std::max<int64_t>(total_words, stack_offsets[s] + 6);
-        }
-        std::vector<int64_t> order(n_stacks);
-        for (int s = 0; s < n_stacks; ++s) order[s] = stack_offsets[s];
-        std::vector<int64_t> sorted = order;
-        std::sort(sorted.begin(), sorted.end());
-        auto record_end = [&](int64_t off) {
-            auto it = std::upper_bound(sorted.begin(), sorted.end(), off);
-            return it == sorted.end() ? int64_t(-1) : *it;
-        };
-        std::vector<std::vector<LayerInfo>> infos(n_stacks);
-        int64_t stack_words = 0;
-        for (int s = 0; s < n_stacks; ++s) {
-            const int64_t off = stack_offsets[s];
-            int64_t end = record_end(off);
-            // the last record: parse with a generous bound, then measure
-            const int64_t avail = end > 0 ? end - off : (int64_t(1) << 40);
-            infos[s] = parse_stack1d(stacks + off, avail, 0);
-            // record length
-            int64_t k = 6;
-            const double* p = stacks + off;
-            for (size_t j = 0; j < infos[s].size(); ++j) {
-                const int kind = static_cast<int>(p[k + 1]);
-                k += 2;
-                if (kind == 0) k += 2;
-                else {
-                    const int K = static_cast<int>(p[k]);
-                    k += 1 + (K + 1) + 2 * K + 2;
-                }
-            }
-            stack_words = std::max(stack_words, off + k);
-        }
-        std::vector<double> h_stacks(stacks, stacks + stack_words);
+        ParsedStacks ps = parse_all(n_stacks, stacks, stack_offsets);
+        std::vector<std::vector<LayerInfo>>& infos = ps.infos;
+        std::vector<double>& h_stacks = ps.words;
 
         auto plan = std::unique_ptr<srcwa_plan>(new srcwa_plan);
         plan->ctx = ctx;
@@ -474,6 +495,136 @@ int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, 
     });
 }
 
+int srcwa_plan_conical_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                              int n_problems, const int* stack_index, const double* wavelength,
+                              const double* theta_deg, const double* phi_deg, const int* M,
+                              int64_t max_workspace_bytes, srcwa_plan** out) {
+    return guarded([&] {
+        if (!ctx || !out) invalid("srcwa_plan_conical_create: null context or output");
+        if (n_stacks < 1 || n_problems < 1 || !stacks || !stack_offsets || !stack_index || !wavelength ||
+            !theta_deg || !phi_deg || !M)
+            invalid("srcwa_plan_conical_create: empty or null inputs");
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        ParsedStacks ps = parse_all(n_stacks, stacks, stack_offsets);
+        auto plan = std::unique_ptr<srcwa_plan>(new srcwa_plan);
+        plan->ctx = ctx;
+        plan->kind = 2;
+        plan->n_inst = n_problems;
+        const int64_t budget = max_workspace_bytes > 0 ? max_workspace_bytes : (int64_t(64) << 30);
+        const int64_t budget_words = budget / int64_t(sizeof(cd));
+        std::vector<Inst1D> h_insts(n_problems);
+        std::vector<LayerTask> h_tasks;
+        std::vector<int64_t> h_layouts;
+        int64_t orders = 0, chunk_ws = 0, max_ws = 0;
+        PlanChunk cur{0, 0, 0, 0};
+        for (int i = 0; i < n_problems; ++i) {
+            const int s = stack_index[i];
+            if (s < 0 || s >= n_stacks) invalid("rcwa: stack index out of range");
+            const double* st = stacks + stack_offsets[s];
+            // rcwa1d_conical.hpp:686-693
+            check_setup(st[0], wavelength[i], st[1], st[2], theta_deg[i], M[i], 0);
+            if (!std::isfinite(phi_deg[i])) invalid("rcwa: azimuth angle must be finite");
+            check_index(st[3], st[4]);
+            if (st[4] != 0.0) invalid("rcwa: conical incidence needs a real index in the exit medium");
+            const std::vector<LayerInfo>& all = ps.infos[s];
+            std::vector<int> eff;                           // layers with d > 0
+            for (int j = 0; j < static_cast<int>(all.size()); ++j)
+                if (all[j].d > 0.0) eff.push_back(j);
+            const int L = static_cast<int>(eff.size());
+            const int N = 2 * M[i] + 1;
+            if (N > EIG_MAX_N_HOST) throw Failure{SRCWA_EUNSUPPORTED, "rcwa: truncation order above this build's limit"};
+            std::vector<bool> general(L + 2, false);
+            for (int j = 0; j < L; ++j) general[j + 1] = all[eff[j]].grating && !all[eff[j]].uniform_values;
+            PlanarCost cost = planar_cost(2 * N, general, 0, true);
+            // degenerate layers borrow four (2N)^2 scratch matrices each from this region
+            int n_general = 0;
+            for (int j = 1; j <= L; ++j) n_general += general[j];
+            cost.bump_words = std::max<int64_t>(cost.bump_words, 16 * int64_t(N) * N * n_general + 64);
+            plan->flops[0] += cost.flops_eig;
+            plan->flops[1] += cost.flops_other;
+            int64_t w = 0;
+            std::vector<int64_t> lay(conical_layout_size(L), -1);
+            lay[0] = w;
+            w += align16(6 * N + 1);
+            for (int r = 0; r < L + 2; ++r) {
+                lay[conical_q(r)] = w;
+                w += align16(2 * N);
+                const bool analytic = r == 0 || r == L + 1 || !general[r];
+                if (analytic) {
+                    lay[conical_w(r)] = w;
+                    w += align16(2 * N);
+                    lay[conical_v(r)] = w;
+                    w += align16(2 * N);
+                }
+            }
+            for (int j = 0; j < L; ++j)
+                if (general[j + 1]) {
+                    lay[conical_slot(L, j)] = w;
+                    w += align16(conical_slot_words(N));
+                }
+            const int64_t ws_lo = w;
+            w += cost.bump_words;
+            if (i > cur.i0 && chunk_ws + w > budget_words) {
+                cur.i1 = i;
+                cur.t1 = static_cast<int>(h_tasks.size());
+                plan->chunks.push_back(cur);
+                max_ws = std::max(max_ws, chunk_ws);
+                cur = PlanChunk{i, i, cur.t1, cur.t1};
+                chunk_ws = 0;
+            }
+            const int first_task = static_cast<int>(h_tasks.size());
+            for (int j = 0; j < L; ++j)
+                if (general[j + 1]) h_tasks.push_back(LayerTask{i - cur.i0, j});
+            const int64_t layout_off = static_cast<int64_t>(h_layouts.size());
+            for (int r = 0; r < conical_layout_size(L); ++r) {
+                int64_t v = lay[r];
+                if (r >= conical_orig(L, 0) && L > 0) v = eff[r - conical_orig(L, 0)];
+                else if (v >= 0) v += chunk_ws;
+                h_layouts.push_back(v);
+            }
+            Inst1D& in = h_insts[i];
+            in.M = M[i];
+            in.N = N;
+            in.pol = 2;
+            in.L = L;
+            in.first_task = first_task - cur.t0;
+            in.n_tasks = static_cast<int>(h_tasks.size()) - first_task;
+            in.wavelength = wavelength[i];
+            in.theta_deg = theta_deg[i];
+            in.phi_deg = phi_deg[i];
+            in.stack_off = stack_offsets[s];
+            in.layout_off = layout_off;
+            in.out_off = orders;
+            in.ws_lo = chunk_ws + ws_lo;
+            in.ws_hi = chunk_ws + w;
+            chunk_ws += w;
+            orders += N;
+        }
+        cur.i1 = n_problems;
+        cur.t1 = static_cast<int>(h_tasks.size());
+        plan->chunks.push_back(cur);
+        max_ws = std::max(max_ws, chunk_ws);
+        plan->n_tasks = static_cast<int>(h_tasks.size());
+        plan->total_orders = orders;
+        hipStream_t st = ctx->stream;
+        plan->insts.upload(h_insts, st);
+        plan->tasks.upload(h_tasks, st);
+        plan->stacks.upload(ps.words, st);
+        plan->layouts.upload(h_layouts, st);
+        plan->ws.alloc(static_cast<size_t>(max_ws));
+        plan->task_info.alloc(4 * std::max(1, plan->n_tasks));
+        plan->R.alloc(8 * orders);
+        plan->T.alloc(8 * orders);
+        plan->DEr.alloc(4 * orders);
+        plan->DEt.alloc(4 * orders);
+        plan->status.alloc(n_problems);
+        plan->eig_info.alloc(n_problems);
+        plan->make_events();
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        *out = plan.release();
+    });
+}
+
 int srcwa_plan_execute(srcwa_plan* plan) {
     return guarded([&] {
         if (!plan) invalid("null plan");
@@ -482,8 +633,14 @@ int srcwa_plan_execute(srcwa_plan* plan) {
         for (size_t c = 0; c < plan->chunks.size(); ++c) {
             const PlanChunk& ch = plan->chunks[c];
             Out1D out{plan->R.p, plan->T.p, plan->DEr.p, plan->DEt.p, plan->status.p + ch.i0, plan->eig_info.p + ch.i0};
-            launch_planar(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p,
-                          plan->layouts.p, plan->ws.p, plan->task_info.p + ch.t0, out, 30, plan->ev.data() + 4 * c);
+            if (plan->kind == 1)
+                launch_planar(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts.p + ch.i0, plan->tasks.p + ch.t0,
+                              plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + ch.t0, out, 30,
+                              plan->ev.data() + 4 * c);
+            else
+                launch_conical(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts.p + ch.i0, plan->tasks.p + ch.t0,
+                               plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + 4 * ch.t0, out, 30,
+                               plan->ev.data() + 4 * c);
             hip_check(hipGetLastError(), "kernel launch");
         }
         plan->timed = true;
@@ -502,7 +659,7 @@ int srcwa_plan_results(srcwa_plan* plan, double* R, double* T, double* DEr, doub
         if (!plan) invalid("null plan");
         hipStream_t s = plan->ctx->stream;
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
-        const int64_t o = plan->total_orders;
+        const int64_t o = plan->total_orders * (plan->kind == 2 ? 4 : 1);
         if (R) hip_check(hipMemcpy(R, plan->R.p, 2 * o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
         if (T) hip_check(hipMemcpy(T, plan->T.p, 2 * o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
         if (DEr) hip_check(hipMemcpy(DEr, plan->DEr.p, o * sizeof(double), hipMemcpyDeviceToHost), "D2H");

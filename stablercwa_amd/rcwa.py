@@ -22,7 +22,10 @@ from .types import (ConicalResult, FourierMethod, Jones, Layer, Layer2D, Polariz
                     Stack, Stack2D, Status2D, XProfile2D, jones_from_angles_2d)
 
 __all__ = ["solve", "sweep", "Point", "PlanarBatch", "Layer", "Stack", "Polarization", "FourierMethod",
-           "Result", "zgeev_batched", "zgesv_batched", "zgemm_batched"]
+           "Result", "solve_conical", "sweep_conical", "ConicalPoint", "ConicalBatch", "ConicalResult",
+           "zgeev_batched", "zgesv_batched", "zgemm_batched"]
+
+ST_OK, ST_NUMERICAL, ST_WORKSPACE, ST_RAYLEIGH, ST_SCHUR = 0, 1, 2, 3, 4
 
 
 @dataclass
@@ -130,6 +133,77 @@ def sweep(stack: Stack, points: Sequence[Point], M: int, pol: Polarization = Pol
           ctx: _lib.Context | None = None) -> List[Result]:
     """Independent solves over points, one batched launch (rcwa::sweep, rcwa1d.hpp:1084-1091)."""
     b = PlanarBatch([stack], [(0, p.wavelength, p.theta_deg, M, pol) for p in points], ctx)
+    b.execute()
+    res = b.results()
+    out = [b.result(i, res) for i in range(len(points))]
+    b.close()
+    return out
+
+
+@dataclass
+class ConicalPoint:
+    wavelength: float = 1.0
+    theta_deg: float = 0.0
+    phi_deg: float = 0.0
+
+
+class ConicalBatch(PlanarBatch):
+    """A device-resident batch of conical solves; problems are (stack_index, wavelength, theta, phi, M)."""
+
+    def __init__(self, stacks: Sequence[Stack], problems, ctx: _lib.Context | None = None,
+                 max_workspace_bytes: int = 0):
+        self.ctx = ctx or _lib.default_context()
+        self.lib = _lib.load()
+        self._flat, self._offsets = _pack_stacks(stacks)
+        P = len(problems)
+        self.stack_index = np.array([p[0] for p in problems], dtype=np.int32)
+        self.wavelength = np.array([p[1] for p in problems], dtype=np.float64)
+        self.theta = np.array([p[2] for p in problems], dtype=np.float64)
+        self.phi = np.array([p[3] for p in problems], dtype=np.float64)
+        self.M = np.array([p[4] for p in problems], dtype=np.int32)
+        self.N = 2 * self.M + 1
+        self.order_offsets = np.concatenate([[0], np.cumsum(self.N)]).astype(np.int64)
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.srcwa_plan_conical_create(
+            self.ctx.handle, len(stacks), _lib.ptr(self._flat), _lib.ptr(self._offsets), P,
+            _lib.ptr(self.stack_index), _lib.ptr(self.wavelength), _lib.ptr(self.theta), _lib.ptr(self.phi),
+            _lib.ptr(self.M), int(max_workspace_bytes), ctypes.byref(h)))
+        self.handle = h
+        self.n = P
+
+    def results(self):
+        o = 4 * int(self.order_offsets[-1])
+        R = np.empty(2 * o); T = np.empty(2 * o); DEr = np.empty(o); DEt = np.empty(o)
+        st = np.empty(self.n, dtype=np.int32); info = np.empty(self.n, dtype=np.int32)
+        _lib.check(self.lib.srcwa_plan_results(self.handle, _lib.ptr(R), _lib.ptr(T), _lib.ptr(DEr), _lib.ptr(DEt),
+                                               _lib.ptr(st), _lib.ptr(info)))
+        return dict(Jr=R.view(np.complex128).reshape(-1, 4), Jt=T.view(np.complex128).reshape(-1, 4),
+                    DEr=DEr.reshape(-1, 4), DEt=DEt.reshape(-1, 4), status=st, eig_info=info)
+
+    def result(self, i: int, res=None) -> ConicalResult:
+        res = res or self.results()
+        a, b = int(self.order_offsets[i]), int(self.order_offsets[i + 1])
+        M = int(self.M[i])
+        return ConicalResult(orders=np.arange(-M, M + 1), kx=None, ky=None, kzI=None, kzII=None, flux_r=None,
+                             flux_t=None, Jr=res["Jr"][a:b].copy(), Jt=res["Jt"][a:b].copy(),
+                             DEr=res["DEr"][a:b].copy(), DEt=res["DEt"][a:b].copy(),
+                             eig_info=int(res["eig_info"][i]))
+
+
+def solve_conical(stack: Stack, wavelength: float, theta_deg: float, phi_deg: float, M: int,
+                  ctx: _lib.Context | None = None) -> ConicalResult:
+    """One conical solve (rcwa::solve_conical, rcwa1d_conical.hpp:985)."""
+    b = ConicalBatch([stack], [(0, wavelength, theta_deg, phi_deg, M)], ctx)
+    b.execute()
+    r = b.result(0)
+    b.close()
+    return r
+
+
+def sweep_conical(stack: Stack, points: Sequence[ConicalPoint], M: int,
+                  ctx: _lib.Context | None = None) -> List[ConicalResult]:
+    """Independent conical solves over points in one batched launch (rcwa1d_conical.hpp:992-999)."""
+    b = ConicalBatch([stack], [(0, p.wavelength, p.theta_deg, p.phi_deg, M) for p in points], ctx)
     b.execute()
     res = b.results()
     out = [b.result(i, res) for i in range(len(points))]
