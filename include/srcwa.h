@@ -73,6 +73,25 @@ int srcwa_plan_conical_create(srcwa_context* ctx, int n_stacks, const double* st
                               const double* theta_deg, const double* phi_deg, const int* M,
                               int64_t max_workspace_bytes, srcwa_plan** out);
 
+/* Batched crossed-grating (2-D) solves.  Replaces rcwa::solve2d (rcwa2d.hpp:1042-1411) over many
+ * (stack, wavelength, theta, phi, M1, M2, jones, factorization) problems.
+ *
+ * stacks: flat crossed-grating records (Lx, Ly, nI, nII.re, nII.im, L, then per layer d, kind,
+ *   uniform: n.re, n.im | patterned: Ky, y_edges[Ky+1], per y segment Kx, x_edges[Kx+1], (n.re, n.im)[Kx];
+ *   reference Stack2D, rcwa2d_types.hpp:39-63).  Patterned layers hold at most 256 y segments.
+ * jones: 4 doubles per problem, [Es.re, Es.im, Ep.re, Ep.im] (rcwa2d_types.hpp:97-107).
+ * factorization: NULL or one int per problem, 0 = Li (default), 1 = Laurent (Factorization2D).
+ * Per problem N = (2 M1 + 1)(2 M2 + 1) orders, harmonic i = (m + M1)(2 M2 + 1) + (n + M2).
+ * Results (srcwa_plan_results): R = [Rs, Rp, Ex_r, Ey_r], T = [Ts, Tp, Ex_t, Ey_t] (N complex each) at
+ * 4 x the order offset;
+ * DEr, DEt (N) at the order offset; status 0 ok, 1 numerical failure, 3 Rayleigh anomaly (Status2D).
+ * Invalid inputs return SRCWA_EINVAL with the reference's std::invalid_argument message
+ * (check_core_inputs / check_layer_core / check_patterned, rcwa2d.hpp:953-1026). */
+int srcwa_plan2d_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                        int n_problems, const int* stack_index, const double* wavelength, const double* theta_deg,
+                        const double* phi_deg, const int* M1, const int* M2, const double* jones,
+                        const int* factorization, int64_t max_workspace_bytes, srcwa_plan** out);
+
 /* Launch all chunks of the plan on the context stream (asynchronous). */
 int srcwa_plan_execute(srcwa_plan* plan);
 /* Wait for the last execute. */

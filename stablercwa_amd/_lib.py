@@ -32,6 +32,8 @@ PROTOTYPES = {
                                            ctypes.c_int64, ctypes.POINTER(_vp)]),
     "srcwa_plan_conical_create": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp,
                                                  _vp, ctypes.c_int64, ctypes.POINTER(_vp)]),
+    "srcwa_plan2d_create": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
+                                           _vp, _vp, _vp, ctypes.c_int64, ctypes.POINTER(_vp)]),
     "srcwa_plan_execute": (ctypes.c_int, [_vp]),
     "srcwa_plan_sync": (ctypes.c_int, [_vp]),
     "srcwa_plan_results": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),

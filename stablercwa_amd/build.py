@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libsrcwa_amd.so")
 
-SOURCES = ["rcwa1d.hip", "conical.hip", "dense.hip", "runtime.cpp"]
+SOURCES = ["rcwa1d.hip", "conical.hip", "rcwa2d.hip", "dense.hip", "runtime.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

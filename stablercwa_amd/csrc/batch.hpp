@@ -57,6 +57,31 @@ __host__ __device__ inline int conical_orig(int L, int j) { return 1 + 3 * (L + 
 __host__ __device__ inline int conical_layout_size(int L) { return 1 + 3 * (L + 2) + 2 * L; }
 __host__ __device__ inline int64_t conical_slot_words(int64_t N) { return 16 * N * N + 7 * N; }
 
+// Layout table of a crossed-grating instance (N harmonics, n = 2N modes):
+//   [0]                      geometry: Kx, Ky (real, N each), kzI, kzII (N each), Rayleigh flag (1)
+//   [1 + 2r], [2 + 2r]       q (2N) and 2x2 block-diagonal V (a11, a12, a21, a22: 4N) of region r;
+//                            the V entry is -1 for patterned layers
+//   [1 + 2(L+2) + j]         slot base of patterned layer j (-1 for a uniform layer)
+//   [1 + 2(L+2) + L + j]     offset (doubles, in the stack buffer) of layer j's x-cut record [Kc, cuts]
+//   [1 + 2(L+2) + 2L + j]    size (cd) of layer j's big scratch region (H, Z, X / operator scratch)
+// Slots of a patterned layer: EPS, CX, CY (N x N), BIG, then lambda, ort, tmp (2N each), pivots (N).
+__host__ __device__ inline int x2_q(int r) { return 1 + 2 * r; }
+__host__ __device__ inline int x2_v(int r) { return 2 + 2 * r; }
+__host__ __device__ inline int x2_slot(int L, int j) { return 1 + 2 * (L + 2) + j; }
+__host__ __device__ inline int x2_cuts(int L, int j) { return 1 + 2 * (L + 2) + L + j; }
+__host__ __device__ inline int x2_big(int L, int j) { return 1 + 2 * (L + 2) + 2 * L + j; }
+__host__ __device__ inline int x2_layout_size(int L) { return 1 + 2 * (L + 2) + 3 * L; }
+
+// One crossed-grating instance.
+struct Inst2D {
+    int M1, M2, N, L;
+    int first_task, n_tasks;
+    int laurent;               // 0: Li factorisation, 1: Laurent rule
+    double wavelength, theta_deg, phi_deg;
+    double jones[4];           // Es.re, Es.im, Ep.re, Ep.im
+    int64_t stack_off, layout_off, out_off, ws_lo, ws_hi;
+};
+
 // Per-instance results.
 struct Out1D {
     double* R;      // complex, N per instance (planar) / 4N Jones (conical)

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <complex>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -21,6 +22,9 @@ void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, 
                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
                    hipEvent_t* ev);
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
+                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
+                    hipEvent_t* ev);
+void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
                     hipEvent_t* ev);
 void launch_zgemm_batched(hipStream_t s, int count, const DenseItem* items, cd* arena);
@@ -301,6 +305,7 @@ struct srcwa_plan {
     int64_t total_orders = 0;
     std::vector<PlanChunk> chunks;
     DevBuf<Inst1D> insts;
+    DevBuf<Inst2D> insts2d;
     DevBuf<LayerTask> tasks;
     DevBuf<double> stacks;
     DevBuf<int64_t> layouts;
@@ -320,7 +325,7 @@ struct srcwa_plan {
         for (hipEvent_t& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
     }
     int64_t bytes() const {
-        return insts.bytes() + tasks.bytes() + stacks.bytes() + layouts.bytes() + ws.bytes() + task_info.bytes() +
+        return insts.bytes() + insts2d.bytes() + tasks.bytes() + stacks.bytes() + layouts.bytes() + ws.bytes() + task_info.bytes() +
                R.bytes() + T.bytes() + DEr.bytes() + DEt.bytes() + status.bytes() + eig_info.bytes();
     }
 };
@@ -637,8 +642,12 @@ int srcwa_plan_execute(srcwa_plan* plan) {
                 launch_planar(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts.p + ch.i0, plan->tasks.p + ch.t0,
                               plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + ch.t0, out, 30,
                               plan->ev.data() + 4 * c);
-            else
+            else if (plan->kind == 2)
                 launch_conical(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts.p + ch.i0, plan->tasks.p + ch.t0,
+                               plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + 4 * ch.t0, out, 30,
+                               plan->ev.data() + 4 * c);
+            else
+                launch_crossed(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts2d.p + ch.i0, plan->tasks.p + ch.t0,
                                plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + 4 * ch.t0, out, 30,
                                plan->ev.data() + 4 * c);
             hip_check(hipGetLastError(), "kernel launch");
@@ -660,6 +669,12 @@ int srcwa_plan_results(srcwa_plan* plan, double* R, double* T, double* DEr, doub
         hipStream_t s = plan->ctx->stream;
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
         const int64_t o = plan->total_orders * (plan->kind == 2 ? 4 : 1);
+        if (plan->kind == 3) {
+            // crossed gratings: R = [Rs, Rp, Ex_r, Ey_r], T = [Ts, Tp, Ex_t, Ey_t] per instance (4N complex)
+            if (R) hip_check(hipMemcpy(R, plan->R.p, 8 * o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+            if (T) hip_check(hipMemcpy(T, plan->T.p, 8 * o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+            R = T = nullptr;
+        }
         if (R) hip_check(hipMemcpy(R, plan->R.p, 2 * o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
         if (T) hip_check(hipMemcpy(T, plan->T.p, 2 * o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
         if (DEr) hip_check(hipMemcpy(DEr, plan->DEr.p, o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
@@ -864,3 +879,337 @@ int srcwa_zgeev_batched(srcwa_context* ctx, int count, const int* n, const doubl
 }
 
 }  // extern "C"
+
+// ---- crossed gratings (rcwa2d.hpp:953-1026, rcwa2d_operators.hpp:244-320) ------------------------
+namespace {
+
+void check_index_2d(double re, double im) {
+    if (!finite_c(re, im) || std::hypot(re, im) == 0.0 || re < 0.0 || im < 0.0)
+        invalid("rcwa2d: indices must be finite, nonzero, with Re n >= 0 and Im n >= 0");
+}
+
+void check_edge_list(const double* e, int count, double period, const char* what) {
+    if (count < 2) invalid(std::string("rcwa2d: ") + what + " need at least two edges");
+    for (int j = 0; j < count; ++j)
+        if (!std::isfinite(e[j]) || (j > 0 && e[j] <= e[j - 1]))
+            invalid(std::string("rcwa2d: ") + what + " must be finite and strictly increasing");
+    if (std::abs((e[count - 1] - e[0]) - period) > 1e-9 * period)
+        invalid(std::string("rcwa2d: ") + what + " must span exactly one period");
+}
+
+double floor_mod_host(double a, double b) {
+    double m = std::fmod(a, b);
+    if (m != 0.0) {
+        if ((b < 0.0) != (m < 0.0)) m += b;
+    } else {
+        m = std::copysign(0.0, b);
+    }
+    return m;
+}
+
+struct Layer2DInfo {
+    bool patterned;
+    int K = 0, maxkx = 1;
+    std::vector<double> cuts;     // merged x cuts (patterned layers)
+};
+
+// Parses and validates one crossed-grating stack record; returns per-layer info and the record length.
+std::vector<Layer2DInfo> parse_stack2d(const double* p, int64_t avail, int64_t& words) {
+    if (avail < 6) invalid("rcwa2d: malformed stack record");
+    const double Lx = p[0], Ly = p[1];
+    if (!(std::isfinite(Lx) && Lx > 0.0) || !(std::isfinite(Ly) && Ly > 0.0))
+        invalid("rcwa2d: wavelength and periods must be finite and positive");
+    if (!std::isfinite(p[2]) || p[2] <= 0.0) invalid("rcwa2d: the incident medium must have a real positive index");
+    check_index_2d(p[3], p[4]);
+    const int L = static_cast<int>(p[5]);
+    if (!(p[5] >= 0.0) || L > 128) invalid("rcwa2d: a stack holds 0..128 layers");
+    std::vector<Layer2DInfo> out;
+    int64_t k = 6;
+    auto need = [&](int64_t n) {
+        if (k + n > avail) invalid("rcwa2d: malformed stack record");
+    };
+    for (int j = 0; j < L; ++j) {
+        need(2);
+        const double d = p[k];
+        const int kind = static_cast<int>(p[k + 1]);
+        k += 2;
+        if (!std::isfinite(d) || d < 0.0) invalid("rcwa2d: layer thickness must be finite and >= 0");
+        Layer2DInfo info;
+        if (kind == 0) {
+            need(2);
+            check_index_2d(p[k], p[k + 1]);
+            k += 2;
+            info.patterned = false;
+        } else if (kind == 1) {
+            need(1);
+            const int K = static_cast<int>(p[k]);
+            if (K < 1 || K > 256) invalid("rcwa2d: a patterned layer needs 1..256 y segments");
+            need(1 + K + 1);
+            check_edge_list(p + k + 1, K + 1, Ly, "y-edges");
+            k += 1 + K + 1;
+            info.patterned = true;
+            info.K = K;
+            std::vector<double> cuts;
+            for (int y = 0; y < K; ++y) {
+                need(1);
+                const int Kx = static_cast<int>(p[k]);
+                if (Kx < 1 || Kx > 4096) invalid("rcwa2d: an x-profile needs K indices and K+1 edges");
+                need(1 + (Kx + 1) + 2 * Kx);
+                const double* xe = p + k + 1;
+                const double* v = xe + Kx + 1;
+                check_edge_list(xe, Kx + 1, Lx, "x-edges");
+                for (int i = 0; i < Kx; ++i) {
+                    check_index_2d(v[2 * i], v[2 * i + 1]);
+                    const std::complex<double> n{v[2 * i], v[2 * i + 1]};
+                    const std::complex<double> eps = n * n;
+                    if (!finite_c(eps.real(), eps.imag()) || eps == std::complex<double>{})
+                        invalid("rcwa2d: eps = n^2 overflows or vanishes in a patterned layer");
+                    const std::complex<double> inv = 1.0 / eps;
+                    if (!finite_c(inv.real(), inv.imag())) invalid("rcwa2d: 1/eps overflows in a patterned layer");
+                }
+                for (int i = 0; i <= Kx; ++i) cuts.push_back(floor_mod_host(xe[i], Lx));
+                info.maxkx = std::max(info.maxkx, Kx);
+                k += 1 + (Kx + 1) + 2 * Kx;
+            }
+            // merge_x_cuts (rcwa2d_operators.hpp:160-174)
+            cuts.push_back(0.0);
+            cuts.push_back(Lx);
+            std::sort(cuts.begin(), cuts.end());
+            cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+            for (double& c : cuts) c = std::rint(c * 1e12) / 1e12;
+            std::sort(cuts.begin(), cuts.end());
+            cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+            cuts.erase(std::remove_if(cuts.begin(), cuts.end(), [&](double c) { return !(c >= -1e-12 && c <= Lx + 1e-12); }),
+                       cuts.end());
+            if (cuts.empty() || cuts.front() > 1e-12) cuts.insert(cuts.begin(), 0.0);
+            if (cuts.back() < Lx - 1e-12) cuts.push_back(Lx);
+            info.cuts = std::move(cuts);
+        } else {
+            invalid("rcwa2d: unknown layer kind");
+        }
+        out.push_back(std::move(info));
+    }
+    words = k;
+    return out;
+}
+
+// cd words of k_ops2d's scratch (same layout as the kernel)
+int64_t ops2d_scratch_words(int K, int Kc, int maxkx, int M1, int M2) {
+    const int64_t n1 = 2 * M1 + 1, n2 = 2 * M2 + 1, n1s = n1 * n1, n2s = n2 * n2;
+    int64_t w = int64_t(K) * (4 * M2 + 1) + 2 * int64_t(K) * n1s + int64_t(Kc) * (4 * M1 + 1) + int64_t(Kc) * n2s;
+    w += std::max(int64_t(K) * n1s, int64_t(Kc) * n2s);
+    w += std::max(int64_t(4 * M2 + 1) * n1s, int64_t(4 * M1 + 1) * n2s);
+    const int64_t coefsz = std::max<int64_t>(maxkx + 4 * M1 + 1, K + 4 * M2 + 1);
+    w += coefsz * std::max(K, Kc);
+    w += (std::max(int64_t(K) * n1, int64_t(Kc) * n2) + 3) / 4 + 16;
+    return w;
+}
+
+enum RKind { RZ, RB, RD };
+
+// Workspace of the crossed-grating recursion and its flops, along the structural kinds.
+PlanarCost crossed_cost(int N, const std::vector<bool>& patterned) {
+    const int L = static_cast<int>(patterned.size()) - 2;
+    const double n = 2.0 * N, n3 = n * n * n, h3 = double(N) * N * N;
+    const int64_t nn = 4 * int64_t(N) * N, b4 = 4 * int64_t(N);
+    PlanarCost c{0, 0.0, 0.0};
+    int64_t persist = 2 * align16(nn);
+    int r = RZ;
+    const double LU = 8.0 / 3.0 * n3, SOLVE_N = 8.0 * n3, GEMM = 8.0 * n3;
+    for (int b = 2 * L; b >= 1; --b) {
+        const int j = b / 2;
+        if (b % 2 == 0) {
+            const bool homog = !patterned[j] && !patterned[j + 1];
+            if (homog) {
+                if (r != RD) {
+                    persist += align16(b4);
+                    r = RB;
+                } else {
+                    persist += 2 * align16(b4) + align16(nn);
+                    c.flops_other += LU + SOLVE_N;
+                    r = RD;
+                }
+            } else {
+                persist += 3 * align16(nn);
+                if (patterned[j + 1]) c.flops_other += 2 * (LU + SOLVE_N);
+                if (r == RD) c.flops_other += 2 * GEMM;
+                c.flops_other += LU + SOLVE_N;
+                r = RD;
+            }
+        } else {
+            persist += align16(b4);
+            if (r != RD) r = RB;
+        }
+    }
+    if (patterned[1]) c.flops_other += LU + 8.0 * n * n + (r == RD ? 2 * GEMM : 0.0);
+    for (int j = 1; j <= L; ++j)
+        if (patterned[j]) {
+            c.flops_eig += 100.0 * n3;
+            c.flops_other += 8.0 / 3.0 * h3 + 8.0 * h3 + 32.0 * h3 + 32.0 * h3;   // Ei, H, V
+        }
+    c.bump_words = persist + 10 * nn + 64 * int64_t(N) + 4096;
+    return c;
+}
+
+}  // namespace
+
+extern "C" int srcwa_plan2d_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                                   int n_problems, const int* stack_index, const double* wavelength,
+                                   const double* theta_deg, const double* phi_deg, const int* M1, const int* M2,
+                                   const double* jones, const int* factorization, int64_t max_workspace_bytes,
+                                   srcwa_plan** out) {
+    return guarded([&] {
+        if (!ctx || !out) invalid("srcwa_plan2d_create: null context or output");
+        if (n_stacks < 1 || n_problems < 1 || !stacks || !stack_offsets || !stack_index || !wavelength ||
+            !theta_deg || !phi_deg || !M1 || !M2 || !jones)
+            invalid("srcwa_plan2d_create: empty or null inputs");
+        hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+        std::vector<int64_t> sorted(stack_offsets, stack_offsets + n_stacks);
+        for (int64_t o : sorted)
+            if (o < 0) invalid("rcwa2d: malformed stack offsets");
+        std::sort(sorted.begin(), sorted.end());
+        std::vector<std::vector<Layer2DInfo>> infos(n_stacks);
+        int64_t stack_words = 0;
+        for (int s = 0; s < n_stacks; ++s) {
+            const int64_t off = stack_offsets[s];
+            auto it = std::upper_bound(sorted.begin(), sorted.end(), off);
+            const int64_t avail = it == sorted.end() ? (int64_t(1) << 40) : *it - off;
+            int64_t words = 0;
+            infos[s] = parse_stack2d(stacks + off, avail, words);
+            stack_words = std::max(stack_words, off + words);
+        }
+        // stack records, then every patterned layer's x-cut record [Kc, cuts]
+        std::vector<double> h_stacks(stacks, stacks + stack_words);
+        std::vector<std::vector<int64_t>> cut_off(n_stacks);
+        for (int s = 0; s < n_stacks; ++s) {
+            cut_off[s].assign(infos[s].size(), -1);
+            for (size_t j = 0; j < infos[s].size(); ++j)
+                if (infos[s][j].patterned) {
+                    cut_off[s][j] = static_cast<int64_t>(h_stacks.size());
+                    h_stacks.push_back(static_cast<double>(infos[s][j].cuts.size() - 1));
+                    for (double c : infos[s][j].cuts) h_stacks.push_back(c);
+                }
+        }
+        auto plan = std::unique_ptr<srcwa_plan>(new srcwa_plan);
+        plan->ctx = ctx;
+        plan->kind = 3;
+        plan->n_inst = n_problems;
+        const int64_t budget = max_workspace_bytes > 0 ? max_workspace_bytes : (int64_t(64) << 30);
+        const int64_t budget_words = budget / int64_t(sizeof(cd));
+        std::vector<Inst2D> h_insts(n_problems);
+        std::vector<LayerTask> h_tasks;
+        std::vector<int64_t> h_layouts;
+        int64_t orders = 0, chunk_ws = 0, max_ws = 0;
+        PlanChunk cur{0, 0, 0, 0};
+        const int64_t limit = (std::numeric_limits<int>::max() - 1) / 4;
+        for (int i = 0; i < n_problems; ++i) {
+            const int s = stack_index[i];
+            if (s < 0 || s >= n_stacks) invalid("rcwa2d: stack index out of range");
+            if (M1[i] < 0 || M2[i] < 0 || (2LL * M1[i] + 1) * (2LL * M2[i] + 1) > limit)
+                invalid("rcwa2d: invalid truncation order M1, M2");
+            if (!std::isfinite(wavelength[i]) || wavelength[i] <= 0.0)
+                invalid("rcwa2d: wavelength and periods must be finite and positive");
+            if (!std::isfinite(theta_deg[i]) || std::abs(theta_deg[i]) >= 90.0)
+                invalid("rcwa2d: incidence angle must lie strictly between -90 and 90 degrees");
+            if (!std::isfinite(phi_deg[i])) invalid("rcwa2d: azimuth angle must be finite");
+            for (int q = 0; q < 4; ++q)
+                if (!std::isfinite(jones[4 * i + q])) invalid("rcwa2d: the Jones vector must be finite");
+            const int fz = factorization ? factorization[i] : 0;
+            if (fz != 0 && fz != 1) invalid("rcwa2d: unknown factorization");
+            const std::vector<Layer2DInfo>& li = infos[s];
+            const int L = static_cast<int>(li.size());
+            const int N = (2 * M1[i] + 1) * (2 * M2[i] + 1);
+            if (2 * N > EIG_MAX_N_HOST)
+                throw Failure{SRCWA_EUNSUPPORTED, "rcwa2d: harmonic count above this build's limit"};
+            std::vector<bool> patterned(L + 2, false);
+            for (int j = 0; j < L; ++j) patterned[j + 1] = li[j].patterned;
+            const PlanarCost cost = crossed_cost(N, patterned);
+            plan->flops[0] += cost.flops_eig;
+            plan->flops[1] += cost.flops_other;
+            const int64_t NN = int64_t(N) * N;
+            int64_t w = 0;
+            std::vector<int64_t> lay(x2_layout_size(L), -1);
+            lay[0] = w;
+            w += align16(4 * N + 1);
+            for (int r = 0; r < L + 2; ++r) {
+                lay[x2_q(r)] = w;
+                w += align16(2 * N);
+                if (!patterned[r]) {
+                    lay[x2_v(r)] = w;
+                    w += align16(4 * N);
+                }
+            }
+            for (int j = 0; j < L; ++j)
+                if (li[j].patterned) {
+                    const int Kc = static_cast<int>(li[j].cuts.size()) - 1;
+                    const int64_t big = align16(std::max<int64_t>(12 * NN, ops2d_scratch_words(li[j].K, Kc, li[j].maxkx,
+                                                                                              M1[i], M2[i])));
+                    lay[x2_slot(L, j)] = w;
+                    lay[x2_big(L, j)] = big;
+                    lay[x2_cuts(L, j)] = cut_off[s][j];
+                    w += align16(3 * NN + big + 7 * int64_t(N));
+                }
+            const int64_t ws_lo = w;
+            w += cost.bump_words;
+            if (i > cur.i0 && chunk_ws + w > budget_words) {
+                cur.i1 = i;
+                cur.t1 = static_cast<int>(h_tasks.size());
+                plan->chunks.push_back(cur);
+                max_ws = std::max(max_ws, chunk_ws);
+                cur = PlanChunk{i, i, cur.t1, cur.t1};
+                chunk_ws = 0;
+            }
+            const int first_task = static_cast<int>(h_tasks.size());
+            for (int j = 0; j < L; ++j)
+                if (li[j].patterned) h_tasks.push_back(LayerTask{i - cur.i0, j});
+            const int64_t layout_off = static_cast<int64_t>(h_layouts.size());
+            for (int r = 0; r < x2_layout_size(L); ++r) {
+                int64_t v = lay[r];
+                const bool is_offset = r < x2_cuts(L, 0) || L == 0;
+                if (is_offset && v >= 0) v += chunk_ws;
+                h_layouts.push_back(v);
+            }
+            Inst2D& in = h_insts[i];
+            in.M1 = M1[i];
+            in.M2 = M2[i];
+            in.N = N;
+            in.L = L;
+            in.first_task = first_task - cur.t0;
+            in.n_tasks = static_cast<int>(h_tasks.size()) - first_task;
+            in.laurent = fz;
+            in.wavelength = wavelength[i];
+            in.theta_deg = theta_deg[i];
+            in.phi_deg = phi_deg[i];
+            for (int q = 0; q < 4; ++q) in.jones[q] = jones[4 * i + q];
+            in.stack_off = stack_offsets[s];
+            in.layout_off = layout_off;
+            in.out_off = orders;
+            in.ws_lo = chunk_ws + ws_lo;
+            in.ws_hi = chunk_ws + w;
+            chunk_ws += w;
+            orders += N;
+        }
+        cur.i1 = n_problems;
+        cur.t1 = static_cast<int>(h_tasks.size());
+        plan->chunks.push_back(cur);
+        max_ws = std::max(max_ws, chunk_ws);
+        plan->n_tasks = static_cast<int>(h_tasks.size());
+        plan->total_orders = orders;
+        hipStream_t st = ctx->stream;
+        plan->insts2d.upload(h_insts, st);
+        plan->tasks.upload(h_tasks, st);
+        plan->stacks.upload(h_stacks, st);
+        plan->layouts.upload(h_layouts, st);
+        plan->ws.alloc(static_cast<size_t>(max_ws));
+        plan->task_info.alloc(4 * std::max(1, plan->n_tasks));
+        plan->R.alloc(8 * orders);
+        plan->T.alloc(8 * orders);
+        plan->DEr.alloc(orders);
+        plan->DEt.alloc(orders);
+        plan->status.alloc(n_problems);
+        plan->eig_info.alloc(n_problems);
+        plan->make_events();
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        *out = plan.release();
+    });
+}

@@ -23,7 +23,8 @@ from .types import (ConicalResult, FourierMethod, Jones, Layer, Layer2D, Polariz
 
 __all__ = ["solve", "sweep", "Point", "PlanarBatch", "Layer", "Stack", "Polarization", "FourierMethod",
            "Result", "solve_conical", "sweep_conical", "ConicalPoint", "ConicalBatch", "ConicalResult",
-           "zgeev_batched", "zgesv_batched", "zgemm_batched"]
+           "solve2d", "CrossedBatch", "Stack2D", "Layer2D", "XProfile2D", "Result2D", "Status2D",
+           "jones_from_angles_2d", "zgeev_batched", "zgesv_batched", "zgemm_batched"]
 
 ST_OK, ST_NUMERICAL, ST_WORKSPACE, ST_RAYLEIGH, ST_SCHUR = 0, 1, 2, 3, 4
 
@@ -209,6 +210,85 @@ def sweep_conical(stack: Stack, points: Sequence[ConicalPoint], M: int,
     out = [b.result(i, res) for i in range(len(points))]
     b.close()
     return out
+
+
+# ---- crossed gratings ------------------------------------------------------------------------------
+_STATUS2D = {ST_OK: Status2D.Ok, ST_NUMERICAL: Status2D.NumericalFailure, ST_RAYLEIGH: Status2D.RayleighAnomaly}
+
+
+class CrossedBatch(PlanarBatch):
+    """A device-resident batch of crossed-grating solves.
+
+    problems: sequence of (stack_index, wavelength, theta_deg, phi_deg, M1, M2, jones[, factorization]) with
+    jones = (Es, Ep) complex and factorization "li" (default) or "laurent" (rcwa2d_types.hpp:27-29)."""
+
+    def __init__(self, stacks: Sequence[Stack2D], problems, ctx: _lib.Context | None = None,
+                 max_workspace_bytes: int = 0):
+        self.ctx = ctx or _lib.default_context()
+        self.lib = _lib.load()
+        self._flat, self._offsets = _pack_stacks(stacks)
+        P = len(problems)
+        self.stack_index = np.array([p[0] for p in problems], dtype=np.int32)
+        self.wavelength = np.array([p[1] for p in problems], dtype=np.float64)
+        self.theta = np.array([p[2] for p in problems], dtype=np.float64)
+        self.phi = np.array([p[3] for p in problems], dtype=np.float64)
+        self.M1 = np.array([p[4] for p in problems], dtype=np.int32)
+        self.M2 = np.array([p[5] for p in problems], dtype=np.int32)
+        jones = []
+        for p in problems:
+            es, ep = complex(p[6][0]), complex(p[6][1])
+            jones += [es.real, es.imag, ep.real, ep.imag]
+        self.jones = np.array(jones, dtype=np.float64)
+        fz = []
+        for p in problems:
+            f = p[7] if len(p) > 7 else "li"
+            if f not in ("li", "laurent"):
+                raise ValueError("rcwa2d: factorization must be 'li' or 'laurent'")
+            fz.append(0 if f == "li" else 1)
+        self.factorization = np.array(fz, dtype=np.int32)
+        self.N = (2 * self.M1.astype(np.int64) + 1) * (2 * self.M2.astype(np.int64) + 1)
+        self.order_offsets = np.concatenate([[0], np.cumsum(self.N)]).astype(np.int64)
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.srcwa_plan2d_create(
+            self.ctx.handle, len(stacks), _lib.ptr(self._flat), _lib.ptr(self._offsets), P,
+            _lib.ptr(self.stack_index), _lib.ptr(self.wavelength), _lib.ptr(self.theta), _lib.ptr(self.phi),
+            _lib.ptr(self.M1), _lib.ptr(self.M2), _lib.ptr(self.jones), _lib.ptr(self.factorization),
+            int(max_workspace_bytes), ctypes.byref(h)))
+        self.handle = h
+        self.n = P
+
+    def results(self):
+        o = int(self.order_offsets[-1])
+        R = np.empty(8 * o); T = np.empty(8 * o); DEr = np.empty(o); DEt = np.empty(o)
+        st = np.empty(self.n, dtype=np.int32); info = np.empty(self.n, dtype=np.int32)
+        _lib.check(self.lib.srcwa_plan_results(self.handle, _lib.ptr(R), _lib.ptr(T), _lib.ptr(DEr), _lib.ptr(DEt),
+                                               _lib.ptr(st), _lib.ptr(info)))
+        return dict(R=R.view(np.complex128), T=T.view(np.complex128), DEr=DEr, DEt=DEt, status=st, eig_info=info)
+
+    def result(self, i: int, res=None) -> Result2D:
+        res = res or self.results()
+        a, b = int(self.order_offsets[i]), int(self.order_offsets[i + 1])
+        N = b - a
+        R, T = res["R"][4 * a:4 * b], res["T"][4 * a:4 * b]
+        code = int(res["status"][i])
+        if code not in _STATUS2D:
+            raise _lib.SrcwaError(code, "rcwa2d: device workspace exhausted")
+        return Result2D(status=_STATUS2D[code], M1=int(self.M1[i]), M2=int(self.M2[i]), N=N,
+                        DEr=res["DEr"][a:b].copy(), DEt=res["DEt"][a:b].copy(),
+                        Rs=R[:N].copy(), Rp=R[N:2 * N].copy(), Ts=T[:N].copy(), Tp=T[N:2 * N].copy(),
+                        Ex_r=R[2 * N:3 * N].copy(), Ey_r=R[3 * N:].copy(),
+                        Ex_t=T[2 * N:3 * N].copy(), Ey_t=T[3 * N:].copy(), eig_info=int(res["eig_info"][i]))
+
+
+def solve2d(stack: Stack2D, wavelength: float, theta_deg: float, phi_deg: float, M1: int, M2: int,
+            jones=None, factorization: str = "li", ctx: _lib.Context | None = None) -> Result2D:
+    """One crossed-grating solve (rcwa::solve2d, rcwa2d.hpp:1355-1411).  jones defaults to s polarisation."""
+    jones = jones if jones is not None else jones_from_angles_2d(90.0)
+    b = CrossedBatch([stack], [(0, wavelength, theta_deg, phi_deg, M1, M2, jones, factorization)], ctx)
+    b.execute()
+    r = b.result(0)
+    b.close()
+    return r
 
 
 # ---- hook-shaped dense batches ---------------------------------------------------------------------

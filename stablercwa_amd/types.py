@@ -220,6 +220,10 @@ class Result2D:
     Rp: np.ndarray
     Ts: np.ndarray
     Tp: np.ndarray
+    Ex_r: np.ndarray = None
+    Ey_r: np.ndarray = None
+    Ex_t: np.ndarray = None
+    Ey_t: np.ndarray = None
     eig_info: int = 0
     message: str = ""
 

@@ -1,0 +1,168 @@
+"""GPU parity of the crossed-grating (2-D) solve through the C-ABI.
+
+* the reference's frozen tables (tests/rcwa2d_reference.inc): DEr, DEt, Rs, Rp, Ts, Tp and the reflected /
+  transmitted Ex, Ey at |a - b| <= atol + rtol |b| (1e-9, 1e-9); the three Rayleigh cases return status 3
+* the reference's compiled solver on seeded circular-pillar instances, DE at 1e-10
+* a y-invariant crossed grating against the planar GPU path (TE, TM)
+* lossless energy conservation
+* seeded instances of the benchmark's pillar generator against the oracle
+* each result bitwise the same alone, in a mixed batch and in a chunked plan
+* invalid inputs raise ValueError (the reference's std::invalid_argument)
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+TABLES = load_golden("rcwa2d_tables.json")["cases"]
+
+
+class _S:
+    def __init__(self, f):
+        self.f = np.asarray(f, dtype=np.float64)
+
+    def flat(self):
+        return self.f
+
+
+def _jones(j):
+    return (complex(j[0], j[1]), complex(j[2], j[3]))
+
+
+def _run(cases, ctx, **kw):
+    from stablercwa_amd import rcwa
+    probs = [(i, c["wavelength"], c["theta"], c["phi"], c["M1"], c["M2"], _jones(c["jones"]),
+              "laurent" if c.get("laurent") else "li") for i, c in enumerate(cases)]
+    b = rcwa.CrossedBatch([_S(c["stack"]) for c in cases], probs, ctx, **kw)
+    b.execute()
+    return b, b.results()
+
+
+def test_rcwa2d_reference_tables(ctx):
+    from stablercwa_amd.types import Status2D
+    b, res = _run(TABLES, ctx)
+    for i, c in enumerate(TABLES):
+        r = b.result(i, res)
+        if c["status"] != 0:
+            assert r.status == Status2D.RayleighAnomaly, c["name"]
+            assert np.all(np.isnan(r.DEr)) and np.all(np.isnan(r.Rs)), c["name"]
+            continue
+        assert r.status == Status2D.Ok and r.eig_info == 0, c["name"]
+        tol = lambda ref: c["atol"] + c["rtol"] * np.abs(ref)
+        for key in ("DEr", "DEt"):
+            ref = np.asarray(c[key])
+            assert np.all(np.abs(getattr(r, key) - ref) <= tol(ref)), (c["name"], key)
+        for key in ("Rs", "Rp", "Ts", "Tp", "Ex_r", "Ey_r", "Ex_t", "Ey_t"):
+            ref = np.asarray(c[key]).view(np.complex128)
+            assert np.all(np.abs(getattr(r, key) - ref) <= tol(ref)), (c["name"], key)
+
+
+def test_rcwa2d_reference_build(ctx):
+    cases = load_golden("rcwa2d_ref.json")["cases"]
+    b, res = _run(cases, ctx)
+    for i, c in enumerate(cases):
+        r = b.result(i, res)
+        assert int(res["status"][i]) == 0, c["name"]
+        assert np.max(np.abs(r.DEr - np.asarray(c["DEr"]))) <= 1e-10, c["name"]
+        assert np.max(np.abs(r.DEt - np.asarray(c["DEt"]))) <= 1e-10, c["name"]
+
+
+def test_y_invariant_matches_planar_gpu(ctx):
+    from stablercwa_amd import rcwa
+    from stablercwa_amd.types import Layer, Layer2D, Stack, Stack2D, XProfile2D
+    edges = [0.0, 0.21, 0.47, 0.9]
+    vals = [1.7, 1.0, 2.1]
+    s2 = Stack2D(1.0, 1.5, [Layer2D.patterned(0.3, [0.0, 0.7], [XProfile2D(edges, vals)])], 0.9, 0.7)
+    s1 = Stack(1.0, [Layer.grating(0.3, edges, vals)], 1.5, 0.9)
+    M = 8
+    te = rcwa.solve(s1, 0.83, 27.0, M, rcwa.Polarization.TE, ctx)
+    tm = rcwa.solve(s1, 0.83, 27.0, M, rcwa.Polarization.TM, ctx)
+    r_te = rcwa.solve2d(s2, 0.83, 27.0, 0.0, M, 0, (1.0, 0.0), ctx=ctx)
+    r_tm = rcwa.solve2d(s2, 0.83, 27.0, 0.0, M, 0, (0.0, 1.0), ctx=ctx)
+    assert np.max(np.abs(r_te.DEr - te.DEr)) <= 1e-11 and np.max(np.abs(r_te.DEt - te.DEt)) <= 1e-11
+    assert np.max(np.abs(r_tm.DEr - tm.DEr)) <= 1e-11 and np.max(np.abs(r_tm.DEt - tm.DEt)) <= 1e-11
+
+
+def test_lossless_energy(ctx):
+    lossless = [c for c in TABLES if c["name"] in ("pillar_sq", "pillar_sq_m6", "pillar_ecc", "c4_normal")]
+    b, res = _run(lossless, ctx)
+    for i, c in enumerate(lossless):
+        r = b.result(i, res)
+        assert abs(r.reflectance() + r.transmittance() - 1.0) <= 1e-12, c["name"]
+
+
+def _pillar_cases(count, orders, seed=0xC4):
+    from stablercwa_amd.workloads import pillar_batch
+    out = []
+    for p in pillar_batch(count, seed=seed, orders=orders):
+        out.append(dict(stack=p.stack.flat().tolist(), wavelength=p.wavelength, theta=p.theta, phi=p.phi,
+                        M1=p.M1, M2=p.M2, jones=[np.sin(np.radians(50.0)), 0.0, np.cos(np.radians(50.0)), 0.0]))
+    return out
+
+
+def test_pillar_generator_against_oracle(ctx):
+    from oracle import rcwa2d
+    cases = _pillar_cases(6, orders=(2, 3, 4))
+    b, res = _run(cases, ctx)
+    for i, c in enumerate(cases):
+        r = b.result(i, res)
+        o = rcwa2d.solve(np.asarray(c["stack"]), c["wavelength"], c["theta"], c["phi"], c["M1"], c["M2"],
+                         _jones(c["jones"]))
+        assert o["status"] == 0 and int(res["status"][i]) == 0, i
+        assert np.max(np.abs(r.DEr - o["DEr"])) <= 1e-10, i
+        assert np.max(np.abs(r.DEt - o["DEt"])) <= 1e-10, i
+        assert np.max(np.abs(r.Rs - o["Rs"])) <= 1e-9 * max(1.0, np.max(np.abs(o["Rs"]))), i
+
+
+def test_full_size_pillar_energy(ctx):
+    """The benchmark's orders (M = 5, 7, 9): lossy pillars only absorb, R + T <= 1, and each solve succeeds."""
+    cases = _pillar_cases(6, orders=(5, 7, 9), seed=0xC5)
+    b, res = _run(cases, ctx)
+    for i, c in enumerate(cases):
+        r = b.result(i, res)
+        assert int(res["status"][i]) == 0 and r.eig_info == 0, i
+        tot = r.reflectance() + r.transmittance()
+        assert 0.0 < tot <= 1.0 + 1e-10, (i, tot)
+        assert np.all(r.DEr >= 0.0) and np.all(r.DEt >= 0.0)
+
+
+def test_alone_equals_mixed_and_chunked(ctx):
+    mix = [TABLES[8], TABLES[13], TABLES[15], TABLES[17]] + _pillar_cases(2, orders=(3,))
+    b, res = _run(mix, ctx)
+    b2, res2 = _run(mix, ctx, max_workspace_bytes=1)   # one instance per chunk
+    assert b2.chunks() == len(mix)
+    for i, c in enumerate(mix):
+        ba, ra = _run([c], ctx)
+        a, m, k = ba.result(0, ra), b.result(i, res), b2.result(i, res2)
+        for key in ("DEr", "DEt", "Rs", "Rp", "Ts", "Tp"):
+            assert np.array_equal(getattr(a, key), getattr(m, key)), (i, key)
+            assert np.array_equal(getattr(a, key), getattr(k, key)), (i, key)
+
+
+@pytest.mark.parametrize("mutate, msg", [
+    (lambda s: s.__setitem__(0, -1.0), "periods"),
+    (lambda s: s.__setitem__(2, 0.0), "incident"),
+    (lambda s: s.__setitem__(5, 300.0), "layers"),
+])
+def test_invalid_stack_raises(ctx, mutate, msg):
+    c = dict(TABLES[8])
+    st = list(c["stack"])
+    mutate(st)
+    c["stack"] = st
+    with pytest.raises(ValueError):
+        _run([c], ctx)
+
+
+def test_invalid_problem_raises(ctx):
+    from stablercwa_amd import rcwa
+    c = TABLES[8]
+    with pytest.raises(ValueError):
+        rcwa.CrossedBatch([_S(c["stack"])], [(0, c["wavelength"], 95.0, 0.0, 2, 2, (1.0, 0.0))], ctx)
+    with pytest.raises(ValueError):
+        rcwa.CrossedBatch([_S(c["stack"])], [(0, -1.0, 0.0, 0.0, 2, 2, (1.0, 0.0))], ctx)
+    with pytest.raises(ValueError):
+        rcwa.CrossedBatch([_S(c["stack"])], [(0, 1.0, 0.0, 0.0, -1, 2, (1.0, 0.0))], ctx)
+    with pytest.raises(ValueError):
+        rcwa.CrossedBatch([_S(c["stack"])], [(0, 1.0, 0.0, 0.0, 2, 2, (float("nan"), 0.0))], ctx)
