@@ -18,8 +18,8 @@
  *         patterned: Ky, y_edges[Ky+1], then per y segment Kx, x_edges[Kx+1], (n.re, n.im)[Kx]
  *
  * Status words per solve: 0 ok, 1 numerical failure (results NaN; the reference's NumericalFailure,
- * rcwa1d.hpp:37-38), 3 Rayleigh anomaly (crossed gratings, rcwa2d_types.hpp:32-37), 4 conical layer
- * needing Schur-basis propagation.
+ * rcwa1d.hpp:37-38), 2 device workspace exhausted (a sizing error of this library, never expected),
+ * 3 Rayleigh anomaly (crossed gratings, rcwa2d_types.hpp:32-37).
  */
 #ifndef SRCWA_H
 #define SRCWA_H
@@ -67,7 +67,8 @@ int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, 
  * Per problem N = 2M+1 orders; R and T receive the Jones matrices Jr, Jt as N x 4 complex
  * (ss, sp, ps, pp per order) and DEr, DEt the N x 4 channel efficiencies flux * |J|^2
  * (ConicalResult, rcwa1d_conical.hpp:105-183).  Result offsets are 4 * the planar order offsets.
- * Status 4 marks a degenerate layer that needs the reference's Schur-basis propagation. */
+ * Degenerate layers (eta = min|lambda| / |ky|^1.5 < 1e-2) take the Schur-basis propagation on the GPU
+ * (rcwa1d_conical.hpp:413-502, 797-835). */
 int srcwa_plan_conical_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
                               int n_problems, const int* stack_index, const double* wavelength,
                               const double* theta_deg, const double* phi_deg, const int* M,
@@ -98,6 +99,11 @@ int srcwa_plan_execute(srcwa_plan* plan);
 int srcwa_plan_sync(srcwa_plan* plan);
 /* Copy results to host buffers (implies sync).  Any pointer may be NULL. */
 int srcwa_plan_results(srcwa_plan* plan, double* R, double* T, double* DEr, double* DEt, int* status, int* eig_info);
+/* Per-order geometry of the last execute of a conical or crossed plan (8 doubles per order, at 8 x the
+ * order offset).  Conical: kx, kzI (re, im), kzII (re, im), flux_r, flux_t, phi_m in degrees
+ * (ConicalResult::kx, kzI, kzII, flux_r, flux_t, phi_m_deg, rcwa1d_conical.hpp:105-112).  Crossed: Kx, Ky,
+ * kzI (re, im), kzII (re, im), 0, 0 (Result2D::Kx, Ky, kzI, kzII, rcwa2d_types.hpp:119-121). */
+int srcwa_plan_geometry(srcwa_plan* plan, double* out);
 /* Total result orders (sum of N over problems), device bytes held, number of launch chunks. */
 int64_t srcwa_plan_orders(const srcwa_plan* plan);
 int64_t srcwa_plan_device_bytes(const srcwa_plan* plan);

@@ -38,6 +38,7 @@ PROTOTYPES = {
     "srcwa_plan_sync": (ctypes.c_int, [_vp]),
     "srcwa_plan_results": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "srcwa_plan_orders": (ctypes.c_int64, [_vp]),
+    "srcwa_plan_geometry": (ctypes.c_int, [_vp, _vp]),
     "srcwa_plan_device_bytes": (ctypes.c_int64, [_vp]),
     "srcwa_plan_chunks": (ctypes.c_int, [_vp]),
     "srcwa_plan_stage_ms": (ctypes.c_int, [_vp, _vp]),

@@ -13,6 +13,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libsrcwa_amd.so")
+ROOT = os.path.dirname(HERE)
+INCLUDE = os.path.join(ROOT, "include")
+# C++ host API driver used by the GPU tests (links the library above)
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "api_driver.cpp")
+DRIVER = os.path.join(OBJ, "api_driver")
 
 SOURCES = ["rcwa1d.hip", "conical.hip", "rcwa2d.hip", "dense.hip", "runtime.cpp"]
 
@@ -46,6 +51,12 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
+    if os.path.exists(DRIVER_SRC):
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, DRIVER_SRC, "-o", DRIVER, "-L", HERE,
+               "-lsrcwa_amd", "-Wl,-rpath,$ORIGIN/.."]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"C++ API driver failed to build:\n{r.stderr[-4000:]}")
     if verbose:
         print(f"built {LIB}")
     return LIB

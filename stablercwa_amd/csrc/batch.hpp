@@ -90,7 +90,13 @@ struct Out1D {
     double* DEt;
     int* status;
     int* eig_info;
+    double* geometry;   // conical / crossed: GEO_WORDS doubles per order (null for planar plans)
 };
+
+// Per-order geometry written by the conical and crossed setup kernels:
+//   conical: kx, kzI.re, kzI.im, kzII.re, kzII.im, flux_r, flux_t, phi_m (degrees)
+//   crossed: Kx, Ky, kzI.re, kzI.im, kzII.re, kzII.im, 0, 0
+constexpr int GEO_WORDS = 8;
 
 // Batched dense kernels that mirror the reference's hook shapes (custom_eig / custom_solve /
 // custom_multiply, rcwa1d.hpp:122-136): one item per workgroup, column-major complex.

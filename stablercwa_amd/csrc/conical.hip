@@ -103,7 +103,8 @@ DEV LayerView layer_at(const double* st, int index) {
 
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(NT) void k_setup_conical(const Inst1D* __restrict__ insts, const double* __restrict__ stacks,
-                                                      const int64_t* __restrict__ layouts, cd* __restrict__ ws) {
+                                                      const int64_t* __restrict__ layouts, cd* __restrict__ ws,
+                                                      double* __restrict__ geometry) {
     const Inst1D in = insts[blockIdx.x];
     const double* st = stacks + in.stack_off;
     const int64_t* lay = layouts + in.layout_off;
@@ -133,6 +134,18 @@ __global__ __launch_bounds__(NT) void k_setup_conical(const Inst1D* __restrict__
         g.kzI[j] = kzI;
         g.kzII[j] = kzII;
         g.flux[j] = cd{kzI.x / flux_unit, kzII.x / flux_unit};
+        if (geometry) {
+            // the diffraction-plane azimuth of each order (rcwa1d_conical.hpp:735-742)
+            double* o = geometry + GEO_WORDS * (in.out_off + j);
+            o[0] = kxj;
+            o[1] = kzI.x;
+            o[2] = kzI.y;
+            o[3] = kzII.x;
+            o[4] = kzII.y;
+            o[5] = kzI.x / flux_unit;
+            o[6] = kzII.x / flux_unit;
+            o[7] = kt <= 1e-14 ? in.phi_deg : atan2(ky, kxj) * 180.0 / kPi;
+        }
     }
     if (threadIdx.x == 0) g.kx[6 * N] = cd{ky, 0.0};
     __syncthreads();
@@ -636,7 +649,7 @@ void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
                     hipEvent_t* ev) {
     if (ev) hipEventRecord(ev[0], s);
-    k_setup_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws);
+    k_setup_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);
     if (n_tasks > 0) k_assemble_conical<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
     if (ev) hipEventRecord(ev[1], s);
     if (n_tasks > 0) k_eig_conical<<<2 * n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info, max_iter);

@@ -90,7 +90,8 @@ DEV void homogeneous_modes2d(cd n, const cd* geo, int N, cd* q, cd* vb) {
 
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(NT) void k_setup2d(const Inst2D* __restrict__ insts, const double* __restrict__ stacks,
-                                                const int64_t* __restrict__ layouts, cd* __restrict__ ws) {
+                                                const int64_t* __restrict__ layouts, cd* __restrict__ ws,
+                                                double* __restrict__ geometry) {
     const Inst2D in = insts[blockIdx.x];
     const double* st = stacks + in.stack_off;
     const int64_t* lay = layouts + in.layout_off;
@@ -110,6 +111,17 @@ __global__ __launch_bounds__(NT) void k_setup2d(const Inst2D* __restrict__ insts
         const cd kzI = kz_over_k0(cd{nI, 0.0}, Kx, Ky), kzII = kz_over_k0(nII, Kx, Ky);
         geo[2 * N + i] = kzI;
         geo[3 * N + i] = kzII;
+        if (geometry) {
+            double* o = geometry + GEO_WORDS * (in.out_off + i);
+            o[0] = Kx;
+            o[1] = Ky;
+            o[2] = kzI.x;
+            o[3] = kzI.y;
+            o[4] = kzII.x;
+            o[5] = kzII.y;
+            o[6] = 0.0;
+            o[7] = 0.0;
+        }
         zero |= is_zero(kzI) || is_zero(kzII);
     }
     // Rayleigh anomaly in any homogeneous medium (rcwa2d.hpp:1101-1114)
@@ -1143,7 +1155,7 @@ void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
                     hipEvent_t* ev) {
     if (ev) hipEventRecord(ev[0], s);
-    k_setup2d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws);
+    k_setup2d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);
     if (n_tasks > 0) {
         k_ops2d<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
         k_pq2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);

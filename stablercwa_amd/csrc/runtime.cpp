@@ -312,6 +312,7 @@ struct srcwa_plan {
     DevBuf<cd> ws;
     DevBuf<int> task_info;
     DevBuf<double> R, T, DEr, DEt;
+    DevBuf<double> geometry;      // conical / crossed: GEO_WORDS per order
     DevBuf<int> status, eig_info;
     std::vector<hipEvent_t> ev;      // 4 per chunk: start, after assembly, after eigensolves, end
     bool timed = false;
@@ -326,7 +327,7 @@ struct srcwa_plan {
     }
     int64_t bytes() const {
         return insts.bytes() + insts2d.bytes() + tasks.bytes() + stacks.bytes() + layouts.bytes() + ws.bytes() + task_info.bytes() +
-               R.bytes() + T.bytes() + DEr.bytes() + DEt.bytes() + status.bytes() + eig_info.bytes();
+               R.bytes() + T.bytes() + DEr.bytes() + DEt.bytes() + status.bytes() + eig_info.bytes() + geometry.bytes();
     }
 };
 
@@ -621,6 +622,7 @@ int srcwa_plan_conical_create(srcwa_context* ctx, int n_stacks, const double* st
         plan->R.alloc(8 * orders);
         plan->T.alloc(8 * orders);
         plan->DEr.alloc(4 * orders);
+        plan->geometry.alloc(GEO_WORDS * orders);
         plan->DEt.alloc(4 * orders);
         plan->status.alloc(n_problems);
         plan->eig_info.alloc(n_problems);
@@ -637,7 +639,8 @@ int srcwa_plan_execute(srcwa_plan* plan) {
         hipStream_t s = plan->ctx->stream;
         for (size_t c = 0; c < plan->chunks.size(); ++c) {
             const PlanChunk& ch = plan->chunks[c];
-            Out1D out{plan->R.p, plan->T.p, plan->DEr.p, plan->DEt.p, plan->status.p + ch.i0, plan->eig_info.p + ch.i0};
+            Out1D out{plan->R.p, plan->T.p, plan->DEr.p, plan->DEt.p, plan->status.p + ch.i0, plan->eig_info.p + ch.i0,
+                      plan->kind == 1 ? nullptr : plan->geometry.p};
             if (plan->kind == 1)
                 launch_planar(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts.p + ch.i0, plan->tasks.p + ch.t0,
                               plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + ch.t0, out, 30,
@@ -686,6 +689,17 @@ int srcwa_plan_results(srcwa_plan* plan, double* R, double* T, double* DEr, doub
 }
 
 int64_t srcwa_plan_orders(const srcwa_plan* plan) { return plan ? plan->total_orders : -1; }
+
+int srcwa_plan_geometry(srcwa_plan* plan, double* out) {
+    return guarded([&] {
+        if (!plan || !out) invalid("srcwa_plan_geometry: null plan or output");
+        if (plan->kind == 1) invalid("srcwa_plan_geometry: planar plans carry no per-order geometry");
+        hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+        hip_check(hipStreamSynchronize(plan->ctx->stream), "hipStreamSynchronize");
+        hip_check(hipMemcpy(out, plan->geometry.p, GEO_WORDS * plan->total_orders * sizeof(double),
+                            hipMemcpyDeviceToHost), "D2H");
+    });
+}
 int64_t srcwa_plan_device_bytes(const srcwa_plan* plan) { return plan ? plan->bytes() : -1; }
 int srcwa_plan_chunks(const srcwa_plan* plan) { return plan ? static_cast<int>(plan->chunks.size()) : -1; }
 
@@ -1204,6 +1218,7 @@ extern "C" int srcwa_plan2d_create(srcwa_context* ctx, int n_stacks, const doubl
         plan->task_info.alloc(4 * std::max(1, plan->n_tasks));
         plan->R.alloc(8 * orders);
         plan->T.alloc(8 * orders);
+        plan->geometry.alloc(GEO_WORDS * orders);
         plan->DEr.alloc(orders);
         plan->DEt.alloc(orders);
         plan->status.alloc(n_problems);

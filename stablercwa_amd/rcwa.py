@@ -178,17 +178,24 @@ class ConicalBatch(PlanarBatch):
         st = np.empty(self.n, dtype=np.int32); info = np.empty(self.n, dtype=np.int32)
         _lib.check(self.lib.srcwa_plan_results(self.handle, _lib.ptr(R), _lib.ptr(T), _lib.ptr(DEr), _lib.ptr(DEt),
                                                _lib.ptr(st), _lib.ptr(info)))
+        geo = np.empty(8 * (o // 4))
+        _lib.check(self.lib.srcwa_plan_geometry(self.handle, _lib.ptr(geo)))
         return dict(Jr=R.view(np.complex128).reshape(-1, 4), Jt=T.view(np.complex128).reshape(-1, 4),
-                    DEr=DEr.reshape(-1, 4), DEt=DEt.reshape(-1, 4), status=st, eig_info=info)
+                    DEr=DEr.reshape(-1, 4), DEt=DEt.reshape(-1, 4), status=st, eig_info=info,
+                    geometry=geo.reshape(-1, 8))
 
     def result(self, i: int, res=None) -> ConicalResult:
         res = res or self.results()
         a, b = int(self.order_offsets[i]), int(self.order_offsets[i + 1])
         M = int(self.M[i])
-        return ConicalResult(orders=np.arange(-M, M + 1), kx=None, ky=None, kzI=None, kzII=None, flux_r=None,
-                             flux_t=None, Jr=res["Jr"][a:b].copy(), Jt=res["Jt"][a:b].copy(),
+        g = res["geometry"][a:b]
+        th, ph = np.radians(self.theta[i]), np.radians(self.phi[i])
+        nI = float(self._flat[int(self._offsets[self.stack_index[i]]) + 1])
+        return ConicalResult(orders=np.arange(-M, M + 1), kx=g[:, 0].copy(), ky=nI * np.sin(th) * np.sin(ph),
+                             kzI=g[:, 1] + 1j * g[:, 2], kzII=g[:, 3] + 1j * g[:, 4], flux_r=g[:, 5].copy(),
+                             flux_t=g[:, 6].copy(), Jr=res["Jr"][a:b].copy(), Jt=res["Jt"][a:b].copy(),
                              DEr=res["DEr"][a:b].copy(), DEt=res["DEt"][a:b].copy(),
-                             eig_info=int(res["eig_info"][i]))
+                             eig_info=int(res["eig_info"][i]), phi_m_deg=g[:, 7].copy())
 
 
 def solve_conical(stack: Stack, wavelength: float, theta_deg: float, phi_deg: float, M: int,
@@ -263,13 +270,17 @@ class CrossedBatch(PlanarBatch):
         st = np.empty(self.n, dtype=np.int32); info = np.empty(self.n, dtype=np.int32)
         _lib.check(self.lib.srcwa_plan_results(self.handle, _lib.ptr(R), _lib.ptr(T), _lib.ptr(DEr), _lib.ptr(DEt),
                                                _lib.ptr(st), _lib.ptr(info)))
-        return dict(R=R.view(np.complex128), T=T.view(np.complex128), DEr=DEr, DEt=DEt, status=st, eig_info=info)
+        geo = np.empty(8 * o)
+        _lib.check(self.lib.srcwa_plan_geometry(self.handle, _lib.ptr(geo)))
+        return dict(R=R.view(np.complex128), T=T.view(np.complex128), DEr=DEr, DEt=DEt, status=st, eig_info=info,
+                    geometry=geo.reshape(-1, 8))
 
     def result(self, i: int, res=None) -> Result2D:
         res = res or self.results()
         a, b = int(self.order_offsets[i]), int(self.order_offsets[i + 1])
         N = b - a
         R, T = res["R"][4 * a:4 * b], res["T"][4 * a:4 * b]
+        g = res["geometry"][a:b]
         code = int(res["status"][i])
         if code not in _STATUS2D:
             raise _lib.SrcwaError(code, "rcwa2d: device workspace exhausted")
@@ -277,7 +288,8 @@ class CrossedBatch(PlanarBatch):
                         DEr=res["DEr"][a:b].copy(), DEt=res["DEt"][a:b].copy(),
                         Rs=R[:N].copy(), Rp=R[N:2 * N].copy(), Ts=T[:N].copy(), Tp=T[N:2 * N].copy(),
                         Ex_r=R[2 * N:3 * N].copy(), Ey_r=R[3 * N:].copy(),
-                        Ex_t=T[2 * N:3 * N].copy(), Ey_t=T[3 * N:].copy(), eig_info=int(res["eig_info"][i]))
+                        Ex_t=T[2 * N:3 * N].copy(), Ey_t=T[3 * N:].copy(), eig_info=int(res["eig_info"][i]),
+                        Kx=g[:, 0].copy(), Ky=g[:, 1].copy(), kzI=g[:, 2] + 1j * g[:, 3], kzII=g[:, 4] + 1j * g[:, 5])
 
 
 def solve2d(stack: Stack2D, wavelength: float, theta_deg: float, phi_deg: float, M1: int, M2: int,

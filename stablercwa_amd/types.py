@@ -139,6 +139,7 @@ class ConicalResult:
     DEt: np.ndarray
     eig_info: int = 0
     n_schur: int = 0
+    phi_m_deg: np.ndarray = None
 
     def flux_unitarity(self) -> np.ndarray:
         """Sum over orders of flux * J^H J for both sides, as [[uss, usp], [ups, upp]] (identity when lossless)."""
@@ -224,6 +225,10 @@ class Result2D:
     Ey_r: np.ndarray = None
     Ex_t: np.ndarray = None
     Ey_t: np.ndarray = None
+    Kx: np.ndarray = None
+    Ky: np.ndarray = None
+    kzI: np.ndarray = None
+    kzII: np.ndarray = None
     eig_info: int = 0
     message: str = ""
 

@@ -29,3 +29,23 @@ def test_python_prototypes_cover_header():
 def test_version_string():
     from stablercwa_amd import _lib
     assert b"gfx950" in _lib.load().srcwa_version()
+
+
+def test_cpp_header_compiles_standalone(tmp_path):
+    """include/stablercwa_amd/rcwa.hpp is self-contained C++17 over srcwa.h (no HIP headers)."""
+    import subprocess
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "stablercwa_amd/rcwa.hpp"\n'
+                   "namespace rcwa = srcwa_amd::rcwa;\n"
+                   "int main() {\n"
+                   "  rcwa::Stack s; s.nII = 1.46; s.period = 0.574;\n"
+                   "  s.layers.push_back(rcwa::Layer::grating(0.38, {0.0, 0.172, 0.574}, {1.46, 1.0}));\n"
+                   "  rcwa::Options o; (void)o; (void)rcwa::jones_from_angles_2d(90.0);\n"
+                   "  return rcwa::harmonic_index_2d(0, 0, 2, 2) == 12 ? 0 : 1;\n}\n")
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                        "-I", os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_cpp_api_driver_is_built():
+    assert os.path.exists(os.path.join(ROOT, "stablercwa_amd", "_obj", "api_driver"))

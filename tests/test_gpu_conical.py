@@ -116,3 +116,22 @@ def test_conical_invalid_input(ctx):
     st = Stack(1.0, [Layer.grating(0.3, [0, 0.2, 0.6], [1.5, 1.0])], 1.5, 0.6)
     with pytest.raises(ValueError):
         rcwa.solve_conical(st, 1.0, 10.0, float("nan"), 5, ctx)
+
+
+def test_conical_geometry_and_flux_unitarity(ctx):
+    """Per-order kx, kz, flux factors and phi_m against the oracle; flux unitarity is the identity when lossless."""
+    from oracle import conical
+    cases = load_golden("conical_tables.json")["cases"]
+    b, res = _run(cases, ctx)
+    for i, c in enumerate(cases):
+        r = b.result(i, res)
+        o = conical.solve(np.array(c["stack"]), c["wavelength"], c["theta"], c["phi"], c["M"])
+        assert np.max(np.abs(r.flux_r - o["flux_r"])) <= 1e-14, c["id"]
+        assert np.max(np.abs(r.flux_t - o["flux_t"])) <= 1e-14, c["id"]
+        nI = float(c["stack"][1])
+        sine = nI * np.sin(np.radians(c["theta"]))
+        kx = sine * np.cos(np.radians(c["phi"])) - np.arange(-c["M"], c["M"] + 1) * c["wavelength"] / c["stack"][0]
+        assert np.max(np.abs(r.kx - kx)) <= 1e-15 * max(1.0, np.max(np.abs(kx))), c["id"]
+        assert np.all(r.kzI.imag >= 0.0) and np.all(np.abs(r.phi_m_deg) <= 180.0), c["id"]
+    r = b.result(0, res)                                   # binary grating, lossless
+    assert np.max(np.abs(r.flux_unitarity() - np.eye(2))) <= 1e-12

@@ -50,6 +50,8 @@ def test_rcwa2d_reference_tables(ctx):
             assert np.all(np.isnan(r.DEr)) and np.all(np.isnan(r.Rs)), c["name"]
             continue
         assert r.status == Status2D.Ok and r.eig_info == 0, c["name"]
+        assert np.max(np.abs(r.Kx - np.asarray(c["Kx"]))) <= 1e-14, c["name"]
+        assert np.max(np.abs(r.Ky - np.asarray(c["Ky"]))) <= 1e-14, c["name"]
         tol = lambda ref: c["atol"] + c["rtol"] * np.abs(ref)
         for key in ("DEr", "DEt"):
             ref = np.asarray(c[key])
