@@ -214,6 +214,7 @@ __global__ __launch_bounds__(NT) void k_assemble_conical(const Inst1D* __restric
 }
 
 // ------------------------------------------------------------------------------------------------
+template <int QC>
 __global__ __launch_bounds__(NT) void k_eig_conical(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                                     const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                                     int* __restrict__ task_info, int max_iter) {
@@ -225,8 +226,8 @@ __global__ __launch_bounds__(NT) void k_eig_conical(const Inst1D* __restrict__ i
     const int N = in.N;
     const ConicalSlots s = slots_of(ws + lay[conical_slot(in.L, tk.layer)], N);
     int info;
-    if (which == 0) info = wg_eig(N, s.A, s.lam1, s.Z1, s.X1, EigScratch{s.ort1, s.tmp1}, max_iter, kEps);
-    else info = wg_eig(N, s.BC, s.lam2, s.Z2, s.X2, EigScratch{s.ort2, s.tmp2}, max_iter, kEps);
+    if (which == 0) info = wg_eig<QC>(N, s.A, s.lam1, s.Z1, s.X1, EigScratch{s.ort1, s.tmp1}, max_iter, kEps);
+    else info = wg_eig<QC>(N, s.BC, s.lam2, s.Z2, s.X2, EigScratch{s.ort2, s.tmp2}, max_iter, kEps);
     if (threadIdx.x == 0) task_info[4 * t + 1 + which] = info;
 }
 
@@ -325,6 +326,7 @@ DEV int wg_triangular_exp(int n, cd* a, const cd* diag_a, cd* e, cd* t1, cd* t2)
 
 // The degenerate layer's Schur-basis modes: returns 1 on numerical failure, else writes q (2N),
 // W (= Z), V (= Q Z G^-1) and the propagation X = exp(-k0 d G) (into the Z1..X2 slots).
+template <int QC>
 DEV int schur_modes(const Inst1D& in, const double* st, const int64_t* lay, cd* ws, int layer, int ordinal,
                     const ConicalGeo& g, const ConicalSlots& s, cd* q, double ky, int max_iter) {
     const int N = in.N, M = in.M, n = 2 * N;
@@ -398,7 +400,7 @@ DEV int schur_modes(const Inst1D& in, const double* st, const int64_t* lay, cd* 
     for (int64_t e = threadIdx.x; e < nn; e += NT) bad |= !finite(E3[e]);
     if (block_max_int(bad)) return 1;
     cd* wv = s.lam1;              // eigenvalues and the two scratch vectors: lambda1 .. tmp2 (6N = 3n entries)
-    if (wg_eig(n, E3, wv, s.W, E4, EigScratch{wv + n, wv + 2 * n}, max_iter, kEps, false) != 0) return 1;
+    if (wg_eig<QC>(n, E3, wv, s.W, E4, EigScratch{wv + n, wv + 2 * n}, max_iter, kEps, false) != 0) return 1;
     // G = sqrt(T) with a residual check ||G G - T||_1 <= 1e-10 ||T||_1
     wg_copy(n, n, E3, n, E1, n);
     if (wg_triangular_root(n, E3)) return 1;
@@ -435,6 +437,7 @@ DEV int schur_modes(const Inst1D& in, const double* st, const int64_t* lay, cd* 
 }
 
 // ------------------------------------------------------------------------------------------------
+template <int QC>
 __global__ __launch_bounds__(NT) void k_modes_conical(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                                       const double* __restrict__ stacks,
                                                       const int64_t* __restrict__ layouts, cd* __restrict__ ws,
@@ -456,7 +459,7 @@ __global__ __launch_bounds__(NT) void k_modes_conical(const Inst1D* __restrict__
     const double kyy = ws[lay[0] + 6 * N].x;
     const double ky_scale = fabs(kyy) * sqrt(fabs(kyy));
     if (kyy != 0.0 && local_min < 1e-2 * ky_scale) {
-        const int fail = schur_modes(in, stacks + in.stack_off, lay, ws, tk.layer, t - in.first_task, g, s, q, kyy, 100);
+        const int fail = schur_modes<QC>(in, stacks + in.stack_off, lay, ws, tk.layer, t - in.first_task, g, s, q, kyy, 100);
         if (threadIdx.x == 0) task_info[4 * t + 3] = fail ? -1 : 3;
         return;
     }
@@ -646,15 +649,15 @@ __global__ __launch_bounds__(NT) void k_smatrix_conical(const Inst1D* __restrict
 
 // ------------------------------------------------------------------------------------------------
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
-                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
+                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int max_n,
                     hipEvent_t* ev) {
     if (ev) hipEventRecord(ev[0], s);
     k_setup_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);
     if (n_tasks > 0) k_assemble_conical<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
     if (ev) hipEventRecord(ev[1], s);
-    if (n_tasks > 0) k_eig_conical<<<2 * n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info, max_iter);
+    if (n_tasks > 0) EIG_LAUNCH(max_n, k_eig_conical, 2 * n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
     if (ev) hipEventRecord(ev[2], s);
-    if (n_tasks > 0) k_modes_conical<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
+    if (n_tasks > 0) EIG_LAUNCH(2 * max_n, k_modes_conical, n_tasks, s, insts, tasks, stacks, layouts, ws, task_info);
     k_smatrix_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
     if (ev) hipEventRecord(ev[3], s);
 }

@@ -18,6 +18,7 @@ __global__ __launch_bounds__(NT) void k_zgesv_batched(const DenseItem* __restric
 }
 
 // A is overwritten by the eigenvectors; scratch: Z, X (n x n each), ort, tmp (n each).
+template <int QC>
 __global__ __launch_bounds__(NT) void k_zgeev_batched(const DenseItem* __restrict__ items, cd* arena, int* info,
                                                        int max_iter) {
     const DenseItem it = items[blockIdx.x];
@@ -25,7 +26,7 @@ __global__ __launch_bounds__(NT) void k_zgeev_batched(const DenseItem* __restric
     const int64_t nn = (int64_t)n * n;
     cd* sc = arena + it.scratch;
     EigScratch s{sc + 2 * nn, sc + 2 * nn + n};
-    const int r = wg_eig(n, arena + it.a, arena + it.w, sc, sc + nn, s, max_iter, kEps);
+    const int r = wg_eig<QC>(n, arena + it.a, arena + it.w, sc, sc + nn, s, max_iter, kEps);
     if (threadIdx.x == 0) info[blockIdx.x] = r;
 }
 
@@ -35,6 +36,7 @@ void launch_zgemm_batched(hipStream_t s, int count, const DenseItem* items, cd* 
 void launch_zgesv_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info) {
     k_zgesv_batched<<<count, NT, 0, s>>>(items, arena, info);
 }
-void launch_zgeev_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info, int max_iter) {
-    k_zgeev_batched<<<count, NT, 0, s>>>(items, arena, info, max_iter);
+void launch_zgeev_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info, int max_iter,
+                          int max_n) {
+    EIG_LAUNCH(max_n, k_zgeev_batched, count, s, items, arena, info, max_iter);
 }

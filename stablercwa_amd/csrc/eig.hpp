@@ -30,6 +30,21 @@
 #include "wg_la.hpp"
 
 constexpr int EIG_MAX_N = EIG_MAX_N_HOST;   // largest order the LDS rotation table holds
+constexpr int EIG_RING = 8;                 // rows each column prefetches ahead in the QR sweep
+
+// Columns per thread in the QR sweep for matrices of order n (the eigen kernels are instantiated
+// for 1, 2 and 4 so small matrices do not pay the registers of the largest).
+inline int eig_columns_per_thread(int n) { return n <= NT ? 1 : n <= 2 * NT ? 2 : 4; }
+
+// Launches KERNEL<QC> for the QC that matrices of order up to max_n need.
+#define EIG_LAUNCH(max_n, KERNEL, GRID, STREAM, ...)                                      \
+    do {                                                                                  \
+        switch (eig_columns_per_thread(max_n)) {                                          \
+            case 1: KERNEL<1><<<(GRID), NT, 0, (STREAM)>>>(__VA_ARGS__); break;           \
+            case 2: KERNEL<2><<<(GRID), NT, 0, (STREAM)>>>(__VA_ARGS__); break;           \
+            default: KERNEL<4><<<(GRID), NT, 0, (STREAM)>>>(__VA_ARGS__); break;          \
+        }                                                                                 \
+    } while (0)
 
 struct EigScratch {
     cd* ort;     // n
@@ -43,8 +58,10 @@ struct EigScratch {
 // With vectors == false the Schur form is returned instead (the reference's keep_transform without
 // vectors): A = T (upper triangular, eigenvalues on the diagonal), Z = the unitary transform,
 // Z^H A_in Z = T.
+template <int QC>
 DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z, cd* __restrict__ X,
                EigScratch s, int max_iter, double eps, bool vectors = true) {
+    if (n > QC * NT) return n + 1;     // the launch chose too few columns per thread
     const int tid = threadIdx.x;
     const int lane = lane_id(), wave = wave_id();
     cd* H = A;
@@ -238,54 +255,78 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
         ++its;
         __syncthreads();
 
-        // (1) left rotations: one thread per column c in [l, n-1]
-        constexpr int QMAX = (EIG_MAX_N + NT - 1) / NT;
-        cd carry[QMAX];
+        // (1) left rotations: one thread per column c in [l, n-1] (QC columns per thread).  The rows a
+        //     column will consume are loaded EIG_RING rotations ahead into a register ring (rotation i
+        //     only touches rows i-1 and i, so later rows are final when prefetched); the global-memory
+        //     latency of the strided row access then overlaps the rotation chain.
+        cd carry[QC];
+        cd ring[QC][EIG_RING];
 #pragma unroll
-        for (int q = 0; q < QMAX; ++q) {
+        for (int q = 0; q < QC; ++q) {
             const int c = l + tid + q * NT;
-            if (c < n) carry[q] = h(l, c);
-        }
-        for (int i = l + 1; i <= en; ++i) {
-            // owner of column i-1 forms rotation G_i from (h(i-1,i-1), h(i,i-1))
-            const int owner = i - 1 - l;
-            if ((owner % NT) == tid) {
-                const int q = owner / NT;
-                cd xc{0.0, 0.0};
+            if (c < n) {
+                carry[q] = h(l, c);
 #pragma unroll
-                for (int qq = 0; qq < QMAX; ++qq)
-                    if (qq == q) xc = carry[qq];
-                double xr = xc.x, xi = xc.y, sub = h(i, i - 1).x;
-                const double mm = fmax(fmax(fabs(xr), fabs(xi)), fabs(sub));
-                int e = 0;
-                if (!(mm > 1e-140 && mm < 1e140)) {
-                    e = binary_exponent(mm);
-                    xr = ldexp(xr, -e);
-                    xi = ldexp(xi, -e);
-                    sub = ldexp(sub, -e);
-                }
-                const double nrm = sqrt(xr * xr + xi * xi + sub * sub);
-                h(i - 1, i - 1) = cd{e ? ldexp(nrm, e) : nrm, 0.0};
-                h(i, i - 1) = cd{0.0, 0.0};
-                rot_c[i] = xr / nrm;
-                rot_ci[i] = xi / nrm;
-                rot_s[i] = sub / nrm;
-            }
-            __syncthreads();
-            const double cr = rot_c[i], ci = rot_ci[i], sn = rot_s[i];
-#pragma unroll
-            for (int q = 0; q < QMAX; ++q) {
-                const int c = l + tid + q * NT;
-                if (c >= i && c < n) {
-                    const cd yv = carry[q];
-                    const cd z = h(i, c);
-                    h(i - 1, c) = cd{cr * yv.x + ci * yv.y + sn * z.x, cr * yv.y - ci * yv.x + sn * z.y};
-                    carry[q] = cd{cr * z.x - ci * z.y - sn * yv.x, cr * z.y + ci * z.x - sn * yv.y};
-                }
+                for (int u = 0; u < EIG_RING; ++u)
+                    if (l + 1 + u <= en) ring[q][u] = h(l + 1 + u, c);
             }
         }
+        for (int ib = l + 1; ib <= en; ib += EIG_RING) {
 #pragma unroll
-        for (int q = 0; q < QMAX; ++q) {
+            for (int u = 0; u < EIG_RING; ++u) {
+                const int i = ib + u;
+                if (i > en) break;
+                cd cur[QC];
+#pragma unroll
+                for (int q = 0; q < QC; ++q) {
+                    const int c = l + tid + q * NT;
+                    cur[q] = ring[q][u];
+                    if (i + EIG_RING <= en && c >= i + EIG_RING - 1 && c < n) ring[q][u] = h(i + EIG_RING, c);
+                }
+                // owner of column i-1 forms rotation G_i from (h(i-1,i-1), h(i,i-1))
+                const int owner = i - 1 - l;
+                if ((owner % NT) == tid) {
+                    const int qo = owner / NT;
+                    cd xc{0.0, 0.0}, zc{0.0, 0.0};
+#pragma unroll
+                    for (int q = 0; q < QC; ++q)
+                        if (q == qo) {
+                            xc = carry[q];
+                            zc = cur[q];
+                        }
+                    double xr = xc.x, xi = xc.y, sub = zc.x;
+                    const double mm = fmax(fmax(fabs(xr), fabs(xi)), fabs(sub));
+                    int e = 0;
+                    if (!(mm > 1e-140 && mm < 1e140)) {
+                        e = binary_exponent(mm);
+                        xr = ldexp(xr, -e);
+                        xi = ldexp(xi, -e);
+                        sub = ldexp(sub, -e);
+                    }
+                    const double nrm = sqrt(xr * xr + xi * xi + sub * sub);
+                    const double inv = 1.0 / nrm;
+                    h(i - 1, i - 1) = cd{e ? ldexp(nrm, e) : nrm, 0.0};
+                    h(i, i - 1) = cd{0.0, 0.0};
+                    rot_c[i] = xr * inv;
+                    rot_ci[i] = xi * inv;
+                    rot_s[i] = sub * inv;
+                }
+                __syncthreads();
+                const double cr = rot_c[i], ci = rot_ci[i], sn = rot_s[i];
+#pragma unroll
+                for (int q = 0; q < QC; ++q) {
+                    const int c = l + tid + q * NT;
+                    if (c >= i && c < n) {
+                        const cd yv = carry[q];
+                        const cd z = cur[q];
+                        h(i - 1, c) = cd{cr * yv.x + ci * yv.y + sn * z.x, cr * yv.y - ci * yv.x + sn * z.y};
+                        carry[q] = cd{cr * z.x - ci * z.y - sn * yv.x, cr * z.y + ci * z.x - sn * yv.y};
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < QC; ++q) {
             const int c = l + tid + q * NT;
             if (c >= en && c < n) h(en, c) = carry[q];
         }
@@ -304,18 +345,31 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
         }
         __syncthreads();
 
-        // (3) right rotations: rows of H (0..en) and Z (0..n-1), one thread per row
+        // (3) right rotations: rows of H (0..en) and Z (0..n-1), one thread per row.  Each row loads
+        //     the next RB entries before rotating through them, so RB loads are in flight at once.
+        constexpr int RB = 8;
         for (int r = tid; r < en + 1 + n; r += NT) {
             const bool zrow = r > en;
             const int row = zrow ? r - en - 1 : r;
             cd* base = zrow ? Z : H;
             const int j0 = zrow ? l + 1 : max(l + 1, row);
             cd a = base[row + (long)(j0 - 1) * n];
-            for (int j = j0; j <= en; ++j) {
-                const double cr = rot_c[j], ci = rot_ci[j], sn = rot_s[j];
-                const cd b = base[row + (long)j * n];
-                base[row + (long)(j - 1) * n] = cd{cr * a.x - ci * a.y + sn * b.x, cr * a.y + ci * a.x + sn * b.y};
-                a = cd{cr * b.x + ci * b.y - sn * a.x, cr * b.y - ci * b.x - sn * a.y};
+            for (int jb = j0; jb <= en; jb += RB) {
+                cd bv[RB];
+#pragma unroll
+                for (int u = 0; u < RB; ++u)
+                    if (jb + u <= en) bv[u] = base[row + (long)(jb + u) * n];
+#pragma unroll
+                for (int u = 0; u < RB; ++u) {
+                    const int j = jb + u;
+                    if (j <= en) {
+                        const double cr = rot_c[j], ci = rot_ci[j], sn = rot_s[j];
+                        const cd b = bv[u];
+                        base[row + (long)(j - 1) * n] =
+                            cd{cr * a.x - ci * a.y + sn * b.x, cr * a.y + ci * a.x + sn * b.y};
+                        a = cd{cr * b.x + ci * b.y - sn * a.x, cr * b.y - ci * b.x - sn * a.y};
+                    }
+                }
             }
             if (has_phase) a = a * phase;
             base[row + (long)en * n] = a;

@@ -99,6 +99,7 @@ __global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ in
 }
 
 // ------------------------------------------------------------------------------------------------
+template <int QC>
 __global__ __launch_bounds__(NT) void k_eig1d(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                               const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                               int* __restrict__ task_info, int max_iter) {
@@ -112,7 +113,7 @@ __global__ __launch_bounds__(NT) void k_eig1d(const Inst1D* __restrict__ insts, 
     cd *S0 = base, *S1 = base + NN, *S2 = base + 2 * NN;
     cd* lam = base + 4 * NN;
     EigScratch s{base + 4 * NN + N, base + 4 * NN + 2 * N};
-    const int info = wg_eig(N, S0, lam, S1, S2, s, max_iter, kEps);
+    const int info = wg_eig<QC>(N, S0, lam, S1, S2, s, max_iter, kEps);
     if (threadIdx.x == 0) task_info[blockIdx.x] = info;
 }
 
@@ -258,13 +259,13 @@ __global__ __launch_bounds__(NT) void k_smatrix1d(const Inst1D* __restrict__ ins
 // ------------------------------------------------------------------------------------------------
 // host launchers
 void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
-                   const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
+                   const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int max_n,
                    hipEvent_t* ev) {
     if (ev) hipEventRecord(ev[0], s);
     k_setup1d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws);
     if (n_tasks > 0) k_assemble1d<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
     if (ev) hipEventRecord(ev[1], s);
-    if (n_tasks > 0) k_eig1d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info, max_iter);
+    if (n_tasks > 0) EIG_LAUNCH(max_n, k_eig1d, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
     if (ev) hipEventRecord(ev[2], s);
     if (n_tasks > 0) k_modes1d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
     k_smatrix1d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);

@@ -505,6 +505,7 @@ __global__ __launch_bounds__(NT) void k_pq2d(const Inst2D* __restrict__ insts, c
     if (threadIdx.x == 0) task_info[4 * blockIdx.x] = bad ? -1 : 0;
 }
 
+template <int QC>
 __global__ __launch_bounds__(NT) void k_eig2d(const Inst2D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                               const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                               int* __restrict__ task_info, int max_iter) {
@@ -514,7 +515,7 @@ __global__ __launch_bounds__(NT) void k_eig2d(const Inst2D* __restrict__ insts, 
     const int64_t* lay = layouts + in.layout_off;
     if (ws[lay[0] + 4 * in.N].x != 0.0) return;
     const Slots2D s = slots2d(ws, lay, in.L, tk.layer, in.N);
-    const int info = wg_eig(2 * in.N, s.H, s.lam, s.Z, s.X, EigScratch{s.ort, s.tmp}, max_iter, kEps);
+    const int info = wg_eig<QC>(2 * in.N, s.H, s.lam, s.Z, s.X, EigScratch{s.ort, s.tmp}, max_iter, kEps);
     if (threadIdx.x == 0) task_info[4 * blockIdx.x + 1] = info;
 }
 
@@ -1152,7 +1153,7 @@ __global__ __launch_bounds__(NT) void k_smatrix2d(const Inst2D* __restrict__ ins
 
 // ------------------------------------------------------------------------------------------------
 void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts, const LayerTask* tasks,
-                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
+                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int max_n,
                     hipEvent_t* ev) {
     if (ev) hipEventRecord(ev[0], s);
     k_setup2d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);
@@ -1161,7 +1162,7 @@ void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts,
         k_pq2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
     }
     if (ev) hipEventRecord(ev[1], s);
-    if (n_tasks > 0) k_eig2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info, max_iter);
+    if (n_tasks > 0) EIG_LAUNCH(2 * max_n, k_eig2d, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
     if (ev) hipEventRecord(ev[2], s);
     if (n_tasks > 0) k_modes2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
     k_smatrix2d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);

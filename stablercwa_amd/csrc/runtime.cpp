@@ -19,17 +19,18 @@
 
 // kernel launchers (rcwa1d.hip, dense.hip)
 void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
-                   const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
+                   const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int max_n,
                    hipEvent_t* ev);
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
-                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
+                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int max_n,
                     hipEvent_t* ev);
 void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts, const LayerTask* tasks,
-                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter,
+                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int max_n,
                     hipEvent_t* ev);
 void launch_zgemm_batched(hipStream_t s, int count, const DenseItem* items, cd* arena);
 void launch_zgesv_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info);
-void launch_zgeev_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info, int max_iter);
+void launch_zgeev_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info, int max_iter,
+                          int max_n);
 
 namespace {
 
@@ -301,6 +302,7 @@ struct PlanChunk {
 struct srcwa_plan {
     srcwa_context* ctx = nullptr;
     int kind = 1;
+    int max_n = 1;                // largest harmonic count N of any instance (sizes the eigen kernels)
     int n_inst = 0, n_tasks = 0;
     int64_t total_orders = 0;
     std::vector<PlanChunk> chunks;
@@ -460,6 +462,7 @@ int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, 
             Inst1D& in = h_insts[i];
             in.M = M[i];
             in.N = N;
+            plan->max_n = std::max(plan->max_n, N);
             in.pol = pol[i];
             in.L = L;
             in.first_task = first_task - cur.t0;
@@ -591,6 +594,7 @@ int srcwa_plan_conical_create(srcwa_context* ctx, int n_stacks, const double* st
             Inst1D& in = h_insts[i];
             in.M = M[i];
             in.N = N;
+            plan->max_n = std::max(plan->max_n, N);
             in.pol = 2;
             in.L = L;
             in.first_task = first_task - cur.t0;
@@ -643,15 +647,15 @@ int srcwa_plan_execute(srcwa_plan* plan) {
                       plan->kind == 1 ? nullptr : plan->geometry.p};
             if (plan->kind == 1)
                 launch_planar(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts.p + ch.i0, plan->tasks.p + ch.t0,
-                              plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + ch.t0, out, 30,
+                              plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + ch.t0, out, 30, plan->max_n,
                               plan->ev.data() + 4 * c);
             else if (plan->kind == 2)
                 launch_conical(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts.p + ch.i0, plan->tasks.p + ch.t0,
-                               plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + 4 * ch.t0, out, 30,
+                               plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + 4 * ch.t0, out, 30, plan->max_n,
                                plan->ev.data() + 4 * c);
             else
                 launch_crossed(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts2d.p + ch.i0, plan->tasks.p + ch.t0,
-                               plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + 4 * ch.t0, out, 30,
+                               plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + 4 * ch.t0, out, 30, plan->max_n,
                                plan->ev.data() + 4 * c);
             hip_check(hipGetLastError(), "kernel launch");
         }
@@ -876,7 +880,7 @@ int srcwa_zgeev_batched(srcwa_context* ctx, int count, const int* n, const doubl
             hip_check(hipMemcpyAsync(arena.p + items[i].a, A + 2 * ao, as * 16, hipMemcpyHostToDevice, s), "H2D");
             ao += as;
         }
-        launch_zgeev_batched(s, count, d_items.p, arena.p, d_info.p, 30);
+        launch_zgeev_batched(s, count, d_items.p, arena.p, d_info.p, 30, *std::max_element(n, n + count));
         hip_check(hipGetLastError(), "kernel launch");
         ao = 0;
         int64_t wo = 0;
@@ -1187,6 +1191,7 @@ extern "C" int srcwa_plan2d_create(srcwa_context* ctx, int n_stacks, const doubl
             in.M1 = M1[i];
             in.M2 = M2[i];
             in.N = N;
+            plan->max_n = std::max(plan->max_n, N);
             in.L = L;
             in.first_task = first_task - cur.t0;
             in.n_tasks = static_cast<int>(h_tasks.size()) - first_task;
