@@ -49,6 +49,15 @@ int srcwa_context_create(int device, srcwa_context** out);
 void srcwa_context_destroy(srcwa_context* ctx);
 int srcwa_context_sync(srcwa_context* ctx);
 
+/* A device memory pool: plans created on a context with a pool take all their device buffers from
+ * it (first fit, 256-byte blocks) instead of hipMalloc, and fail with SRCWA_ENOMEM when it is full.
+ * Several contexts may share one pool.  Destroy plans before the pool. */
+typedef struct srcwa_pool srcwa_pool;
+int srcwa_pool_create(int device, int64_t bytes, srcwa_pool** out);
+int srcwa_pool_destroy(srcwa_pool* pool);
+int64_t srcwa_pool_free_bytes(srcwa_pool* pool);
+int srcwa_context_set_pool(srcwa_context* ctx, srcwa_pool* pool);
+
 /* ---------------------------------------------------------------------------------------------
  * Batched planar (TE/TM) solves.  Replaces rcwa::solve / rcwa::sweep (rcwa1d.hpp:1077-1091): each
  * problem is (stack, wavelength, theta_deg, M, pol) with pol 0 = TE, 1 = TM, and yields N = 2M+1
@@ -93,8 +102,24 @@ int srcwa_plan2d_create(srcwa_context* ctx, int n_stacks, const double* stacks, 
                         const double* phi_deg, const int* M1, const int* M2, const double* jones,
                         const int* factorization, int64_t max_workspace_bytes, srcwa_plan** out);
 
+/* Device bytes a plan with these arguments would hold, without allocating anything; the inputs are
+ * validated exactly as by the matching *_create call (same error codes and messages). */
+int srcwa_plan1d_bytes(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                       int n_problems, const int* stack_index, const double* wavelength, const double* theta_deg,
+                       const int* M, const int* pol, int64_t max_workspace_bytes, int64_t* bytes);
+int srcwa_plan_conical_bytes(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                             int n_problems, const int* stack_index, const double* wavelength,
+                             const double* theta_deg, const double* phi_deg, const int* M,
+                             int64_t max_workspace_bytes, int64_t* bytes);
+int srcwa_plan2d_bytes(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                       int n_problems, const int* stack_index, const double* wavelength, const double* theta_deg,
+                       const double* phi_deg, const int* M1, const int* M2, const double* jones,
+                       const int* factorization, int64_t max_workspace_bytes, int64_t* bytes);
+
 /* Launch all chunks of the plan on the context stream (asynchronous). */
 int srcwa_plan_execute(srcwa_plan* plan);
+/* 1 when the last execute has finished, 0 while it runs, a negative code on a device error. */
+int srcwa_plan_poll(srcwa_plan* plan);
 /* Wait for the last execute. */
 int srcwa_plan_sync(srcwa_plan* plan);
 /* Copy results to host buffers (implies sync).  Any pointer may be NULL. */
@@ -137,6 +162,52 @@ int srcwa_zgesv_batched(srcwa_context* ctx, int count, const int* n, const int* 
                         double* X, int* info);
 int srcwa_zgemm_batched(srcwa_context* ctx, int count, const int* m, const int* n, const int* k, const double* A,
                         const double* B, double* C);
+
+/* ---------------------------------------------------------------------------------------------
+ * Request server with continuous batching (no counterpart in the reference, whose solves are
+ * synchronous calls; it serves the reference's solve/sweep requests to many callers at once).
+ *
+ * A worker thread keeps up to `slots` batches on the GPU, each on its own HIP stream, with all device
+ * memory drawn from one pool of hbm_bytes.  Requests that arrive while batches run are admitted as
+ * soon as a slot and memory free up; a batch holds requests of one kind in arrival order, up to
+ * max_batch_points points.  Every result is bitwise the same as solving the request alone.
+ *
+ * Result layout of a request with P points and N orders per point (N = 2M+1, or (2M1+1)(2M2+1)):
+ *   planar : R, T  P*N complex;            DEr, DEt P*N
+ *   conical: R, T  P*N*4 complex (Jones);  DEr, DEt P*N*4
+ *   crossed: R, T  P*4N complex ([Rs, Rp, Ex, Ey] per point); DEr, DEt P*N
+ *   status, eig_info: P ints (as srcwa_plan_results).
+ */
+typedef struct srcwa_server srcwa_server;
+typedef struct srcwa_request {
+    int kind;                  /* 1 planar, 2 conical, 3 crossed grating */
+    int n_points;              /* points of the sweep (>= 1) */
+    const double* stack;       /* one flat stack record (planar layout for kinds 1, 2; crossed for 3) */
+    int64_t stack_len;
+    const double* wavelength;  /* n_points each */
+    const double* theta_deg;
+    const double* phi_deg;     /* kinds 2, 3 */
+    int M;                     /* planar / conical M; crossed M1 */
+    int M2;                    /* crossed */
+    int pol;                   /* planar: 0 TE, 1 TM */
+    int factorization;         /* crossed: 0 Li, 1 Laurent */
+    const double* jones;       /* crossed: Es.re, Es.im, Ep.re, Ep.im */
+} srcwa_request;
+
+int srcwa_server_create(int device, int64_t hbm_bytes, int slots, int max_batch_points, srcwa_server** out);
+/* Validates the request (same errors as the plan calls) and queues it; inputs are copied. */
+int srcwa_server_submit(srcwa_server* server, const srcwa_request* request, int64_t* id);
+/* Waits up to timeout_s seconds (negative: no limit); *done = 1 once the request has finished. */
+int srcwa_server_wait(srcwa_server* server, int64_t id, double timeout_s, int* done);
+/* Copies a finished request's results (any pointer may be NULL); times[0] = submit-to-finish seconds,
+ * times[1] = seconds queued before its batch started. */
+int srcwa_server_result(srcwa_server* server, int64_t id, double* R, double* T, double* DEr, double* DEt,
+                        int* status, int* eig_info, double* times);
+int srcwa_server_release(srcwa_server* server, int64_t id);
+/* batches launched, most batches in flight at once, requests finished, points finished */
+int srcwa_server_stats(srcwa_server* server, int64_t* stats4);
+const char* srcwa_server_last_error(void);
+void srcwa_server_destroy(srcwa_server* server);
 
 #ifdef __cplusplus
 }

@@ -34,7 +34,27 @@ PROTOTYPES = {
                                                  _vp, ctypes.c_int64, ctypes.POINTER(_vp)]),
     "srcwa_plan2d_create": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
                                            _vp, _vp, _vp, ctypes.c_int64, ctypes.POINTER(_vp)]),
+    "srcwa_plan1d_bytes": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
+                                          ctypes.c_int64, _vp]),
+    "srcwa_plan_conical_bytes": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp,
+                                                _vp, ctypes.c_int64, _vp]),
+    "srcwa_plan2d_bytes": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp,
+                                          _vp, _vp, _vp, ctypes.c_int64, _vp]),
     "srcwa_plan_execute": (ctypes.c_int, [_vp]),
+    "srcwa_plan_poll": (ctypes.c_int, [_vp]),
+    "srcwa_pool_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.POINTER(_vp)]),
+    "srcwa_pool_destroy": (ctypes.c_int, [_vp]),
+    "srcwa_pool_free_bytes": (ctypes.c_int64, [_vp]),
+    "srcwa_context_set_pool": (ctypes.c_int, [_vp, _vp]),
+    "srcwa_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(_vp)]),
+    "srcwa_server_submit": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_int64)]),
+    "srcwa_server_wait": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]),
+    "srcwa_server_result": (ctypes.c_int, [_vp, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "srcwa_server_release": (ctypes.c_int, [_vp, ctypes.c_int64]),
+    "srcwa_server_stats": (ctypes.c_int, [_vp, _vp]),
+    "srcwa_server_last_error": (ctypes.c_char_p, []),
+    "srcwa_server_destroy": (None, [_vp]),
     "srcwa_plan_sync": (ctypes.c_int, [_vp]),
     "srcwa_plan_results": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "srcwa_plan_orders": (ctypes.c_int64, [_vp]),

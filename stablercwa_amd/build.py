@@ -19,7 +19,7 @@ INCLUDE = os.path.join(ROOT, "include")
 DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "api_driver.cpp")
 DRIVER = os.path.join(OBJ, "api_driver")
 
-SOURCES = ["rcwa1d.hip", "conical.hip", "rcwa2d.hip", "dense.hip", "runtime.cpp"]
+SOURCES = ["rcwa1d.hip", "conical.hip", "rcwa2d.hip", "dense.hip", "runtime.cpp", "server.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -46,7 +46,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp] + objs + ["-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")

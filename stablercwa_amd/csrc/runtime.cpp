@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <map>
 #include <complex>
 #include <cmath>
 #include <cstdint>
@@ -67,23 +69,82 @@ int guarded(F&& f) {
 
 void invalid(const std::string& m) { throw Failure{SRCWA_EINVAL, m}; }
 
+}  // namespace
+
+// A device arena carved first-fit into 256-byte aligned blocks.  Plans created on a context that has
+// a pool take every device buffer from it, so many plans can come and go (a request server) without
+// hipMalloc / hipFree, and a plan that does not fit fails with SRCWA_ENOMEM instead of growing.
+struct srcwa_pool {
+    int device = 0;
+    char* base = nullptr;
+    size_t bytes = 0;
+    size_t used = 0;
+    std::mutex lock;
+    std::map<size_t, size_t> free_blocks;    // offset -> size, coalesced
+
+    static size_t round(size_t n) { return (n + 255) & ~size_t(255); }
+    void* get(size_t n) {
+        n = round(n);
+        std::lock_guard<std::mutex> g(lock);
+        for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
+            if (it->second < n) continue;
+            const size_t off = it->first, left = it->second - n;
+            free_blocks.erase(it);
+            if (left) free_blocks[off + n] = left;
+            used += n;
+            return base + off;
+        }
+        return nullptr;
+    }
+    void put(void* p, size_t n) {
+        n = round(n);
+        std::lock_guard<std::mutex> g(lock);
+        size_t off = static_cast<size_t>(static_cast<char*>(p) - base);
+        used -= n;
+        auto next = free_blocks.lower_bound(off);
+        if (next != free_blocks.end() && off + n == next->first) {
+            n += next->second;
+            next = free_blocks.erase(next);
+        }
+        if (next != free_blocks.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += n;
+                return;
+            }
+        }
+        free_blocks[off] = n;
+    }
+};
+
+namespace {
+
 template <class T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    srcwa_pool* pool = nullptr;     // allocate from this arena instead of hipMalloc
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (pool) pool->put(p, n * sizeof(T));
+            else (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }
     void alloc(size_t count) {
         release();
         if (count == 0) count = 1;
-        hip_check(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)), "hipMalloc");
+        if (pool) {
+            p = static_cast<T*>(pool->get(count * sizeof(T)));
+            if (!p) throw Failure{SRCWA_ENOMEM, "device pool exhausted"};
+        } else {
+            hip_check(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)), "hipMalloc");
+        }
         n = count;
     }
     void upload(const std::vector<T>& v, hipStream_t s) {
@@ -293,6 +354,7 @@ PlanarCost planar_cost(int N, const std::vector<bool>& general, int pol, bool co
 struct srcwa_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    srcwa_pool* pool = nullptr;     // plans created on this context draw device memory from it
 };
 
 struct PlanChunk {
@@ -323,6 +385,23 @@ struct srcwa_plan {
         for (hipEvent_t& e : ev)
             if (e) (void)hipEventDestroy(e);
     }
+    // every device buffer of the plan comes from the context's pool, when it has one
+    void bind_pool(srcwa_pool* p) {
+        insts.pool = p;
+        insts2d.pool = p;
+        tasks.pool = p;
+        stacks.pool = p;
+        layouts.pool = p;
+        ws.pool = p;
+        task_info.pool = p;
+        R.pool = p;
+        T.pool = p;
+        DEr.pool = p;
+        DEt.pool = p;
+        geometry.pool = p;
+        status.pool = p;
+        eig_info.pool = p;
+    }
     void make_events() {
         ev.assign(4 * chunks.size(), nullptr);
         for (hipEvent_t& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
@@ -334,6 +413,13 @@ struct srcwa_plan {
 };
 
 namespace {
+// Device-to-host copy on the plan's own stream (the legacy null stream would order it against every
+// other stream of the process), followed by a wait for that stream only.
+void copy_out(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, const char* what, srcwa_plan* plan) {
+    hip_check(hipMemcpyAsync(dst, src, bytes, kind, plan->ctx->stream), what);
+    hip_check(hipStreamSynchronize(plan->ctx->stream), what);
+}
+
 // ---- hook-shaped dense batches ------------------------------------------------------------------
 template <class Body>
 int dense_batch(srcwa_context* ctx, Body&& body) {
@@ -387,9 +473,46 @@ int srcwa_context_sync(srcwa_context* ctx) {
     });
 }
 
-int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+int srcwa_pool_create(int device, int64_t bytes, srcwa_pool** out) {
+    return guarded([&] {
+        if (!out || bytes <= 0) invalid("srcwa_pool_create: null output or non-positive size");
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        auto pool = std::unique_ptr<srcwa_pool>(new srcwa_pool);
+        pool->device = device;
+        pool->bytes = srcwa_pool::round(static_cast<size_t>(bytes));
+        hip_check(hipMalloc(reinterpret_cast<void**>(&pool->base), pool->bytes), "hipMalloc (pool)");
+        pool->free_blocks[0] = pool->bytes;
+        *out = pool.release();
+    });
+}
+
+int srcwa_pool_destroy(srcwa_pool* pool) {
+    return guarded([&] {
+        if (!pool) return;
+        if (pool->used != 0) invalid("srcwa_pool_destroy: plans still hold memory of this pool");
+        hip_check(hipSetDevice(pool->device), "hipSetDevice");
+        hip_check(hipFree(pool->base), "hipFree (pool)");
+        delete pool;
+    });
+}
+
+int64_t srcwa_pool_free_bytes(srcwa_pool* pool) {
+    if (!pool) return -1;
+    std::lock_guard<std::mutex> g(pool->lock);
+    return static_cast<int64_t>(pool->bytes - pool->used);
+}
+
+int srcwa_context_set_pool(srcwa_context* ctx, srcwa_pool* pool) {
+    return guarded([&] {
+        if (!ctx) invalid("srcwa_context_set_pool: null context");
+        if (pool && pool->device != ctx->device) invalid("srcwa_context_set_pool: pool and context are on different devices");
+        ctx->pool = pool;
+    });
+}
+
+static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
                         int n_problems, const int* stack_index, const double* wavelength, const double* theta_deg,
-                        const int* M, const int* pol, int64_t max_workspace_bytes, srcwa_plan** out) {
+                        const int* M, const int* pol, int64_t max_workspace_bytes, srcwa_plan** out, int64_t* need) {
     return guarded([&] {
         if (!ctx || !out) invalid("srcwa_plan1d_create: null context or output");
         if (n_stacks < 1 || n_problems < 1 || !stacks || !stack_offsets || !stack_index || !wavelength ||
@@ -401,6 +524,7 @@ int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, 
         std::vector<double>& h_stacks = ps.words;
 
         auto plan = std::unique_ptr<srcwa_plan>(new srcwa_plan);
+        plan->bind_pool(ctx->pool);
         plan->ctx = ctx;
         plan->kind = 1;
         plan->n_inst = n_problems;
@@ -485,6 +609,14 @@ int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, 
         plan->n_tasks = static_cast<int>(h_tasks.size());
         plan->total_orders = orders;
 
+        if (need) {
+            const auto b = [](size_t x) { return static_cast<int64_t>(srcwa_pool::round(x)); };
+            *need = b(h_insts.size() * sizeof(Inst1D)) + b(h_tasks.size() * sizeof(LayerTask)) +
+                    b(h_stacks.size() * sizeof(double)) + b(h_layouts.size() * sizeof(int64_t)) +
+                    b(max_ws * sizeof(cd)) + b(std::max(1, plan->n_tasks) * sizeof(int)) +
+                    2 * b(2 * orders * sizeof(double)) + 2 * b(orders * sizeof(double)) + 2 * b(n_problems * sizeof(int));
+            return;
+        }
         hipStream_t st = ctx->stream;
         plan->insts.upload(h_insts, st);
         plan->tasks.upload(h_tasks, st);
@@ -504,10 +636,29 @@ int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, 
     });
 }
 
-int srcwa_plan_conical_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                        int n_problems, const int* stack_index, const double* wavelength, const double* theta_deg,
+                        const int* M, const int* pol, int64_t max_workspace_bytes, srcwa_plan** out) {
+    return plan1d_create_impl(ctx, n_stacks, stacks, stack_offsets, n_problems, stack_index, wavelength, theta_deg,
+           M, pol, max_workspace_bytes, out, nullptr);
+}
+
+int srcwa_plan1d_bytes(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                        int n_problems, const int* stack_index, const double* wavelength, const double* theta_deg,
+                        const int* M, const int* pol, int64_t max_workspace_bytes, int64_t* bytes) {
+    if (!bytes) {
+        g_error = "null output";
+        return SRCWA_EINVAL;
+    }
+    srcwa_plan* unused = nullptr;
+    return plan1d_create_impl(ctx, n_stacks, stacks, stack_offsets, n_problems, stack_index, wavelength, theta_deg,
+           M, pol, max_workspace_bytes, &unused, bytes);
+}
+
+static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
                               int n_problems, const int* stack_index, const double* wavelength,
                               const double* theta_deg, const double* phi_deg, const int* M,
-                              int64_t max_workspace_bytes, srcwa_plan** out) {
+                              int64_t max_workspace_bytes, srcwa_plan** out, int64_t* need) {
     return guarded([&] {
         if (!ctx || !out) invalid("srcwa_plan_conical_create: null context or output");
         if (n_stacks < 1 || n_problems < 1 || !stacks || !stack_offsets || !stack_index || !wavelength ||
@@ -516,6 +667,7 @@ int srcwa_plan_conical_create(srcwa_context* ctx, int n_stacks, const double* st
         hip_check(hipSetDevice(ctx->device), "hipSetDevice");
         ParsedStacks ps = parse_all(n_stacks, stacks, stack_offsets);
         auto plan = std::unique_ptr<srcwa_plan>(new srcwa_plan);
+        plan->bind_pool(ctx->pool);
         plan->ctx = ctx;
         plan->kind = 2;
         plan->n_inst = n_problems;
@@ -616,6 +768,15 @@ int srcwa_plan_conical_create(srcwa_context* ctx, int n_stacks, const double* st
         max_ws = std::max(max_ws, chunk_ws);
         plan->n_tasks = static_cast<int>(h_tasks.size());
         plan->total_orders = orders;
+        if (need) {
+            const auto b = [](size_t x) { return static_cast<int64_t>(srcwa_pool::round(x)); };
+            *need = b(h_insts.size() * sizeof(Inst1D)) + b(h_tasks.size() * sizeof(LayerTask)) +
+                    b(ps.words.size() * sizeof(double)) + b(h_layouts.size() * sizeof(int64_t)) +
+                    b(max_ws * sizeof(cd)) + b(4 * std::max(1, plan->n_tasks) * sizeof(int)) +
+                    2 * b(8 * orders * sizeof(double)) + 2 * b(4 * orders * sizeof(double)) +
+                    b(GEO_WORDS * orders * sizeof(double)) + 2 * b(n_problems * sizeof(int));
+            return;
+        }
         hipStream_t st = ctx->stream;
         plan->insts.upload(h_insts, st);
         plan->tasks.upload(h_tasks, st);
@@ -634,6 +795,27 @@ int srcwa_plan_conical_create(srcwa_context* ctx, int n_stacks, const double* st
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
         *out = plan.release();
     });
+}
+
+int srcwa_plan_conical_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                              int n_problems, const int* stack_index, const double* wavelength,
+                              const double* theta_deg, const double* phi_deg, const int* M,
+                              int64_t max_workspace_bytes, srcwa_plan** out) {
+    return plan_conical_create_impl(ctx, n_stacks, stacks, stack_offsets, n_problems, stack_index, wavelength,
+           theta_deg, phi_deg, M, max_workspace_bytes, out, nullptr);
+}
+
+int srcwa_plan_conical_bytes(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                              int n_problems, const int* stack_index, const double* wavelength,
+                              const double* theta_deg, const double* phi_deg, const int* M,
+                              int64_t max_workspace_bytes, int64_t* bytes) {
+    if (!bytes) {
+        g_error = "null output";
+        return SRCWA_EINVAL;
+    }
+    srcwa_plan* unused = nullptr;
+    return plan_conical_create_impl(ctx, n_stacks, stacks, stack_offsets, n_problems, stack_index, wavelength,
+           theta_deg, phi_deg, M, max_workspace_bytes, &unused, bytes);
 }
 
 int srcwa_plan_execute(srcwa_plan* plan) {
@@ -670,6 +852,19 @@ int srcwa_plan_sync(srcwa_plan* plan) {
     });
 }
 
+int srcwa_plan_poll(srcwa_plan* plan) {
+    if (!plan) {
+        g_error = "null plan";
+        return SRCWA_EINVAL;
+    }
+    if (plan->ev.empty()) return 1;
+    const hipError_t e = hipEventQuery(plan->ev.back());
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    g_error = std::string("hipEventQuery: ") + hipGetErrorString(e);
+    return SRCWA_EDEVICE;
+}
+
 int srcwa_plan_results(srcwa_plan* plan, double* R, double* T, double* DEr, double* DEt, int* status, int* eig_info) {
     return guarded([&] {
         if (!plan) invalid("null plan");
@@ -678,17 +873,17 @@ int srcwa_plan_results(srcwa_plan* plan, double* R, double* T, double* DEr, doub
         const int64_t o = plan->total_orders * (plan->kind == 2 ? 4 : 1);
         if (plan->kind == 3) {
             // crossed gratings: R = [Rs, Rp, Ex_r, Ey_r], T = [Ts, Tp, Ex_t, Ey_t] per instance (4N complex)
-            if (R) hip_check(hipMemcpy(R, plan->R.p, 8 * o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
-            if (T) hip_check(hipMemcpy(T, plan->T.p, 8 * o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
+            if (R) copy_out(R, plan->R.p, 8 * o * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
+            if (T) copy_out(T, plan->T.p, 8 * o * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
             R = T = nullptr;
         }
-        if (R) hip_check(hipMemcpy(R, plan->R.p, 2 * o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
-        if (T) hip_check(hipMemcpy(T, plan->T.p, 2 * o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
-        if (DEr) hip_check(hipMemcpy(DEr, plan->DEr.p, o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
-        if (DEt) hip_check(hipMemcpy(DEt, plan->DEt.p, o * sizeof(double), hipMemcpyDeviceToHost), "D2H");
-        if (status) hip_check(hipMemcpy(status, plan->status.p, plan->n_inst * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+        if (R) copy_out(R, plan->R.p, 2 * o * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
+        if (T) copy_out(T, plan->T.p, 2 * o * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
+        if (DEr) copy_out(DEr, plan->DEr.p, o * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
+        if (DEt) copy_out(DEt, plan->DEt.p, o * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
+        if (status) copy_out(status, plan->status.p, plan->n_inst * sizeof(int), hipMemcpyDeviceToHost, "D2H", plan);
         if (eig_info)
-            hip_check(hipMemcpy(eig_info, plan->eig_info.p, plan->n_inst * sizeof(int), hipMemcpyDeviceToHost), "D2H");
+            copy_out(eig_info, plan->eig_info.p, plan->n_inst * sizeof(int), hipMemcpyDeviceToHost, "D2H", plan);
     });
 }
 
@@ -700,8 +895,9 @@ int srcwa_plan_geometry(srcwa_plan* plan, double* out) {
         if (plan->kind == 1) invalid("srcwa_plan_geometry: planar plans carry no per-order geometry");
         hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
         hip_check(hipStreamSynchronize(plan->ctx->stream), "hipStreamSynchronize");
-        hip_check(hipMemcpy(out, plan->geometry.p, GEO_WORDS * plan->total_orders * sizeof(double),
-                            hipMemcpyDeviceToHost), "D2H");
+        hip_check(hipMemcpyAsync(out, plan->geometry.p, GEO_WORDS * plan->total_orders * sizeof(double),
+                                 hipMemcpyDeviceToHost, plan->ctx->stream), "D2H");
+        hip_check(hipStreamSynchronize(plan->ctx->stream), "hipStreamSynchronize");
     });
 }
 int64_t srcwa_plan_device_bytes(const srcwa_plan* plan) { return plan ? plan->bytes() : -1; }
@@ -1071,11 +1267,11 @@ PlanarCost crossed_cost(int N, const std::vector<bool>& patterned) {
 
 }  // namespace
 
-extern "C" int srcwa_plan2d_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
                                    int n_problems, const int* stack_index, const double* wavelength,
                                    const double* theta_deg, const double* phi_deg, const int* M1, const int* M2,
                                    const double* jones, const int* factorization, int64_t max_workspace_bytes,
-                                   srcwa_plan** out) {
+                                   srcwa_plan** out, int64_t* need) {
     return guarded([&] {
         if (!ctx || !out) invalid("srcwa_plan2d_create: null context or output");
         if (n_stacks < 1 || n_problems < 1 || !stacks || !stack_offsets || !stack_index || !wavelength ||
@@ -1109,6 +1305,7 @@ extern "C" int srcwa_plan2d_create(srcwa_context* ctx, int n_stacks, const doubl
                 }
         }
         auto plan = std::unique_ptr<srcwa_plan>(new srcwa_plan);
+        plan->bind_pool(ctx->pool);
         plan->ctx = ctx;
         plan->kind = 3;
         plan->n_inst = n_problems;
@@ -1214,6 +1411,15 @@ extern "C" int srcwa_plan2d_create(srcwa_context* ctx, int n_stacks, const doubl
         max_ws = std::max(max_ws, chunk_ws);
         plan->n_tasks = static_cast<int>(h_tasks.size());
         plan->total_orders = orders;
+        if (need) {
+            const auto b = [](size_t x) { return static_cast<int64_t>(srcwa_pool::round(x)); };
+            *need = b(h_insts.size() * sizeof(Inst2D)) + b(h_tasks.size() * sizeof(LayerTask)) +
+                    b(h_stacks.size() * sizeof(double)) + b(h_layouts.size() * sizeof(int64_t)) +
+                    b(max_ws * sizeof(cd)) + b(4 * std::max(1, plan->n_tasks) * sizeof(int)) +
+                    2 * b(8 * orders * sizeof(double)) + 2 * b(orders * sizeof(double)) +
+                    b(GEO_WORDS * orders * sizeof(double)) + 2 * b(n_problems * sizeof(int));
+            return;
+        }
         hipStream_t st = ctx->stream;
         plan->insts2d.upload(h_insts, st);
         plan->tasks.upload(h_tasks, st);
@@ -1232,4 +1438,27 @@ extern "C" int srcwa_plan2d_create(srcwa_context* ctx, int n_stacks, const doubl
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
         *out = plan.release();
     });
+}
+
+extern "C" int srcwa_plan2d_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                                   int n_problems, const int* stack_index, const double* wavelength,
+                                   const double* theta_deg, const double* phi_deg, const int* M1, const int* M2,
+                                   const double* jones, const int* factorization, int64_t max_workspace_bytes,
+                                   srcwa_plan** out) {
+    return plan2d_create_impl(ctx, n_stacks, stacks, stack_offsets, n_problems, stack_index, wavelength, theta_deg,
+           phi_deg, M1, M2, jones, factorization, max_workspace_bytes, out, nullptr);
+}
+
+extern "C" int srcwa_plan2d_bytes(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
+                                   int n_problems, const int* stack_index, const double* wavelength,
+                                   const double* theta_deg, const double* phi_deg, const int* M1, const int* M2,
+                                   const double* jones, const int* factorization, int64_t max_workspace_bytes,
+                                   int64_t* bytes) {
+    if (!bytes) {
+        g_error = "null output";
+        return SRCWA_EINVAL;
+    }
+    srcwa_plan* unused = nullptr;
+    return plan2d_create_impl(ctx, n_stacks, stacks, stack_offsets, n_problems, stack_index, wavelength, theta_deg,
+           phi_deg, M1, M2, jones, factorization, max_workspace_bytes, &unused, bytes);
 }

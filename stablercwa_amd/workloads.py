@@ -13,6 +13,10 @@
                         (N = 121 / 225 / 361): radius in [0.2, 0.4], pillar index
                         in [1.5, 3.5] + i[0, 0.05] in air, 1-3 patterned layers on substrate 1.46,
                         normal or oblique incidence (theta <= 40 deg), lambda in [1.0, 2.0].
+* ``served_mix``     -- a request stream for the server: 60 % planar TE/TM (M in [20, 150]; stacks from
+                        the conical generator's layer recipe, or the mirror stack), 25 % conical
+                        (M in [20, 80]) and 15 % crossed pillars (M1 = M2 in [3, 9]); each request is a
+                        sweep of 1-64 wavelengths around its instance's lambda; Poisson arrival times.
 """
 from __future__ import annotations
 
@@ -107,4 +111,49 @@ def pillar_batch(count: int = 512, seed: int = 0xC4, orders=(5, 7, 9), oblique_f
         if lam == 1.0:
             lam = 1.0 + 1e-9
         out.append(CrossedProblem(st, lam, theta, float(rng.uniform(0.0, 90.0)), M, M))
+    return out
+
+
+@dataclass
+class ServedRequest:
+    kind: str                 # "planar" | "conical" | "crossed"
+    stack: object
+    wavelengths: np.ndarray
+    theta: float
+    phi: float
+    M: int
+    pol: int = 0
+    arrival_s: float = 0.0
+
+
+def served_mix(count: int = 400, seed: int = 0xC5, rate_per_s: float = 50.0,
+               max_points: int = 64) -> List[ServedRequest]:
+    """Seeded mixed request stream with exponential inter-arrival times at rate_per_s."""
+    rng = np.random.default_rng(seed)
+    out = []
+    t = 0.0
+    for i in range(count):
+        t += float(rng.exponential(1.0 / rate_per_s))
+        u = rng.random()
+        points = int(rng.integers(1, max_points + 1))
+        if u < 0.60:
+            M = int(rng.integers(20, 151))
+            if rng.random() < 0.25:
+                st, lam, theta = mirror_stack(), 1.05, 66.36
+            else:
+                p = conical_batch(1, seed=int(rng.integers(1 << 31)), m_lo=M, m_hi=M)[0]
+                st, lam, theta = p.stack, p.wavelength, p.theta
+            lams = lam * (1.0 + 0.05 * np.linspace(-1.0, 1.0, points)) if points > 1 else np.array([lam])
+            out.append(ServedRequest("planar", st, lams, theta, 0.0, M, int(rng.integers(0, 2)), t))
+        elif u < 0.85:
+            M = int(rng.integers(20, 81))
+            p = conical_batch(1, seed=int(rng.integers(1 << 31)), m_lo=M, m_hi=M)[0]
+            lams = p.wavelength * (1.0 + 0.05 * np.linspace(-1.0, 1.0, points)) if points > 1 else np.array([p.wavelength])
+            out.append(ServedRequest("conical", p.stack, lams, p.theta, p.phi, M, 0, t))
+        else:
+            M = int(rng.integers(3, 10))
+            p = pillar_batch(1, seed=int(rng.integers(1 << 31)), orders=(M,))[0]
+            lams = p.wavelength * (1.0 + 0.02 * np.linspace(-1.0, 1.0, points)) if points > 1 else np.array([p.wavelength])
+            lams = np.where(np.abs(lams - 1.0) < 1e-9, 1.0 + 1e-6, lams)     # keep clear of the lambda = period anomaly
+            out.append(ServedRequest("crossed", p.stack, lams, p.theta, p.phi, M, 0, t))
     return out
