@@ -8,9 +8,13 @@ Workload (one step = one pass of the whole hot path over one batch, inputs resid
 
 Prints one JSON line: solves/s, the dominant kernel's FP64 roofline fraction (HIP-event timed),
 the reference CPU path timed on this host's cores on a bounded sample, and the plain
-one-solve-per-launch GPU form for comparison.
+one-solve-per-launch GPU form for comparison.  Extra fields hold the other workloads:
+  conical  -- 2048 seeded conical instances (M 20..120), one timed pass
+  crossed  -- 512 seeded circular-pillar crossed gratings (M1 = M2 in {5, 7, 9}), one timed pass
+  served   -- the request server under Poisson arrivals of mixed planar / conical / crossed sweeps:
+              throughput and p50 / p99 latency
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--no-extra] [--only LEG]
 """
 from __future__ import annotations
 
@@ -64,6 +68,101 @@ def cpu_baseline(stack, M, n_points):
                 sample=f"{k} wavelengths x TE+TM at M={M} (numpy oracle, 1 thread)")
 
 
+def _timed(batch, n):
+    batch.execute()
+    batch.sync()
+    t0 = time.perf_counter()
+    batch.execute()
+    batch.sync()
+    dt = time.perf_counter() - t0
+    ms = batch.stage_ms()
+    fl = batch.flops()
+    return dict(solves=n, seconds=dt, solves_per_s=n / dt, stages_ms=np.round(ms, 3).tolist(),
+                eig_tflops=float(fl[0] / (ms[1] * 1e-3) / 1e12) if ms[1] > 0 else None,
+                total_tflops=float(fl.sum() / dt / 1e12))
+
+
+def conical_leg(count=2048):
+    """Uneven-size conical batch: 2048 seeded instances, M uniform in [20, 120]; oracle parity on two."""
+    from oracle import conical
+    from stablercwa_amd import rcwa
+    from stablercwa_amd.workloads import conical_batch
+    probs = conical_batch(count)
+    b = rcwa.ConicalBatch([p.stack for p in probs], [(i, p.wavelength, p.theta, p.phi, p.M) for i, p in enumerate(probs)])
+    out = _timed(b, count)
+    res = b.results()
+    out["failed"] = int(np.sum(res["status"] != 0))
+    d = 0.0
+    for i in sorted(range(count), key=lambda k: probs[k].M)[:2]:
+        r = b.result(i, res)
+        o = conical.solve(probs[i].stack.flat(), probs[i].wavelength, probs[i].theta, probs[i].phi, probs[i].M)
+        d = max(d, float(np.max(np.abs(r.DEr - o["DEr"]))), float(np.max(np.abs(r.DEt - o["DEt"]))))
+    out["max_abs_dDE_vs_oracle_sample"] = d
+    out["workload"] = "2048 seeded conical instances, M in [20,120], 1-4 grating layers, FP64"
+    b.close()
+    return out
+
+
+def crossed_leg(count=512):
+    """2-D pillar batch: 512 seeded instances, M1 = M2 in {5, 7, 9}; oracle parity on one."""
+    from oracle import rcwa2d
+    from stablercwa_amd import rcwa
+    from stablercwa_amd.workloads import pillar_batch
+    probs = pillar_batch(count)
+    jones = (np.sin(np.radians(50.0)) + 0j, np.cos(np.radians(50.0)) + 0j)
+    b = rcwa.CrossedBatch([p.stack for p in probs],
+                          [(i, p.wavelength, p.theta, p.phi, p.M1, p.M2, jones) for i, p in enumerate(probs)])
+    out = _timed(b, count)
+    res = b.results()
+    out["failed"] = int(np.sum(res["status"] != 0))
+    i = min(range(count), key=lambda k: (probs[k].M1, len(probs[k].stack.layers)))
+    r = b.result(i, res)
+    o = rcwa2d.solve(probs[i].stack.flat(), probs[i].wavelength, probs[i].theta, probs[i].phi, probs[i].M1,
+                     probs[i].M2, jones)
+    out["max_abs_dDE_vs_oracle_sample"] = max(float(np.max(np.abs(r.DEr - o["DEr"]))),
+                                              float(np.max(np.abs(r.DEt - o["DEt"]))))
+    out["workload"] = "512 seeded circular-pillar crossed gratings, M1=M2 in {5,7,9}, 1-3 layers, FP64"
+    b.close()
+    return out
+
+
+def served_leg(rate, count):
+    """Served mixed workload: Poisson arrivals of 1-64 point sweeps (60 % planar, 25 % conical, 15 % crossed)."""
+    from stablercwa_amd.serve import Server
+    from stablercwa_amd.workloads import served_mix
+    reqs = served_mix(count, rate_per_s=rate)
+    jones = (0.6 + 0j, 0.8 + 0j)
+    with Server(hbm_gib=64, slots=4, max_batch_points=4096) as srv:
+        ids = []
+        t0 = time.perf_counter()
+        for r in reqs:
+            wait = r.arrival_s - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            if r.kind == "planar":
+                ids.append(srv.submit_planar(r.stack, r.wavelengths, r.theta, r.M, r.pol))
+            elif r.kind == "conical":
+                ids.append(srv.submit_conical(r.stack, r.wavelengths, r.theta, r.phi, r.M))
+            else:
+                ids.append(srv.submit_crossed(r.stack, r.wavelengths, r.theta, r.phi, r.M, r.M, jones))
+        res = [srv.result(i) for i in ids]
+        wall = time.perf_counter() - t0
+        stats = srv.stats()
+    lat = np.array([x["latency_s"] for x in res]) * 1e3
+    points = int(sum(x["points"] for x in res))
+    kinds = {}
+    for r, x in zip(reqs, res):
+        kinds.setdefault(r.kind, []).append(x["latency_s"] * 1e3)
+    return dict(rate_per_s=rate, requests=count, points=points, wall_s=wall, solves_per_s=points / wall,
+                p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                failed=int(sum(int(np.sum(x["status"] != 0)) for x in res)),
+                by_kind={k: dict(requests=len(v), p50_ms=float(np.percentile(v, 50)), p99_ms=float(np.percentile(v, 99)))
+                         for k, v in kinds.items()},
+                batches=stats["batches"], max_batches_in_flight=stats["max_inflight"],
+                workload="Poisson arrivals; 60% planar TE/TM M in [20,150], 25% conical M in [20,80], "
+                         "15% crossed pillars M in [3,9]; 1-64 wavelengths per request; 4 HIP streams, 64 GiB pool")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,10 +173,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-points", type=int, default=128)
     ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--no-extra", action="store_true", help="skip the conical, crossed and served legs")
+    ap.add_argument("--serve-rate", type=float, default=5.0, help="request arrivals per second (Poisson)")
+    ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--only", choices=["conical", "crossed", "served"],
+                    help="run one extra leg alone and print its result (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not provided"}))
         sys.exit(2)
+    if args.only:
+        leg = {"conical": conical_leg, "crossed": crossed_leg,
+               "served": lambda: served_leg(args.serve_rate, args.serve_requests)}[args.only]
+        print(json.dumps({args.only: leg()}), flush=True)
+        return
 
     from stablercwa_amd import rcwa
     st = mirror_stack()
@@ -144,6 +253,8 @@ def main():
             b.close()
 
     cpu = None if args.no_cpu else cpu_baseline(st, M, args.cpu_points)
+    extra = {} if args.no_extra else dict(conical=conical_leg(), crossed=crossed_leg(),
+                                          served=served_leg(args.serve_rate, args.serve_requests))
     line = {
         "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
@@ -160,6 +271,7 @@ def main():
         "plain_gpu": plain,
         "batched_over_plain": (value / plain["value"]) if plain else None,
     }
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 

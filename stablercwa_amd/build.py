@@ -31,14 +31,27 @@ FLAGS = [
 ]
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(OBJ, src + ".o")
+def _compile(src: str, obj_dir: str = OBJ, defines=()) -> str:
+    obj = os.path.join(obj_dir, src + ".o")
     lang = ["-x", "hip"] if src.endswith(".hip") else []
-    cmd = [HIPCC] + FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC] + FLAGS + [f"-D{d}" for d in defines] + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr[-6000:]}")
     return obj
+
+
+def build_variant(out_lib: str, defines=(), obj_dir: str = None) -> str:
+    """Builds the library with extra preprocessor definitions into out_lib (used by the tuner)."""
+    obj_dir = obj_dir or os.path.join(os.path.dirname(out_lib), "_obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+        objs = list(ex.map(lambda src: _compile(src, obj_dir, defines), SOURCES))
+    r = subprocess.run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", out_lib] + objs + ["-lpthread"],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
+    return out_lib
 
 
 def build(verbose: bool = False) -> str:

@@ -17,6 +17,7 @@
 // root, V = Q Z G^-1, and a dense exp(-k0 d G) propagation block.
 #include "batch.hpp"
 #include "eig.hpp"
+#include "wave_eig.hpp"
 #include "rcwa_common.hpp"
 
 // Conical layout table (int64 cd offsets), see conical_layout_* in batch.hpp.
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(NT) void k_assemble_conical(const Inst1D* __restric
 
 // ------------------------------------------------------------------------------------------------
 template <int QC>
-__global__ __launch_bounds__(NT) void k_eig_conical(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
+__global__ EIG_KERNEL_BOUNDS_OF(SRCWA_EIG_OCC_CONICAL) void k_eig_conical(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                                     const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                                     int* __restrict__ task_info, int max_iter) {
     const int t = blockIdx.x >> 1, which = blockIdx.x & 1;
@@ -224,6 +225,7 @@ __global__ __launch_bounds__(NT) void k_eig_conical(const Inst1D* __restrict__ i
     const Inst1D in = insts[tk.inst];
     const int64_t* lay = layouts + in.layout_off;
     const int N = in.N;
+    if (N <= WAVE_EIG_MAX_N) return;               // the wave eigensolver's matrix
     const ConicalSlots s = slots_of(ws + lay[conical_slot(in.L, tk.layer)], N);
     int info;
     if (which == 0) info = wg_eig<QC>(N, s.A, s.lam1, s.Z1, s.X1, EigScratch{s.ort1, s.tmp1}, max_iter, kEps);
@@ -231,6 +233,31 @@ __global__ __launch_bounds__(NT) void k_eig_conical(const Inst1D* __restrict__ i
     if (threadIdx.x == 0) task_info[4 * t + 1 + which] = info;
 }
 
+
+// One wave per eigenproblem (two per layer: A and B C), four per workgroup.
+template <int QC>
+__global__ __launch_bounds__(NT) void k_eig_conical_wave(int n_items, const Inst1D* __restrict__ insts,
+                                                         const LayerTask* __restrict__ tasks,
+                                                         const int64_t* __restrict__ layouts, cd* __restrict__ ws,
+                                                         int* __restrict__ task_info, int max_iter) {
+    __shared__ double s_rot[NWAVE][3 * (64 * QC + 1)];
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * NWAVE + wave_id());   // wave-uniform
+    if (item >= n_items) return;
+    const int t = item >> 1, which = item & 1;
+    if (task_info[4 * t] != 0) return;
+    const LayerTask tk = tasks[t];
+    const Inst1D in = insts[tk.inst];
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N;
+    if (N > WAVE_EIG_MAX_N) return;                // the workgroup eigensolver's matrix
+    const ConicalSlots s = slots_of(ws + lay[conical_slot(in.L, tk.layer)], N);
+    int info;
+    if (which == 0)
+        info = wave_eig<QC>(N, s.A, s.lam1, s.Z1, s.X1, EigScratch{s.ort1, s.tmp1}, s_rot[wave_id()], max_iter, kEps);
+    else
+        info = wave_eig<QC>(N, s.BC, s.lam2, s.Z2, s.X2, EigScratch{s.ort2, s.tmp2}, s_rot[wave_id()], max_iter, kEps);
+    if (lane_id() == 0) task_info[4 * t + 1 + which] = info;
+}
 
 // ------------------------------------------------------------------------------------------------
 // Schur-basis modes of a degenerate layer (rcwa1d_conical.hpp:413-502), order n = 2N.
@@ -649,13 +676,18 @@ __global__ __launch_bounds__(NT) void k_smatrix_conical(const Inst1D* __restrict
 
 // ------------------------------------------------------------------------------------------------
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
-                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int max_n,
+                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
                     hipEvent_t* ev) {
     if (ev) hipEventRecord(ev[0], s);
     k_setup_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);
     if (n_tasks > 0) k_assemble_conical<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
     if (ev) hipEventRecord(ev[1], s);
-    if (n_tasks > 0) EIG_LAUNCH(max_n, k_eig_conical, 2 * n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
+    // each matrix goes to the wave or the workgroup eigensolver by its own order
+    if (n_tasks > 0 && min_n <= WAVE_EIG_MAX_N)
+        EIG_WAVE_LAUNCH(WAVE_EIG_MAX_N, k_eig_conical_wave, 2 * n_tasks, s, insts, tasks, layouts, ws, task_info,
+                        max_iter);
+    if (n_tasks > 0 && max_n > WAVE_EIG_MAX_N)
+        EIG_LAUNCH(max_n, k_eig_conical, 2 * n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
     if (ev) hipEventRecord(ev[2], s);
     if (n_tasks > 0) EIG_LAUNCH(2 * max_n, k_modes_conical, n_tasks, s, insts, tasks, stacks, layouts, ws, task_info);
     k_smatrix_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);

@@ -27,10 +27,11 @@
 #pragma once
 
 #include "batch.hpp"
+#include "tuned.hpp"
 #include "wg_la.hpp"
 
 constexpr int EIG_MAX_N = EIG_MAX_N_HOST;   // largest order the LDS rotation table holds
-constexpr int EIG_RING = 8;                 // rows each column prefetches ahead in the QR sweep
+constexpr int EIG_RING = SRCWA_EIG_RING;    // rows each column prefetches ahead in the QR sweep
 
 // Columns per thread in the QR sweep for matrices of order n (the eigen kernels are instantiated
 // for 1, 2 and 4 so small matrices do not pay the registers of the largest).
@@ -50,6 +51,91 @@ struct EigScratch {
     cd* ort;     // n
     cd* tmp;     // n
 };
+
+// The left half of one QR sweep, run by a single wave: rotations G_{l+1..en} (complex cosine, real
+// sine, each formed from the diagonal carry of column i-1 and the real sub-diagonal h(i,i-1)) applied to
+// rows l..en of columns l..n-1, the rotations stored in rot_c / rot_ci / rot_s for the right half.
+// Lane c owns columns l + c + 64 q (q < LC); each column's next RD rows are loaded ahead into registers
+// (rotation i only touches rows i-1 and i, so later rows are final when prefetched).  The chain stays
+// inside one wave: the owner lane's values are broadcast with readlane, with no LDS round trip and no
+// workgroup barrier per rotation.  Requires n <= 64 LC; the caller orders memory afterwards.
+template <int LC, int RD>
+DEV void wave_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c, double* rot_ci, double* rot_s) {
+    const int lane = lane_id();
+    auto h = [&](int i, int j) -> cd& { return H[i + j * n]; };
+    cd carry[LC];
+    cd ring[LC][RD];
+#pragma unroll
+    for (int q = 0; q < LC; ++q) {
+        const int c = l + lane + 64 * q;
+        if (c < n) {
+            carry[q] = h(l, c);
+#pragma unroll
+            for (int u = 0; u < RD; ++u)
+                if (l + 1 + u <= en) ring[q][u] = h(l + 1 + u, c);
+        }
+    }
+    for (int ib = l + 1; ib <= en; ib += RD) {
+#pragma unroll
+        for (int u = 0; u < RD; ++u) {
+            const int i = ib + u;
+            if (i > en) break;
+            cd cur[LC];
+#pragma unroll
+            for (int q = 0; q < LC; ++q) {
+                const int c = l + lane + 64 * q;
+                cur[q] = ring[q][u];
+                if (i + RD <= en && c >= i + RD - 1 && c < n) ring[q][u] = h(i + RD, c);
+            }
+            // G_i from the owner column i-1: (h(i-1,i-1), h(i,i-1)) broadcast from its lane
+            const int owner = i - 1 - l;
+            const int oq = owner >> 6, ol = owner & 63;
+            cd xc{0.0, 0.0}, zc{0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < LC; ++q)
+                if (q == oq) {
+                    xc = carry[q];
+                    zc = cur[q];
+                }
+            double xr = read_lane(xc.x, ol), xi = read_lane(xc.y, ol), sub = read_lane(zc.x, ol);
+            const double mm = fmax(fmax(fabs(xr), fabs(xi)), fabs(sub));
+            int e = 0;
+            if (!(mm > 1e-140 && mm < 1e140)) {
+                e = binary_exponent(mm);
+                xr = ldexp(xr, -e);
+                xi = ldexp(xi, -e);
+                sub = ldexp(sub, -e);
+            }
+            const double nrm = sqrt(xr * xr + xi * xi + sub * sub);
+            const double inv = 1.0 / nrm;
+            const double cr = xr * inv, ci = xi * inv, sn = sub * inv;
+            if (lane == ol) {
+                h(i - 1, i - 1) = cd{e ? ldexp(nrm, e) : nrm, 0.0};
+                h(i, i - 1) = cd{0.0, 0.0};
+            }
+            if (lane == 0) {
+                rot_c[i] = cr;
+                rot_ci[i] = ci;
+                rot_s[i] = sn;
+            }
+#pragma unroll
+            for (int q = 0; q < LC; ++q) {
+                const int c = l + lane + 64 * q;
+                if (c >= i && c < n) {
+                    const cd yv = carry[q];
+                    const cd z = cur[q];
+                    h(i - 1, c) = cd{cr * yv.x + ci * yv.y + sn * z.x, cr * yv.y - ci * yv.x + sn * z.y};
+                    carry[q] = cd{cr * z.x - ci * z.y - sn * yv.x, cr * z.y + ci * z.x - sn * yv.y};
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < LC; ++q) {
+        const int c = l + lane + 64 * q;
+        if (c >= en && c < n) h(en, c) = carry[q];
+    }
+}
 
 // Eigen-decompose the n x n matrix A (column-major, leading dimension n) in place.
 //   in : A = the matrix (overwritten)

@@ -13,6 +13,7 @@
 //                  the efficiencies                        (rcwa1d.hpp:549-808, 963-1055)
 #include "batch.hpp"
 #include "eig.hpp"
+#include "wave_eig.hpp"
 #include "rcwa_common.hpp"
 
 // ------------------------------------------------------------------------------------------------
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ in
 
 // ------------------------------------------------------------------------------------------------
 template <int QC>
-__global__ __launch_bounds__(NT) void k_eig1d(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
+__global__ EIG_KERNEL_BOUNDS_OF(SRCWA_EIG_OCC_PLANAR) void k_eig1d(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                               const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                               int* __restrict__ task_info, int max_iter) {
     if (task_info[blockIdx.x] != 0) return;        // assembly failed: nothing to decompose
@@ -108,6 +109,7 @@ __global__ __launch_bounds__(NT) void k_eig1d(const Inst1D* __restrict__ insts, 
     const Inst1D in = insts[tk.inst];
     const int64_t* lay = layouts + in.layout_off;
     const int N = in.N;
+    if (N <= WAVE_EIG_MAX_N) return;               // the wave eigensolver's matrix
     const int64_t NN = (int64_t)N * N;
     cd* base = ws + lay[layout_layer(in.L, tk.layer)];
     cd *S0 = base, *S1 = base + NN, *S2 = base + 2 * NN;
@@ -115,6 +117,29 @@ __global__ __launch_bounds__(NT) void k_eig1d(const Inst1D* __restrict__ insts, 
     EigScratch s{base + 4 * NN + N, base + 4 * NN + 2 * N};
     const int info = wg_eig<QC>(N, S0, lam, S1, S2, s, max_iter, kEps);
     if (threadIdx.x == 0) task_info[blockIdx.x] = info;
+}
+
+// One wave per layer matrix (orders up to WAVE_EIG_MAX_N): four layers per workgroup.
+template <int QC>
+__global__ __launch_bounds__(NT) void k_eig1d_wave(int n_tasks, const Inst1D* __restrict__ insts,
+                                                   const LayerTask* __restrict__ tasks,
+                                                   const int64_t* __restrict__ layouts, cd* __restrict__ ws,
+                                                   int* __restrict__ task_info, int max_iter) {
+    __shared__ double s_rot[NWAVE][3 * (64 * QC + 1)];
+    const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * NWAVE + wave_id());   // wave-uniform
+    if (t >= n_tasks || task_info[t] != 0) return;
+    const LayerTask tk = tasks[t];
+    const Inst1D in = insts[tk.inst];
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N;
+    if (N > WAVE_EIG_MAX_N) return;                // the workgroup eigensolver's matrix
+    const int64_t NN = (int64_t)N * N;
+    cd* base = ws + lay[layout_layer(in.L, tk.layer)];
+    cd *S0 = base, *S1 = base + NN, *S2 = base + 2 * NN;
+    cd* lam = base + 4 * NN;
+    EigScratch s{base + 4 * NN + N, base + 4 * NN + 2 * N};
+    const int info = wave_eig<QC>(N, S0, lam, S1, S2, s, s_rot[wave_id()], max_iter, kEps);
+    if (lane_id() == 0) task_info[t] = info;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -259,13 +284,18 @@ __global__ __launch_bounds__(NT) void k_smatrix1d(const Inst1D* __restrict__ ins
 // ------------------------------------------------------------------------------------------------
 // host launchers
 void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
-                   const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int max_n,
+                   const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
                    hipEvent_t* ev) {
     if (ev) hipEventRecord(ev[0], s);
     k_setup1d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws);
     if (n_tasks > 0) k_assemble1d<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
     if (ev) hipEventRecord(ev[1], s);
-    if (n_tasks > 0) EIG_LAUNCH(max_n, k_eig1d, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
+    // each matrix goes to the wave or the workgroup eigensolver by its own order, so a result does not
+    // depend on what else is in the batch
+    if (n_tasks > 0 && min_n <= WAVE_EIG_MAX_N)
+        EIG_WAVE_LAUNCH(WAVE_EIG_MAX_N, k_eig1d_wave, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
+    if (n_tasks > 0 && max_n > WAVE_EIG_MAX_N)
+        EIG_LAUNCH(max_n, k_eig1d, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
     if (ev) hipEventRecord(ev[2], s);
     if (n_tasks > 0) k_modes1d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
     k_smatrix1d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);

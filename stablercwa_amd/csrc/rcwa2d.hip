@@ -15,6 +15,7 @@
 //                 the downward sweep, efficiencies and s/p amplitudes (rcwa2d.hpp:560-910, 1180-1267)
 #include "batch.hpp"
 #include "eig.hpp"
+#include "wave_eig.hpp"
 #include "rcwa_common.hpp"
 
 // ------------------------------------------------------------------------------------------------
@@ -506,7 +507,7 @@ __global__ __launch_bounds__(NT) void k_pq2d(const Inst2D* __restrict__ insts, c
 }
 
 template <int QC>
-__global__ __launch_bounds__(NT) void k_eig2d(const Inst2D* __restrict__ insts, const LayerTask* __restrict__ tasks,
+__global__ EIG_KERNEL_BOUNDS_OF(SRCWA_EIG_OCC_CROSSED) void k_eig2d(const Inst2D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                               const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                               int* __restrict__ task_info, int max_iter) {
     if (task_info[4 * blockIdx.x] != 0) return;
@@ -514,9 +515,30 @@ __global__ __launch_bounds__(NT) void k_eig2d(const Inst2D* __restrict__ insts, 
     const Inst2D in = insts[tk.inst];
     const int64_t* lay = layouts + in.layout_off;
     if (ws[lay[0] + 4 * in.N].x != 0.0) return;
+    if (2 * in.N <= WAVE_EIG_MAX_N) return;        // the wave eigensolver's matrix
     const Slots2D s = slots2d(ws, lay, in.L, tk.layer, in.N);
     const int info = wg_eig<QC>(2 * in.N, s.H, s.lam, s.Z, s.X, EigScratch{s.ort, s.tmp}, max_iter, kEps);
     if (threadIdx.x == 0) task_info[4 * blockIdx.x + 1] = info;
+}
+
+// One wave per layer eigenproblem of order 2N <= WAVE_EIG_MAX_N, four per workgroup.
+template <int QC>
+__global__ __launch_bounds__(NT) void k_eig2d_wave(int n_tasks, const Inst2D* __restrict__ insts,
+                                                   const LayerTask* __restrict__ tasks,
+                                                   const int64_t* __restrict__ layouts, cd* __restrict__ ws,
+                                                   int* __restrict__ task_info, int max_iter) {
+    __shared__ double s_rot[NWAVE][3 * (64 * QC + 1)];
+    const int t = __builtin_amdgcn_readfirstlane(blockIdx.x * NWAVE + wave_id());   // wave-uniform
+    if (t >= n_tasks || task_info[4 * t] != 0) return;
+    const LayerTask tk = tasks[t];
+    const Inst2D in = insts[tk.inst];
+    const int64_t* lay = layouts + in.layout_off;
+    if (ws[lay[0] + 4 * in.N].x != 0.0) return;
+    if (2 * in.N > WAVE_EIG_MAX_N) return;         // the workgroup eigensolver's matrix
+    const Slots2D s = slots2d(ws, lay, in.L, tk.layer, in.N);
+    const int info = wave_eig<QC>(2 * in.N, s.H, s.lam, s.Z, s.X, EigScratch{s.ort, s.tmp}, s_rot[wave_id()], max_iter,
+                                  kEps);
+    if (lane_id() == 0) task_info[4 * t + 1] = info;
 }
 
 // q = branch(sqrt(w)), V = (Q W) / q into Z (rcwa2d.hpp:499-533)
@@ -1153,7 +1175,7 @@ __global__ __launch_bounds__(NT) void k_smatrix2d(const Inst2D* __restrict__ ins
 
 // ------------------------------------------------------------------------------------------------
 void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts, const LayerTask* tasks,
-                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int max_n,
+                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
                     hipEvent_t* ev) {
     if (ev) hipEventRecord(ev[0], s);
     k_setup2d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);
@@ -1162,7 +1184,11 @@ void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts,
         k_pq2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
     }
     if (ev) hipEventRecord(ev[1], s);
-    if (n_tasks > 0) EIG_LAUNCH(2 * max_n, k_eig2d, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
+    // each matrix goes to the wave or the workgroup eigensolver by its own order
+    if (n_tasks > 0 && 2 * min_n <= WAVE_EIG_MAX_N)
+        EIG_WAVE_LAUNCH(WAVE_EIG_MAX_N, k_eig2d_wave, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
+    if (n_tasks > 0 && 2 * max_n > WAVE_EIG_MAX_N)
+        EIG_LAUNCH(2 * max_n, k_eig2d, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
     if (ev) hipEventRecord(ev[2], s);
     if (n_tasks > 0) k_modes2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
     k_smatrix2d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);

@@ -21,13 +21,13 @@
 
 // kernel launchers (rcwa1d.hip, dense.hip)
 void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
-                   const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int max_n,
+                   const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
                    hipEvent_t* ev);
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
-                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int max_n,
+                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
                     hipEvent_t* ev);
 void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts, const LayerTask* tasks,
-                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int max_n,
+                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
                     hipEvent_t* ev);
 void launch_zgemm_batched(hipStream_t s, int count, const DenseItem* items, cd* arena);
 void launch_zgesv_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info);
@@ -365,6 +365,7 @@ struct srcwa_plan {
     srcwa_context* ctx = nullptr;
     int kind = 1;
     int max_n = 1;                // largest harmonic count N of any instance (sizes the eigen kernels)
+    int min_n = 1 << 30;          // smallest N (whether the small-matrix eigen kernel runs at all)
     int n_inst = 0, n_tasks = 0;
     int64_t total_orders = 0;
     std::vector<PlanChunk> chunks;
@@ -587,6 +588,7 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
             in.M = M[i];
             in.N = N;
             plan->max_n = std::max(plan->max_n, N);
+            plan->min_n = std::min(plan->min_n, N);
             in.pol = pol[i];
             in.L = L;
             in.first_task = first_task - cur.t0;
@@ -747,6 +749,7 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
             in.M = M[i];
             in.N = N;
             plan->max_n = std::max(plan->max_n, N);
+            plan->min_n = std::min(plan->min_n, N);
             in.pol = 2;
             in.L = L;
             in.first_task = first_task - cur.t0;
@@ -829,15 +832,15 @@ int srcwa_plan_execute(srcwa_plan* plan) {
                       plan->kind == 1 ? nullptr : plan->geometry.p};
             if (plan->kind == 1)
                 launch_planar(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts.p + ch.i0, plan->tasks.p + ch.t0,
-                              plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + ch.t0, out, 30, plan->max_n,
+                              plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + ch.t0, out, 30, plan->min_n, plan->max_n,
                               plan->ev.data() + 4 * c);
             else if (plan->kind == 2)
                 launch_conical(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts.p + ch.i0, plan->tasks.p + ch.t0,
-                               plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + 4 * ch.t0, out, 30, plan->max_n,
+                               plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + 4 * ch.t0, out, 30, plan->min_n, plan->max_n,
                                plan->ev.data() + 4 * c);
             else
                 launch_crossed(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts2d.p + ch.i0, plan->tasks.p + ch.t0,
-                               plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + 4 * ch.t0, out, 30, plan->max_n,
+                               plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + 4 * ch.t0, out, 30, plan->min_n, plan->max_n,
                                plan->ev.data() + 4 * c);
             hip_check(hipGetLastError(), "kernel launch");
         }
@@ -1389,6 +1392,7 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
             in.M2 = M2[i];
             in.N = N;
             plan->max_n = std::max(plan->max_n, N);
+            plan->min_n = std::min(plan->min_n, N);
             in.L = L;
             in.first_task = first_task - cur.t0;
             in.n_tasks = static_cast<int>(h_tasks.size()) - first_task;

@@ -1,0 +1,33 @@
+// Compile-time tuning constants of the eigen kernels (override with -D at build time; the values
+// here are the ones measured best on MI355X by stablercwa_amd/tune.py).
+#pragma once
+
+#ifndef SRCWA_EIG_RING
+#define SRCWA_EIG_RING 2        // rows each column prefetches ahead in the QR sweep
+#endif
+// minimum waves per SIMD each family's workgroup eigen kernel is built for (0: the compiler's choice)
+#ifndef SRCWA_EIG_OCC_PLANAR
+#define SRCWA_EIG_OCC_PLANAR 3
+#endif
+#ifndef SRCWA_EIG_OCC_CONICAL
+#define SRCWA_EIG_OCC_CONICAL 0
+#endif
+#ifndef SRCWA_EIG_OCC_CROSSED
+#define SRCWA_EIG_OCC_CROSSED 0
+#endif
+#ifndef SRCWA_WAVE_EIG_MAX_N
+#define SRCWA_WAVE_EIG_MAX_N 64 // orders up to which one wave solves a matrix
+#endif
+
+// SRCWA_EIG_OCC (one value for every family) is what the tuner's variant builds set
+#ifdef SRCWA_EIG_OCC
+#undef SRCWA_EIG_OCC_PLANAR
+#undef SRCWA_EIG_OCC_CONICAL
+#undef SRCWA_EIG_OCC_CROSSED
+#define SRCWA_EIG_OCC_PLANAR SRCWA_EIG_OCC
+#define SRCWA_EIG_OCC_CONICAL SRCWA_EIG_OCC
+#define SRCWA_EIG_OCC_CROSSED SRCWA_EIG_OCC
+#endif
+
+// (a minimum of one wave per SIMD is the same as no bound)
+#define EIG_KERNEL_BOUNDS_OF(occ) __launch_bounds__(NT, ((occ) > 0 ? (occ) : 1))

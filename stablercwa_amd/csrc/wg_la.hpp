@@ -36,6 +36,26 @@ DEV double wave_max(double v) {
     return v;
 }
 
+// Orders the memory operations of the lanes of one wave (wave-scope release/acquire).
+DEV void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+DEV double read_lane(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+DEV int wave_max_int(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 // Sum over the workgroup; every thread receives the result.
 DEV double block_sum(double v) {
     __shared__ double red[NWAVE];
@@ -99,13 +119,15 @@ DEV int block_argmax(double v, int idx) {
 // result element r of lane l is D[l/16 + 4*r][l%16] (checked on gfx950 against numpy).
 enum { GEMM_SET = 0, GEMM_SUB = 1, GEMM_ADD = 2 };
 
-template <int MODE, bool BT>
+// WAVE = true: the calling wave computes every tile alone, with no workgroup barrier (for code in
+// which each wave works on its own matrices).
+template <int MODE, bool BT, bool WAVE = false>
 DEV void wg_gemm(int m, int n, int k, const cd* __restrict__ A, int lda, const cd* __restrict__ B, int ldb,
                  cd* C, int ldc) {
     const int lane = lane_id();
     const int li = lane & 15, lk = lane >> 4;
     const int tm = (m + 31) >> 5, tn = (n + 31) >> 5, tiles = tm * tn;
-    for (int t = wave_id(); t < tiles; t += NWAVE) {
+    for (int t = WAVE ? 0 : wave_id(); t < tiles; t += WAVE ? 1 : NWAVE) {
         const int r0 = (t % tm) << 5, c0 = (t / tm) << 5;
         d4 re[2][2], im[2][2];
 #pragma unroll
@@ -155,7 +177,7 @@ DEV void wg_gemm(int m, int n, int k, const cd* __restrict__ A, int lda, const c
                 }
             }
     }
-    __syncthreads();
+    if (!WAVE) __syncthreads();
 }
 
 // ------------------------------------------------------------------------------------------------
