@@ -249,6 +249,16 @@ def main():
             b.sync()
         tq = time.perf_counter()
         plain = dict(value=args.plain_sample / (tq - tp), unit="solves/s", sample=f"{args.plain_sample} solves, one per launch")
+        # the same single-solve plans replayed from captured HIP graphs
+        for b in singles:
+            b.use_graph(True)
+        singles[0].execute(); singles[0].sync()
+        tp = time.perf_counter()
+        for b in singles:
+            b.execute()
+            b.sync()
+        tq = time.perf_counter()
+        plain["graph_replay_value"] = args.plain_sample / (tq - tp)
         for b in singles:
             b.close()
 

@@ -66,7 +66,10 @@ int srcwa_context_set_pool(srcwa_context* ctx, srcwa_pool* pool);
  *
  * stacks: n_stacks flat records concatenated; stack_offsets[s] = start (in doubles) of record s.
  * max_workspace_bytes bounds the device workspace of one launch chunk (0 = 64 GiB); problems are
- * admitted into chunks in order until the bound is reached.
+ * admitted into chunks in order until the bound is reached.  With several chunks, execution runs on
+ * the context stream and a second stream of the context: the eigensolves of chunk c+1 overlap the
+ * S-matrix stage of chunk c, and two workspace regions alternate between chunks (device workspace =
+ * 2 x the largest chunk's).  Results do not depend on the chunking.
  */
 int srcwa_plan1d_create(srcwa_context* ctx, int n_stacks, const double* stacks, const int64_t* stack_offsets,
                         int n_problems, const int* stack_index, const double* wavelength, const double* theta_deg,
@@ -116,6 +119,11 @@ int srcwa_plan2d_bytes(srcwa_context* ctx, int n_stacks, const double* stacks, c
                        const double* phi_deg, const int* M1, const int* M2, const double* jones,
                        const int* factorization, int64_t max_workspace_bytes, int64_t* bytes);
 
+/* Capture the plan's launch sequence into a HIP graph (enable = 1) that srcwa_plan_execute then
+ * replays with one hipGraphLaunch; enable = 0 returns to launching kernel by kernel.  One-chunk plans
+ * only (SRCWA_EUNSUPPORTED otherwise).  A replayed plan reports only the total in srcwa_plan_stage_ms
+ * (the stage entries are -1). */
+int srcwa_plan_set_graph(srcwa_plan* plan, int enable);
 /* Launch all chunks of the plan on the context stream (asynchronous). */
 int srcwa_plan_execute(srcwa_plan* plan);
 /* 1 when the last execute has finished, 0 while it runs, a negative code on a device error. */

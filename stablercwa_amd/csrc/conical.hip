@@ -677,19 +677,26 @@ __global__ __launch_bounds__(NT) void k_smatrix_conical(const Inst1D* __restrict
 // ------------------------------------------------------------------------------------------------
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    hipEvent_t* ev) {
-    if (ev) hipEventRecord(ev[0], s);
-    k_setup_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);
-    if (n_tasks > 0) k_assemble_conical<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
-    if (ev) hipEventRecord(ev[1], s);
-    // each matrix goes to the wave or the workgroup eigensolver by its own order
-    if (n_tasks > 0 && min_n <= WAVE_EIG_MAX_N)
-        EIG_WAVE_LAUNCH(WAVE_EIG_MAX_N, k_eig_conical_wave, 2 * n_tasks, s, insts, tasks, layouts, ws, task_info,
-                        max_iter);
-    if (n_tasks > 0 && max_n > WAVE_EIG_MAX_N)
-        EIG_LAUNCH(max_n, k_eig_conical, 2 * n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
-    if (ev) hipEventRecord(ev[2], s);
-    if (n_tasks > 0) EIG_LAUNCH(2 * max_n, k_modes_conical, n_tasks, s, insts, tasks, stacks, layouts, ws, task_info);
-    k_smatrix_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
-    if (ev) hipEventRecord(ev[3], s);
+                    int part, hipEvent_t* ev) {
+    // part 1: setup, assembly, eigensolves (events ev[0..2]); part 2: modes, S-matrix, efficiencies
+    // (events ev[3], ev[4]); the host may run the parts of different chunks on two streams
+    if (part & 1) {
+        if (ev) hipEventRecord(ev[0], s);
+        k_setup_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);
+        if (n_tasks > 0) k_assemble_conical<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
+        if (ev) hipEventRecord(ev[1], s);
+        // each matrix goes to the wave or the workgroup eigensolver by its own order
+        if (n_tasks > 0 && min_n <= WAVE_EIG_MAX_N)
+            EIG_WAVE_LAUNCH(WAVE_EIG_MAX_N, k_eig_conical_wave, 2 * n_tasks, s, insts, tasks, layouts, ws, task_info,
+                            max_iter);
+        if (n_tasks > 0 && max_n > WAVE_EIG_MAX_N)
+            EIG_LAUNCH(max_n, k_eig_conical, 2 * n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
+        if (ev) hipEventRecord(ev[2], s);
+    }
+    if (part & 2) {
+        if (ev) hipEventRecord(ev[3], s);
+        if (n_tasks > 0) EIG_LAUNCH(2 * max_n, k_modes_conical, n_tasks, s, insts, tasks, stacks, layouts, ws, task_info);
+        k_smatrix_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
+        if (ev) hipEventRecord(ev[4], s);
+    }
 }

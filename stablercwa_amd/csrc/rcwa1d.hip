@@ -285,19 +285,26 @@ __global__ __launch_bounds__(NT) void k_smatrix1d(const Inst1D* __restrict__ ins
 // host launchers
 void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                   hipEvent_t* ev) {
-    if (ev) hipEventRecord(ev[0], s);
-    k_setup1d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws);
-    if (n_tasks > 0) k_assemble1d<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
-    if (ev) hipEventRecord(ev[1], s);
-    // each matrix goes to the wave or the workgroup eigensolver by its own order, so a result does not
-    // depend on what else is in the batch
-    if (n_tasks > 0 && min_n <= WAVE_EIG_MAX_N)
-        EIG_WAVE_LAUNCH(WAVE_EIG_MAX_N, k_eig1d_wave, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
-    if (n_tasks > 0 && max_n > WAVE_EIG_MAX_N)
-        EIG_LAUNCH(max_n, k_eig1d, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
-    if (ev) hipEventRecord(ev[2], s);
-    if (n_tasks > 0) k_modes1d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
-    k_smatrix1d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
-    if (ev) hipEventRecord(ev[3], s);
+                   int part, hipEvent_t* ev) {
+    // part 1: setup, assembly, eigensolves (events ev[0..2]); part 2: modes, S-matrix, efficiencies
+    // (events ev[3], ev[4]); the host may run the parts of different chunks on two streams
+    if (part & 1) {
+        if (ev) hipEventRecord(ev[0], s);
+        k_setup1d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws);
+        if (n_tasks > 0) k_assemble1d<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
+        if (ev) hipEventRecord(ev[1], s);
+        // each matrix goes to the wave or the workgroup eigensolver by its own order, so a result does not
+        // depend on what else is in the batch
+        if (n_tasks > 0 && min_n <= WAVE_EIG_MAX_N)
+            EIG_WAVE_LAUNCH(WAVE_EIG_MAX_N, k_eig1d_wave, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
+        if (n_tasks > 0 && max_n > WAVE_EIG_MAX_N)
+            EIG_LAUNCH(max_n, k_eig1d, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
+        if (ev) hipEventRecord(ev[2], s);
+    }
+    if (part & 2) {
+        if (ev) hipEventRecord(ev[3], s);
+        if (n_tasks > 0) k_modes1d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
+        k_smatrix1d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
+        if (ev) hipEventRecord(ev[4], s);
+    }
 }

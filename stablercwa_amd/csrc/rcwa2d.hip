@@ -1176,21 +1176,28 @@ __global__ __launch_bounds__(NT) void k_smatrix2d(const Inst2D* __restrict__ ins
 // ------------------------------------------------------------------------------------------------
 void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    hipEvent_t* ev) {
-    if (ev) hipEventRecord(ev[0], s);
-    k_setup2d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);
-    if (n_tasks > 0) {
-        k_ops2d<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
-        k_pq2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
+                    int part, hipEvent_t* ev) {
+    // part 1: setup, assembly, eigensolves (events ev[0..2]); part 2: modes, S-matrix, efficiencies
+    // (events ev[3], ev[4]); the host may run the parts of different chunks on two streams
+    if (part & 1) {
+        if (ev) hipEventRecord(ev[0], s);
+        k_setup2d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);
+        if (n_tasks > 0) {
+            k_ops2d<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
+            k_pq2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
+        }
+        if (ev) hipEventRecord(ev[1], s);
+        // each matrix goes to the wave or the workgroup eigensolver by its own order
+        if (n_tasks > 0 && 2 * min_n <= WAVE_EIG_MAX_N)
+            EIG_WAVE_LAUNCH(WAVE_EIG_MAX_N, k_eig2d_wave, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
+        if (n_tasks > 0 && 2 * max_n > WAVE_EIG_MAX_N)
+            EIG_LAUNCH(2 * max_n, k_eig2d, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
+        if (ev) hipEventRecord(ev[2], s);
     }
-    if (ev) hipEventRecord(ev[1], s);
-    // each matrix goes to the wave or the workgroup eigensolver by its own order
-    if (n_tasks > 0 && 2 * min_n <= WAVE_EIG_MAX_N)
-        EIG_WAVE_LAUNCH(WAVE_EIG_MAX_N, k_eig2d_wave, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
-    if (n_tasks > 0 && 2 * max_n > WAVE_EIG_MAX_N)
-        EIG_LAUNCH(2 * max_n, k_eig2d, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
-    if (ev) hipEventRecord(ev[2], s);
-    if (n_tasks > 0) k_modes2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
-    k_smatrix2d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
-    if (ev) hipEventRecord(ev[3], s);
+    if (part & 2) {
+        if (ev) hipEventRecord(ev[3], s);
+        if (n_tasks > 0) k_modes2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
+        k_smatrix2d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
+        if (ev) hipEventRecord(ev[4], s);
+    }
 }

@@ -22,13 +22,13 @@
 // kernel launchers (rcwa1d.hip, dense.hip)
 void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                   hipEvent_t* ev);
+                   int part, hipEvent_t* ev);
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    hipEvent_t* ev);
+                    int part, hipEvent_t* ev);
 void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    hipEvent_t* ev);
+                    int part, hipEvent_t* ev);
 void launch_zgemm_batched(hipStream_t s, int count, const DenseItem* items, cd* arena);
 void launch_zgesv_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info);
 void launch_zgeev_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info, int max_iter,
@@ -354,6 +354,7 @@ PlanarCost planar_cost(int N, const std::vector<bool>& general, int pol, bool co
 struct srcwa_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t back = nullptr;     // second stream, created for the first plan with several chunks
     srcwa_pool* pool = nullptr;     // plans created on this context draw device memory from it
 };
 
@@ -379,12 +380,21 @@ struct srcwa_plan {
     DevBuf<double> R, T, DEr, DEt;
     DevBuf<double> geometry;      // conical / crossed: GEO_WORDS per order
     DevBuf<int> status, eig_info;
-    std::vector<hipEvent_t> ev;      // 4 per chunk: start, after assembly, after eigensolves, end
+    // 5 timing events per chunk: front start, after assembly, after eigensolves (front stream); back
+    // start, back end (back stream); then the final event on the front stream
+    std::vector<hipEvent_t> ev;
+    std::vector<hipEvent_t> front_done, back_done;   // ordering only
+    int64_t region_words = 0;        // workspace of one chunk; two regions alternate between chunks
+    hipGraph_t graph = nullptr;      // captured launch sequence (srcwa_plan_set_graph)
+    hipGraphExec_t graph_exec = nullptr;
     bool timed = false;
     double flops[2] = {0.0, 0.0};
     ~srcwa_plan() {
-        for (hipEvent_t& e : ev)
-            if (e) (void)hipEventDestroy(e);
+        for (std::vector<hipEvent_t>* v : {&ev, &front_done, &back_done})
+            for (hipEvent_t& e : *v)
+                if (e) (void)hipEventDestroy(e);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
     }
     // every device buffer of the plan comes from the context's pool, when it has one
     void bind_pool(srcwa_pool* p) {
@@ -404,8 +414,16 @@ struct srcwa_plan {
         eig_info.pool = p;
     }
     void make_events() {
-        ev.assign(4 * chunks.size(), nullptr);
+        const size_t c = chunks.size();
+        ev.assign(5 * c + 1, nullptr);
         for (hipEvent_t& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+        if (c > 1) {
+            front_done.assign(c, nullptr);
+            back_done.assign(c, nullptr);
+            for (std::vector<hipEvent_t>* v : {&front_done, &back_done})
+                for (hipEvent_t& e : *v) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+            if (!ctx->back) hip_check(hipStreamCreateWithFlags(&ctx->back, hipStreamNonBlocking), "hipStreamCreate");
+        }
     }
     int64_t bytes() const {
         return insts.bytes() + insts2d.bytes() + tasks.bytes() + stacks.bytes() + layouts.bytes() + ws.bytes() + task_info.bytes() +
@@ -434,6 +452,48 @@ int dense_batch(srcwa_context* ctx, Body&& body) {
 
 }  // namespace
 
+namespace {
+// Enqueues every chunk of the plan on the context's stream(s); with timing, the stage events too.
+void enqueue(srcwa_plan* plan, bool timing) {
+    hipStream_t front = plan->ctx->stream;
+    const size_t nc = plan->chunks.size();
+    // One chunk: every stage on the context stream.  Several chunks: the eigensolves of chunk c+1
+    // (front stream) overlap the S-matrix work of chunk c (back stream); chunks alternate between two
+    // workspace regions, so chunk c+2 starts only after chunk c's back half has released its region.
+    hipStream_t back = nc > 1 ? plan->ctx->back : front;
+    auto launch = [&](size_t c, hipStream_t st, int part) {
+        const PlanChunk& ch = plan->chunks[c];
+        Out1D out{plan->R.p, plan->T.p, plan->DEr.p, plan->DEt.p, plan->status.p + ch.i0, plan->eig_info.p + ch.i0,
+                  plan->kind == 1 ? nullptr : plan->geometry.p};
+        cd* ws = plan->ws.p + (nc > 1 ? static_cast<int64_t>(c % 2) * plan->region_words : 0);
+        hipEvent_t* ev = timing ? plan->ev.data() + 5 * c : nullptr;
+        const int ni = ch.i1 - ch.i0, nt = ch.t1 - ch.t0;
+        if (plan->kind == 1)
+            launch_planar(st, ni, nt, plan->insts.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p, plan->layouts.p,
+                          ws, plan->task_info.p + ch.t0, out, 30, plan->min_n, plan->max_n, part, ev);
+        else if (plan->kind == 2)
+            launch_conical(st, ni, nt, plan->insts.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p, plan->layouts.p,
+                           ws, plan->task_info.p + 4 * ch.t0, out, 30, plan->min_n, plan->max_n, part, ev);
+        else
+            launch_crossed(st, ni, nt, plan->insts2d.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p,
+                           plan->layouts.p, ws, plan->task_info.p + 4 * ch.t0, out, 30, plan->min_n, plan->max_n,
+                           part, ev);
+        hip_check(hipGetLastError(), "kernel launch");
+    };
+    for (size_t c = 0; c < nc; ++c) {
+        if (nc > 1 && c >= 2) hip_check(hipStreamWaitEvent(front, plan->back_done[c - 2], 0), "hipStreamWaitEvent");
+        launch(c, front, 1);
+        if (nc > 1) {
+            hip_check(hipEventRecord(plan->front_done[c], front), "hipEventRecord");
+            hip_check(hipStreamWaitEvent(back, plan->front_done[c], 0), "hipStreamWaitEvent");
+        }
+        launch(c, back, 2);
+        if (nc > 1) hip_check(hipEventRecord(plan->back_done[c], back), "hipEventRecord");
+    }
+    if (nc > 1) hip_check(hipStreamWaitEvent(front, plan->back_done[nc - 1], 0), "hipStreamWaitEvent");
+}
+}  // namespace
+
 extern "C" {
 
 const char* srcwa_last_error(void) { return g_error.c_str(); }
@@ -460,10 +520,11 @@ int srcwa_context_create(int device, srcwa_context** out) {
 void srcwa_context_destroy(srcwa_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
-    if (ctx->stream) {
-        (void)hipStreamSynchronize(ctx->stream);
-        (void)hipStreamDestroy(ctx->stream);
-    }
+    for (hipStream_t st : {ctx->stream, ctx->back})
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
     delete ctx;
 }
 
@@ -615,7 +676,7 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
             const auto b = [](size_t x) { return static_cast<int64_t>(srcwa_pool::round(x)); };
             *need = b(h_insts.size() * sizeof(Inst1D)) + b(h_tasks.size() * sizeof(LayerTask)) +
                     b(h_stacks.size() * sizeof(double)) + b(h_layouts.size() * sizeof(int64_t)) +
-                    b(max_ws * sizeof(cd)) + b(std::max(1, plan->n_tasks) * sizeof(int)) +
+                    b((plan->chunks.size() > 1 ? 2 : 1) * max_ws * sizeof(cd)) + b(std::max(1, plan->n_tasks) * sizeof(int)) +
                     2 * b(2 * orders * sizeof(double)) + 2 * b(orders * sizeof(double)) + 2 * b(n_problems * sizeof(int));
             return;
         }
@@ -624,7 +685,8 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         plan->tasks.upload(h_tasks, st);
         plan->stacks.upload(h_stacks, st);
         plan->layouts.upload(h_layouts, st);
-        plan->ws.alloc(static_cast<size_t>(max_ws));
+        plan->region_words = max_ws;
+        plan->ws.alloc(static_cast<size_t>(plan->chunks.size() > 1 ? 2 * max_ws : max_ws));
         plan->task_info.alloc(std::max(1, plan->n_tasks));
         plan->R.alloc(2 * orders);
         plan->T.alloc(2 * orders);
@@ -775,7 +837,7 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
             const auto b = [](size_t x) { return static_cast<int64_t>(srcwa_pool::round(x)); };
             *need = b(h_insts.size() * sizeof(Inst1D)) + b(h_tasks.size() * sizeof(LayerTask)) +
                     b(ps.words.size() * sizeof(double)) + b(h_layouts.size() * sizeof(int64_t)) +
-                    b(max_ws * sizeof(cd)) + b(4 * std::max(1, plan->n_tasks) * sizeof(int)) +
+                    b((plan->chunks.size() > 1 ? 2 : 1) * max_ws * sizeof(cd)) + b(4 * std::max(1, plan->n_tasks) * sizeof(int)) +
                     2 * b(8 * orders * sizeof(double)) + 2 * b(4 * orders * sizeof(double)) +
                     b(GEO_WORDS * orders * sizeof(double)) + 2 * b(n_problems * sizeof(int));
             return;
@@ -785,7 +847,8 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
         plan->tasks.upload(h_tasks, st);
         plan->stacks.upload(ps.words, st);
         plan->layouts.upload(h_layouts, st);
-        plan->ws.alloc(static_cast<size_t>(max_ws));
+        plan->region_words = max_ws;
+        plan->ws.alloc(static_cast<size_t>(plan->chunks.size() > 1 ? 2 * max_ws : max_ws));
         plan->task_info.alloc(4 * std::max(1, plan->n_tasks));
         plan->R.alloc(8 * orders);
         plan->T.alloc(8 * orders);
@@ -825,26 +888,46 @@ int srcwa_plan_execute(srcwa_plan* plan) {
     return guarded([&] {
         if (!plan) invalid("null plan");
         hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
-        hipStream_t s = plan->ctx->stream;
-        for (size_t c = 0; c < plan->chunks.size(); ++c) {
-            const PlanChunk& ch = plan->chunks[c];
-            Out1D out{plan->R.p, plan->T.p, plan->DEr.p, plan->DEt.p, plan->status.p + ch.i0, plan->eig_info.p + ch.i0,
-                      plan->kind == 1 ? nullptr : plan->geometry.p};
-            if (plan->kind == 1)
-                launch_planar(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts.p + ch.i0, plan->tasks.p + ch.t0,
-                              plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + ch.t0, out, 30, plan->min_n, plan->max_n,
-                              plan->ev.data() + 4 * c);
-            else if (plan->kind == 2)
-                launch_conical(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts.p + ch.i0, plan->tasks.p + ch.t0,
-                               plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + 4 * ch.t0, out, 30, plan->min_n, plan->max_n,
-                               plan->ev.data() + 4 * c);
-            else
-                launch_crossed(s, ch.i1 - ch.i0, ch.t1 - ch.t0, plan->insts2d.p + ch.i0, plan->tasks.p + ch.t0,
-                               plan->stacks.p, plan->layouts.p, plan->ws.p, plan->task_info.p + 4 * ch.t0, out, 30, plan->min_n, plan->max_n,
-                               plan->ev.data() + 4 * c);
-            hip_check(hipGetLastError(), "kernel launch");
+        hipStream_t front = plan->ctx->stream;
+        if (plan->graph_exec) {
+            hip_check(hipEventRecord(plan->ev.front(), front), "hipEventRecord");
+            hip_check(hipGraphLaunch(plan->graph_exec, front), "hipGraphLaunch");
+        } else {
+            enqueue(plan, true);
         }
+        hip_check(hipEventRecord(plan->ev.back(), front), "hipEventRecord");
         plan->timed = true;
+    });
+}
+
+int srcwa_plan_set_graph(srcwa_plan* plan, int enable) {
+    return guarded([&] {
+        if (!plan) invalid("null plan");
+        hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+        if (plan->graph_exec) {
+            hip_check(hipGraphExecDestroy(plan->graph_exec), "hipGraphExecDestroy");
+            plan->graph_exec = nullptr;
+        }
+        if (plan->graph) {
+            hip_check(hipGraphDestroy(plan->graph), "hipGraphDestroy");
+            plan->graph = nullptr;
+        }
+        if (!enable) return;
+        if (plan->chunks.size() > 1)
+            throw Failure{SRCWA_EUNSUPPORTED, "srcwa_plan_set_graph: graph replay needs a plan of one chunk"};
+        hipStream_t st = plan->ctx->stream;
+        hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+        try {
+            enqueue(plan, false);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(st, &plan->graph), "hipStreamEndCapture");
+        hip_check(hipGraphInstantiate(&plan->graph_exec, plan->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     });
 }
 
@@ -911,24 +994,24 @@ int srcwa_plan_stage_ms(const srcwa_plan* plan, double* ms4) {
         if (!plan || !ms4) invalid("null plan or output");
         if (!plan->timed) invalid("plan has not been executed");
         hip_check(hipEventSynchronize(plan->ev.back()), "hipEventSynchronize");
-        double a = 0, b = 0, c = 0, t = 0;
-        for (size_t k = 0; k < plan->chunks.size(); ++k) {
-            const hipEvent_t* e = plan->ev.data() + 4 * k;
+        double a = 0, b = 0, c = 0;
+        if (plan->graph_exec) a = b = c = -1.0;     // a replayed graph carries no stage events
+        for (size_t k = 0; k < plan->chunks.size() && !plan->graph_exec; ++k) {
+            const hipEvent_t* e = plan->ev.data() + 5 * k;
             float x = 0, y = 0, z = 0;
             hip_check(hipEventElapsedTime(&x, e[0], e[1]), "hipEventElapsedTime");
             hip_check(hipEventElapsedTime(&y, e[1], e[2]), "hipEventElapsedTime");
-            hip_check(hipEventElapsedTime(&z, e[2], e[3]), "hipEventElapsedTime");
+            hip_check(hipEventElapsedTime(&z, e[3], e[4]), "hipEventElapsedTime");
             a += x;
             b += y;
             c += z;
         }
         float tt = 0;
         hip_check(hipEventElapsedTime(&tt, plan->ev.front(), plan->ev.back()), "hipEventElapsedTime");
-        t = tt;
         ms4[0] = a;
         ms4[1] = b;
         ms4[2] = c;
-        ms4[3] = t;
+        ms4[3] = tt;
     });
 }
 
@@ -1419,7 +1502,7 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
             const auto b = [](size_t x) { return static_cast<int64_t>(srcwa_pool::round(x)); };
             *need = b(h_insts.size() * sizeof(Inst2D)) + b(h_tasks.size() * sizeof(LayerTask)) +
                     b(h_stacks.size() * sizeof(double)) + b(h_layouts.size() * sizeof(int64_t)) +
-                    b(max_ws * sizeof(cd)) + b(4 * std::max(1, plan->n_tasks) * sizeof(int)) +
+                    b((plan->chunks.size() > 1 ? 2 : 1) * max_ws * sizeof(cd)) + b(4 * std::max(1, plan->n_tasks) * sizeof(int)) +
                     2 * b(8 * orders * sizeof(double)) + 2 * b(orders * sizeof(double)) +
                     b(GEO_WORDS * orders * sizeof(double)) + 2 * b(n_problems * sizeof(int));
             return;
@@ -1429,7 +1512,8 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         plan->tasks.upload(h_tasks, st);
         plan->stacks.upload(h_stacks, st);
         plan->layouts.upload(h_layouts, st);
-        plan->ws.alloc(static_cast<size_t>(max_ws));
+        plan->region_words = max_ws;
+        plan->ws.alloc(static_cast<size_t>(plan->chunks.size() > 1 ? 2 * max_ws : max_ws));
         plan->task_info.alloc(4 * std::max(1, plan->n_tasks));
         plan->R.alloc(8 * orders);
         plan->T.alloc(8 * orders);

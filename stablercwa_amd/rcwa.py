@@ -74,6 +74,10 @@ class PlanarBatch:
     def execute(self) -> None:
         _lib.check(self.lib.srcwa_plan_execute(self.handle))
 
+    def use_graph(self, enable: bool = True) -> None:
+        """Capture the launch sequence into a HIP graph that execute() replays (one-chunk plans)."""
+        _lib.check(self.lib.srcwa_plan_set_graph(self.handle, 1 if enable else 0))
+
     def sync(self) -> None:
         _lib.check(self.lib.srcwa_plan_sync(self.handle))
 
