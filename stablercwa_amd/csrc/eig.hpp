@@ -33,9 +33,9 @@
 constexpr int EIG_MAX_N = EIG_MAX_N_HOST;   // largest order the LDS rotation table holds
 constexpr int EIG_RING = SRCWA_EIG_RING;    // rows each column prefetches ahead in the QR sweep
 
-// Columns per thread in the QR sweep for matrices of order n (the eigen kernels are instantiated
-// for 1, 2 and 4 so small matrices do not pay the registers of the largest).
-inline int eig_columns_per_thread(int n) { return n <= NT ? 1 : n <= 2 * NT ? 2 : 4; }
+// Columns per lane of one wave for matrices of order n (QC): the eigen kernels are instantiated
+// for 1, 2, 4, 8 and 16 so small matrices do not pay the registers of the largest.
+inline int eig_columns_per_thread(int n) { return n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 512 ? 8 : 16; }
 
 // Launches KERNEL<QC> for the QC that matrices of order up to max_n need.
 #define EIG_LAUNCH(max_n, KERNEL, GRID, STREAM, ...)                                      \
@@ -43,7 +43,9 @@ inline int eig_columns_per_thread(int n) { return n <= NT ? 1 : n <= 2 * NT ? 2 
         switch (eig_columns_per_thread(max_n)) {                                          \
             case 1: KERNEL<1><<<(GRID), NT, 0, (STREAM)>>>(__VA_ARGS__); break;           \
             case 2: KERNEL<2><<<(GRID), NT, 0, (STREAM)>>>(__VA_ARGS__); break;           \
-            default: KERNEL<4><<<(GRID), NT, 0, (STREAM)>>>(__VA_ARGS__); break;          \
+            case 4: KERNEL<4><<<(GRID), NT, 0, (STREAM)>>>(__VA_ARGS__); break;           \
+            case 8: KERNEL<8><<<(GRID), NT, 0, (STREAM)>>>(__VA_ARGS__); break;           \
+            default: KERNEL<16><<<(GRID), NT, 0, (STREAM)>>>(__VA_ARGS__); break;         \
         }                                                                                 \
     } while (0)
 
@@ -59,10 +61,10 @@ struct EigScratch {
 // (rotation i only touches rows i-1 and i, so later rows are final when prefetched).  The chain stays
 // inside one wave: the owner lane's values are broadcast with readlane, with no LDS round trip and no
 // workgroup barrier per rotation.  Requires n <= 64 LC; the caller orders memory afterwards.
-template <int LC, int RD>
+template <int LC, int RD, bool ROWMAJOR = false>
 DEV void wave_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c, double* rot_ci, double* rot_s) {
     const int lane = lane_id();
-    auto h = [&](int i, int j) -> cd& { return H[i + j * n]; };
+    auto h = [&](int i, int j) -> cd& { return ROWMAJOR ? H[i * n + j] : H[i + j * n]; };
     cd carry[LC];
     cd ring[LC][RD];
 #pragma unroll
@@ -106,8 +108,14 @@ DEV void wave_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c
                 xi = ldexp(xi, -e);
                 sub = ldexp(sub, -e);
             }
-            const double nrm = sqrt(xr * xr + xi * xi + sub * sub);
-            const double inv = 1.0 / nrm;
+            // 1/|(x, sub)|: the hardware reciprocal square root refined by two Newton steps
+            // (y <- y + y (1 - s y^2) / 2, each doubling the correct bits), a dependent chain of
+            // about ten FP64 operations instead of a correctly rounded sqrt and a division
+            const double ss = xr * xr + xi * xi + sub * sub;
+            double inv = __builtin_amdgcn_rsq(ss);
+            inv = fma(0.5 * inv, fma(-ss * inv, inv, 1.0), inv);
+            inv = fma(0.5 * inv, fma(-ss * inv, inv, 1.0), inv);
+            const double nrm = ss * inv;
             const double cr = xr * inv, ci = xi * inv, sn = sub * inv;
             if (lane == ol) {
                 h(i - 1, i - 1) = cd{e ? ldexp(nrm, e) : nrm, 0.0};
@@ -137,6 +145,88 @@ DEV void wave_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c
     }
 }
 
+// The left half of one QR sweep with every thread of the workgroup: thread t owns columns
+// l + t + NT q (q < CPT); the owner of column i-1 forms G_i, publishes it in LDS, and after a barrier
+// every thread applies it to its own columns.  Each column's next RD rows are loaded ahead into
+// registers.  H is row-major.  For large orders, where one wave would hold too many columns.
+template <int CPT, int RD>
+DEV void block_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c, double* rot_ci, double* rot_s) {
+    const int tid = threadIdx.x;
+    auto h = [&](int i, int j) -> cd& { return H[i * n + j]; };
+    cd carry[CPT];
+    cd ring[CPT][RD];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int c = l + tid + q * NT;
+        if (c < n) {
+            carry[q] = h(l, c);
+#pragma unroll
+            for (int u = 0; u < RD; ++u)
+                if (l + 1 + u <= en) ring[q][u] = h(l + 1 + u, c);
+        }
+    }
+    for (int ib = l + 1; ib <= en; ib += RD) {
+#pragma unroll
+        for (int u = 0; u < RD; ++u) {
+            const int i = ib + u;
+            if (i > en) break;
+            cd cur[CPT];
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) {
+                const int c = l + tid + q * NT;
+                cur[q] = ring[q][u];
+                if (i + RD <= en && c >= i + RD - 1 && c < n) ring[q][u] = h(i + RD, c);
+            }
+            // owner of column i-1 forms rotation G_i from (h(i-1,i-1), h(i,i-1))
+            const int owner = i - 1 - l;
+            if ((owner % NT) == tid) {
+                const int qo = owner / NT;
+                cd xc{0.0, 0.0}, zc{0.0, 0.0};
+#pragma unroll
+                for (int q = 0; q < CPT; ++q)
+                    if (q == qo) {
+                        xc = carry[q];
+                        zc = cur[q];
+                    }
+                double xr = xc.x, xi = xc.y, sub = zc.x;
+                const double mm = fmax(fmax(fabs(xr), fabs(xi)), fabs(sub));
+                int e = 0;
+                if (!(mm > 1e-140 && mm < 1e140)) {
+                    e = binary_exponent(mm);
+                    xr = ldexp(xr, -e);
+                    xi = ldexp(xi, -e);
+                    sub = ldexp(sub, -e);
+                }
+                const double nrm = sqrt(xr * xr + xi * xi + sub * sub);
+                const double inv = 1.0 / nrm;
+                h(i - 1, i - 1) = cd{e ? ldexp(nrm, e) : nrm, 0.0};
+                h(i, i - 1) = cd{0.0, 0.0};
+                rot_c[i] = xr * inv;
+                rot_ci[i] = xi * inv;
+                rot_s[i] = sub * inv;
+            }
+            __syncthreads();
+            const double cr = rot_c[i], ci = rot_ci[i], sn = rot_s[i];
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) {
+                const int c = l + tid + q * NT;
+                if (c >= i && c < n) {
+                    const cd yv = carry[q];
+                    const cd z = cur[q];
+                    h(i - 1, c) = cd{cr * yv.x + ci * yv.y + sn * z.x, cr * yv.y - ci * yv.x + sn * z.y};
+                    carry[q] = cd{cr * z.x - ci * z.y - sn * yv.x, cr * z.y + ci * z.x - sn * yv.y};
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int c = l + tid + q * NT;
+        if (c >= en && c < n) h(en, c) = carry[q];
+    }
+    __syncthreads();
+}
+
 // Eigen-decompose the n x n matrix A (column-major, leading dimension n) in place.
 //   in : A = the matrix (overwritten)
 //   out: A = right eigenvectors (unit 2-norm columns) when info == 0; w = eigenvalues
@@ -147,7 +237,7 @@ DEV void wave_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c
 template <int QC>
 DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z, cd* __restrict__ X,
                EigScratch s, int max_iter, double eps, bool vectors = true) {
-    if (n > QC * NT) return n + 1;     // the launch chose too few columns per thread
+    if (n > 64 * QC) return n + 1;     // the launch chose too few columns per lane
     const int tid = threadIdx.x;
     const int lane = lane_id(), wave = wave_id();
     cd* H = A;
@@ -292,10 +382,18 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
     __syncthreads();
 
     // ---- 4. shifted QR iterations (COMQR2)
+    // H is iterated in row-major order in X (free until the back-substitution): the left rotations
+    // walk rows with one thread per column, so each rotation's loads and stores are coalesced across
+    // the threads; the right rotations of H walk each row from one thread.  Z stays column-major.
+    for (int e = tid; e < n * n; e += NT) X[e] = H[(e / n) + (e % n) * n];
+    __syncthreads();
     __shared__ double rot_c[EIG_MAX_N + 1], rot_ci[EIG_MAX_N + 1], rot_s[EIG_MAX_N + 1];
     __shared__ int s_info;
     cd t{0.0, 0.0};
     int en = n - 1, its = 0, info = 0;
+    {
+    cd* HR = X;
+    auto h = [&](int i, int j) -> cd& { return HR[i * n + j]; };
     while (en >= 0) {
         int lc = 0;
         for (int l = 1 + tid; l <= en; l += NT)
@@ -341,80 +439,14 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
         ++its;
         __syncthreads();
 
-        // (1) left rotations: one thread per column c in [l, n-1] (QC columns per thread).  The rows a
-        //     column will consume are loaded EIG_RING rotations ahead into a register ring (rotation i
-        //     only touches rows i-1 and i, so later rows are final when prefetched); the global-memory
-        //     latency of the strided row access then overlaps the rotation chain.
-        cd carry[QC];
-        cd ring[QC][EIG_RING];
-#pragma unroll
-        for (int q = 0; q < QC; ++q) {
-            const int c = l + tid + q * NT;
-            if (c < n) {
-                carry[q] = h(l, c);
-#pragma unroll
-                for (int u = 0; u < EIG_RING; ++u)
-                    if (l + 1 + u <= en) ring[q][u] = h(l + 1 + u, c);
-            }
-        }
-        for (int ib = l + 1; ib <= en; ib += EIG_RING) {
-#pragma unroll
-            for (int u = 0; u < EIG_RING; ++u) {
-                const int i = ib + u;
-                if (i > en) break;
-                cd cur[QC];
-#pragma unroll
-                for (int q = 0; q < QC; ++q) {
-                    const int c = l + tid + q * NT;
-                    cur[q] = ring[q][u];
-                    if (i + EIG_RING <= en && c >= i + EIG_RING - 1 && c < n) ring[q][u] = h(i + EIG_RING, c);
-                }
-                // owner of column i-1 forms rotation G_i from (h(i-1,i-1), h(i,i-1))
-                const int owner = i - 1 - l;
-                if ((owner % NT) == tid) {
-                    const int qo = owner / NT;
-                    cd xc{0.0, 0.0}, zc{0.0, 0.0};
-#pragma unroll
-                    for (int q = 0; q < QC; ++q)
-                        if (q == qo) {
-                            xc = carry[q];
-                            zc = cur[q];
-                        }
-                    double xr = xc.x, xi = xc.y, sub = zc.x;
-                    const double mm = fmax(fmax(fabs(xr), fabs(xi)), fabs(sub));
-                    int e = 0;
-                    if (!(mm > 1e-140 && mm < 1e140)) {
-                        e = binary_exponent(mm);
-                        xr = ldexp(xr, -e);
-                        xi = ldexp(xi, -e);
-                        sub = ldexp(sub, -e);
-                    }
-                    const double nrm = sqrt(xr * xr + xi * xi + sub * sub);
-                    const double inv = 1.0 / nrm;
-                    h(i - 1, i - 1) = cd{e ? ldexp(nrm, e) : nrm, 0.0};
-                    h(i, i - 1) = cd{0.0, 0.0};
-                    rot_c[i] = xr * inv;
-                    rot_ci[i] = xi * inv;
-                    rot_s[i] = sub * inv;
-                }
-                __syncthreads();
-                const double cr = rot_c[i], ci = rot_ci[i], sn = rot_s[i];
-#pragma unroll
-                for (int q = 0; q < QC; ++q) {
-                    const int c = l + tid + q * NT;
-                    if (c >= i && c < n) {
-                        const cd yv = carry[q];
-                        const cd z = cur[q];
-                        h(i - 1, c) = cd{cr * yv.x + ci * yv.y + sn * z.x, cr * yv.y - ci * yv.x + sn * z.y};
-                        carry[q] = cd{cr * z.x - ci * z.y - sn * yv.x, cr * z.y + ci * z.x - sn * yv.y};
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < QC; ++q) {
-            const int c = l + tid + q * NT;
-            if (c >= en && c < n) h(en, c) = carry[q];
+        // (1) left rotations.  Orders up to 256: the rotation chain runs inside wave 0 (QC columns per
+        //     lane, rows of the row-major H loaded coalesced EIG_RING rows ahead) while the other waves
+        //     wait at the barrier.  Larger orders: every thread owns columns and each rotation is
+        //     published through LDS with a barrier.
+        if constexpr (QC <= 4) {
+            if (wave == 0) wave_left_sweep<QC, EIG_RING, true>(n, l, en, HR, rot_c, rot_ci, rot_s);
+        } else {
+            block_left_sweep<QC / 4, EIG_RING>(n, l, en, HR, rot_c, rot_ci, rot_s);
         }
         __syncthreads();
 
@@ -437,31 +469,35 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
         for (int r = tid; r < en + 1 + n; r += NT) {
             const bool zrow = r > en;
             const int row = zrow ? r - en - 1 : r;
-            cd* base = zrow ? Z : H;
+            cd* base = zrow ? Z : HR;
+            const int rs = zrow ? 1 : n, cs = zrow ? n : 1;     // Z column-major, H row-major
             const int j0 = zrow ? l + 1 : max(l + 1, row);
-            cd a = base[row + (long)(j0 - 1) * n];
+            cd a = base[row * rs + (j0 - 1) * cs];
             for (int jb = j0; jb <= en; jb += RB) {
                 cd bv[RB];
 #pragma unroll
                 for (int u = 0; u < RB; ++u)
-                    if (jb + u <= en) bv[u] = base[row + (long)(jb + u) * n];
+                    if (jb + u <= en) bv[u] = base[row * rs + (jb + u) * cs];
 #pragma unroll
                 for (int u = 0; u < RB; ++u) {
                     const int j = jb + u;
                     if (j <= en) {
                         const double cr = rot_c[j], ci = rot_ci[j], sn = rot_s[j];
                         const cd b = bv[u];
-                        base[row + (long)(j - 1) * n] =
+                        base[row * rs + (j - 1) * cs] =
                             cd{cr * a.x - ci * a.y + sn * b.x, cr * a.y + ci * a.x + sn * b.y};
                         a = cd{cr * b.x + ci * b.y - sn * a.x, cr * b.y - ci * b.x - sn * a.y};
                     }
                 }
             }
             if (has_phase) a = a * phase;
-            base[row + (long)en * n] = a;
+            base[row * rs + en * cs] = a;
         }
         __syncthreads();
     }
+    }
+    // back to column-major for the eigenvectors (or the Schur form the caller keeps)
+    for (int e = tid; e < n * n; e += NT) H[(e / n) + (e % n) * n] = X[e];
     if (tid == 0) s_info = info;
     __syncthreads();
     info = s_info;
