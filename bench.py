@@ -68,6 +68,20 @@ def cpu_baseline(stack, M, n_points):
                 sample=f"{k} wavelengths x TE+TM at M={M} (numpy oracle, 1 thread)")
 
 
+def measured_eig_traffic():
+    """HBM bytes per launch of the planar eigen kernel on this workload, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE measurement (scaled by the copy-kernel calibration in the same file)."""
+    path = os.path.join(ROOT, "profiles", "eig_planar_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    import re
+    for line in open(path):
+        m = re.search(r"= ([0-9.]+) TB per launch", line)
+        if m:
+            return dict(bytes=float(m.group(1)) * 1e12, source="profiles/eig_planar_hbm_traffic.txt")
+    return None
+
+
 def _timed(batch, n):
     batch.execute()
     batch.sync()
@@ -176,12 +190,22 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the conical, crossed and served legs")
     ap.add_argument("--serve-rate", type=float, default=5.0, help="request arrivals per second (Poisson)")
     ap.add_argument("--serve-requests", type=int, default=60)
-    ap.add_argument("--only", choices=["conical", "crossed", "served"],
-                    help="run one extra leg alone and print its result (for profiling)")
+    ap.add_argument("--only", choices=["conical", "crossed", "served", "copy"],
+                    help="run one extra leg alone and print its result (for profiling); 'copy' copies 1 GiB on "
+                         "the GPU once, the known-size kernel that calibrates the HBM traffic counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not provided"}))
         sys.exit(2)
+    if args.only == "copy":
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")     # 1 GiB
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy": {"bytes_read": src.numel() * 8, "bytes_written": dst.numel() * 8}}), flush=True)
+        return
     if args.only:
         leg = {"conical": conical_leg, "crossed": crossed_leg,
                "served": lambda: served_leg(args.serve_rate, args.serve_requests)}[args.only]
@@ -237,6 +261,12 @@ def main():
     roofline = dict(bound="mfma", kernel=name, achieved=achieved, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=achieved / FP64_PEAK_TFLOPS, traffic=None,
                     flops_per_launch=float(flops), launch_ms=ms)
+    traffic = measured_eig_traffic() if (name == "k_eig1d" and args.points == 4096 and M == 60) else None
+    if traffic:
+        roofline["traffic"] = traffic["bytes"]
+        roofline["traffic_source"] = traffic["source"]
+        roofline["traffic_rate_gbs"] = traffic["bytes"] / (ms * 1e-3) / 1e9
+        roofline["hbm_peak_gbs"] = HBM_PEAK_GBS
 
     # plain form: one solve per launch, inputs resident
     plain = None

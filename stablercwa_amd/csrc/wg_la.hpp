@@ -139,16 +139,29 @@ DEV void wg_gemm(int m, int n, int k, const cd* __restrict__ A, int lda, const c
             }
         const int ra[2] = {r0 + li, r0 + 16 + li};
         const int cb[2] = {c0 + li, c0 + 16 + li};
-        for (int k0 = 0; k0 < k; k0 += 4) {
+        // operands of k-step k0 (lane: row / column li, k index k0 + lk)
+        auto fetch = [&](int k0, cd* a, cd* b) {
             const int kk = k0 + lk;
             const bool kin = kk < k;
-            cd a[2], b[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i) {
                 a[i] = (kin && ra[i] < m) ? A[ra[i] + (long)kk * lda] : cd{0.0, 0.0};
                 if (BT) b[i] = (kin && cb[i] < n) ? B[cb[i] + (long)kk * ldb] : cd{0.0, 0.0};
                 else b[i] = (kin && cb[i] < n) ? B[kk + (long)cb[i] * ldb] : cd{0.0, 0.0};
             }
+        };
+        // the operands of the next two k-steps are in flight while the MFMAs of this one run
+        cd a0[2], b0[2], a1[2], b1[2];
+        fetch(0, a0, b0);
+        fetch(4, a1, b1);
+        for (int k0 = 0; k0 < k; k0 += 4) {
+            cd a[2] = {a0[0], a0[1]}, b[2] = {b0[0], b0[1]};
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                a0[i] = a1[i];
+                b0[i] = b1[i];
+            }
+            if (k0 + 8 < k) fetch(k0 + 8, a1, b1);
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
