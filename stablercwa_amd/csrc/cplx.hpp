@@ -13,6 +13,15 @@ struct __attribute__((aligned(16))) cd {
 
 #define DEV __device__ __forceinline__
 
+// Member-wise load / store of one complex value.  An aggregate copy (`a = *p`) is a 16-byte memcpy to
+// the compiler, which can keep register arrays of cd from being promoted out of private memory; these
+// are two double accesses that the backend merges into one 16-byte access.
+DEV cd ldc(const cd* p) { return cd{p->x, p->y}; }
+DEV void stc(cd* p, cd a) {
+    p->x = a.x;
+    p->y = a.y;
+}
+
 DEV cd mk(double x, double y = 0.0) { return cd{x, y}; }
 DEV cd operator+(cd a, cd b) { return cd{a.x + b.x, a.y + b.y}; }
 DEV cd operator-(cd a, cd b) { return cd{a.x - b.x, a.y - b.y}; }

@@ -43,6 +43,10 @@ DEV void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Compiler-only barrier: keeps the LDS loads of a staged triangular block next to their use (per row
+// and per loop trip) instead of hoisting all of them into registers ahead of the loop.
+DEV void keep_loads_in_loop() { asm volatile("" ::: "memory"); }
+
 DEV double read_lane(double v, int lane) {
     const long long b = __double_as_longlong(v);
     const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffLL), lane);
@@ -120,8 +124,10 @@ DEV int block_argmax(double v, int idx) {
 enum { GEMM_SET = 0, GEMM_SUB = 1, GEMM_ADD = 2 };
 
 // WAVE = true: the calling wave computes every tile alone, with no workgroup barrier (for code in
-// which each wave works on its own matrices).
-template <int MODE, bool BT, bool WAVE = false>
+// which each wave works on its own matrices).  CPRE = true (with MODE SUB or ADD): a tile's C values
+// are loaded together with its first operands, so their latency is hidden behind the MFMAs (for short
+// k, as in the LU and substitution updates).
+template <int MODE, bool BT, bool WAVE = false, bool CPRE = false>
 DEV void wg_gemm(int m, int n, int k, const cd* __restrict__ A, int lda, const cd* __restrict__ B, int ldb,
                  cd* C, int ldc) {
     const int lane = lane_id();
@@ -154,6 +160,20 @@ DEV void wg_gemm(int m, int n, int k, const cd* __restrict__ A, int lda, const c
         cd a0[2], b0[2], a1[2], b1[2];
         fetch(0, a0, b0);
         fetch(4, a1, b1);
+        cd cpre[2][2][4];
+        if (CPRE && MODE != GEMM_SET) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const int col = c0 + 16 * j + li;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int row = r0 + 16 * i + lk + 4 * q;
+                        cpre[i][j][q] = (col < n && row < m) ? C[row + (long)col * ldc] : cd{0.0, 0.0};
+                    }
+                }
+        }
         for (int k0 = 0; k0 < k; k0 += 4) {
             cd a[2] = {a0[0], a0[1]}, b[2] = {b0[0], b0[1]};
 #pragma unroll
@@ -184,13 +204,32 @@ DEV void wg_gemm(int m, int n, int k, const cd* __restrict__ A, int lda, const c
                     if (row >= m) continue;
                     cd* p = C + row + (long)col * ldc;
                     const cd v{re[i][j][q], im[i][j][q]};
+                    const cd c = (CPRE && MODE != GEMM_SET) ? cpre[i][j][q] : (MODE == GEMM_SET ? v : *p);
                     if (MODE == GEMM_SET) *p = v;
-                    else if (MODE == GEMM_SUB) *p = *p - v;
-                    else *p = *p + v;
+                    else if (MODE == GEMM_SUB) *p = c - v;
+                    else *p = c + v;
                 }
             }
     }
     if (!WAVE) __syncthreads();
+}
+
+// C (m x nrhs) -= A (m x k) B (k x nrhs) for the short k of the blocked substitutions: MFMA tiles with
+// C prefetched, or one VALU dot product per entry when there are too few right-hand sides to fill a
+// 32-column tile.
+DEV void wg_update(int m, int nrhs, int k, const cd* A, int lda, const cd* B, int ldb, cd* C, int ldc) {
+    if (nrhs >= 8) {
+        wg_gemm<GEMM_SUB, false, false, true>(m, nrhs, k, A, lda, B, ldb, C, ldc);
+        return;
+    }
+    for (int e = threadIdx.x; e < m * nrhs; e += NT) {
+        const int i = e % m, c = e / m;
+        const cd* b = B + (long)c * ldb;
+        cd s{0.0, 0.0};
+        for (int kk = 0; kk < k; ++kk) s = s + A[i + (long)kk * lda] * b[kk];
+        C[i + (long)c * ldc] = C[i + (long)c * ldc] - s;
+    }
+    __syncthreads();
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -241,56 +280,376 @@ DEV void wg_scale_cols(int m, int n, const cd* src, int lds, const cd* d, cd* ds
 // ------------------------------------------------------------------------------------------------
 // LU with partial pivoting.  piv[j] = row exchanged with row j at step j.  Returns 1 when a pivot
 // was exactly zero (the factorization is then only a failure marker).
-DEV int wg_lu(int n, cd* A, int lda, int* piv) {
+
+// Largest value, ties to the smaller index, over one wave; every lane receives the winner.
+DEV void wave_argmax(double& v, int& idx) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(v, o, 64);
+        const int oi = __shfl_xor(idx, o, 64);
+        if (ov > v || (ov == v && oi < idx)) {
+            v = ov;
+            idx = oi;
+        }
+    }
+}
+
+// Workgroup scratch of one panel step of the LU.
+struct LuPanelShared {
+    cd prow[LU_NB];              // the pivot row of the current column
+    cd jrow[LU_NB];              // the row it replaces
+    cd L11[LU_NB * LU_NB];       // unit-lower diagonal block of the finished panel (strict lower part)
+    double rv[NWAVE];            // per-wave pivot candidates
+    int ri[NWAVE];
+    int porig, jorig;            // original panel rows of the two exchanged rows
+    int piv[LU_NB];              // pivot row of each column (panel-relative)
+    int src[LU_NB];              // panel row k (< nb) now holds what was panel row src[k]
+    int pdst[LU_NB], psrc[LU_NB];// rows >= nb that changed: pdst[j] now holds what was psrc[j]
+    int npairs;
+    int sing;
+};
+
+// Panel factorization with the panel in registers: thread t holds panel rows t, t + NT, ..., t + (R-1) NT
+// (all LU_NB columns).  Per column: pivot search (wave reduction + one barrier), exchange of the pivot
+// row through LDS (one barrier), multipliers and the rank-1 update in registers.  The pivot rule, the
+// multipliers (by 1/pivot when finite, else by division) and the update order are those of the
+// column-at-a-time LU.  Leaves in S the pivots, L11 and the row permutation of the panel.
+template <int R>
+DEV void lu_panel_regs(int m, int nb, cd* Ap, int lda, LuPanelShared& S) {
     const int tid = threadIdx.x;
-    int singular = 0;
-    for (int ks = 0; ks < n; ks += LU_NB) {
-        const int ke = min(ks + LU_NB, n);
-        for (int j = ks; j < ke; ++j) {
-            cd* cj = A + (long)j * lda;
-            double best = -1.0;
-            int bi = n;
-            for (int i = j + tid; i < n; i += NT) {
-                const double v = cabs(cj[i]);
-                if (v > best) { best = v; bi = i; }
+    cd v[R][LU_NB];
+    int orig[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int row = tid + r * NT;
+        orig[r] = row;
+#pragma unroll
+        for (int c = 0; c < LU_NB; ++c) v[r][c] = (row < m && c < nb) ? ldc(Ap + row + (long)c * lda) : cd{0.0, 0.0};
+    }
+    int sing = 0;
+    // the column loop stays rolled; column jj of a register row is picked by unrolled selects, so the
+    // rows never need dynamic register indexing
+    for (int jj = 0; jj < nb; ++jj) {
+        double best = -1.0;
+        int bi = m;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = tid + r * NT;
+            cd vj = v[r][0];
+#pragma unroll
+            for (int c = 1; c < LU_NB; ++c)
+                if (c == jj) vj = v[r][c];
+            if (row >= jj && row < m) {
+                const double a = cabs(vj);
+                if (a > best) {
+                    best = a;
+                    bi = row;
+                }
             }
-            int p = block_argmax(best, bi);
-            if (p >= n) p = j;
-            if (tid == 0) piv[j] = p;
-            if (p != j)
-                for (int c = ks + tid; c < ke; c += NT) {
-                    cd* col = A + (long)c * lda;
-                    const cd t = col[j];
-                    col[j] = col[p];
+        }
+        wave_argmax(best, bi);
+        if (lane_id() == 0) {
+            S.rv[wave_id()] = best;
+            S.ri[wave_id()] = bi;
+        }
+        __syncthreads();
+        double bv = S.rv[0];
+        int p = S.ri[0];
+#pragma unroll
+        for (int w = 1; w < NWAVE; ++w)
+            if (S.rv[w] > bv || (S.rv[w] == bv && S.ri[w] < p)) {
+                bv = S.rv[w];
+                p = S.ri[w];
+            }
+        if (p >= m) p = jj;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = tid + r * NT;
+            if (row == p) {
+#pragma unroll
+                for (int c = 0; c < LU_NB; ++c) S.prow[c] = v[r][c];
+                S.porig = orig[r];
+            }
+            if (row == jj && p != jj) {
+#pragma unroll
+                for (int c = 0; c < LU_NB; ++c) S.jrow[c] = v[r][c];
+                S.jorig = orig[r];
+            }
+        }
+        if (tid == 0) S.piv[jj] = p;
+        __syncthreads();
+        const cd pivot = S.prow[jj];
+        const bool zero = is_zero(pivot);
+        sing |= zero;
+        const cd inv = zero ? cd{0.0, 0.0} : cdiv(cd{1.0, 0.0}, pivot);
+        const bool fin = finite(inv);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int row = tid + r * NT;
+            if (p != jj) {
+                if (row == jj) {
+#pragma unroll
+                    for (int c = 0; c < LU_NB; ++c) v[r][c] = S.prow[c];
+                    orig[r] = S.porig;
+                } else if (row == p) {
+#pragma unroll
+                    for (int c = 0; c < LU_NB; ++c) v[r][c] = S.jrow[c];
+                    orig[r] = S.jorig;
+                }
+            }
+            if (row > jj && row < m) {
+                cd vj = v[r][0];
+#pragma unroll
+                for (int c = 1; c < LU_NB; ++c)
+                    if (c == jj) vj = v[r][c];
+                const cd l = zero ? cd{0.0, 0.0} : (fin ? vj * inv : cdiv(vj, pivot));
+#pragma unroll
+                for (int c = 0; c < LU_NB; ++c) {
+                    if (c == jj) v[r][c] = l;
+                    else if (c > jj && c < nb) v[r][c] = v[r][c] - l * S.prow[c];
+                }
+            }
+        }
+    }
+    // write back; L11 and the permutation to LDS
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int row = tid + r * NT;
+        if (row < m) {
+#pragma unroll
+            for (int c = 0; c < LU_NB; ++c)
+                if (c < nb) stc(Ap + row + (long)c * lda, v[r][c]);
+        }
+        if (row < nb) {
+#pragma unroll
+            for (int c = 0; c < LU_NB; ++c) S.L11[row + c * LU_NB] = c < row ? v[r][c] : cd{0.0, 0.0};
+            S.src[row] = orig[r];
+        } else if (row < m && orig[r] != row) {
+            const int k = atomicAdd(&S.npairs, 1);
+            S.pdst[k] = row;
+            S.psrc[k] = orig[r];
+        }
+    }
+    if (tid == 0 && sing) S.sing = 1;
+}
+
+// Same outputs as lu_panel_regs for panels too tall for registers: one wave factors the panel in
+// place in global memory, then thread 0 derives the panel's row permutation from the pivots.
+DEV void lu_panel_global(int m, int nb, cd* Ap, int lda, LuPanelShared& S, int* trow, int* tsrc) {
+    const int tid = threadIdx.x, lane = lane_id();
+    if (wave_id() == 0) {
+        int sing = 0;
+        for (int jj = 0; jj < nb; ++jj) {
+            cd* cj = Ap + (long)jj * lda;
+            double best = -1.0;
+            int bi = m;
+            for (int i = jj + lane; i < m; i += 64) {
+                const double a = cabs(cj[i]);
+                if (a > best) {
+                    best = a;
+                    bi = i;
+                }
+            }
+            wave_argmax(best, bi);
+            const int p = bi >= m ? jj : bi;
+            if (lane == 0) S.piv[jj] = p;
+            if (p != jj)
+                for (int c = lane; c < nb; c += 64) {
+                    cd* col = Ap + (long)c * lda;
+                    const cd t = col[jj];
+                    col[jj] = col[p];
                     col[p] = t;
                 }
-            __syncthreads();
-            const cd pivot = cj[j];
-            if (is_zero(pivot)) {
-                singular = 1;
-                for (int i = j + 1 + tid; i < n; i += NT) cj[i] = cd{0.0, 0.0};
-            } else {
-                const cd inv = cdiv(cd{1.0, 0.0}, pivot);
-                if (finite(inv)) {
-                    for (int i = j + 1 + tid; i < n; i += NT) cj[i] = cj[i] * inv;
-                } else {
-                    for (int i = j + 1 + tid; i < n; i += NT) cj[i] = cdiv(cj[i], pivot);
+            wave_sync();
+            const cd pivot = cj[jj];
+            const bool zero = is_zero(pivot);
+            sing |= zero;
+            const cd inv = zero ? cd{0.0, 0.0} : cdiv(cd{1.0, 0.0}, pivot);
+            const bool fin = finite(inv);
+            for (int i = jj + 1 + lane; i < m; i += 64) {
+                const cd l = zero ? cd{0.0, 0.0} : (fin ? cj[i] * inv : cdiv(cj[i], pivot));
+                cj[i] = l;
+                for (int c = jj + 1; c < nb; ++c) Ap[i + (long)c * lda] = Ap[i + (long)c * lda] - l * Ap[jj + (long)c * lda];
+            }
+            wave_sync();
+        }
+        if (lane == 0 && sing) S.sing = 1;
+    }
+    __syncthreads();
+    if (tid < LU_NB * LU_NB) {
+        const int i = tid % LU_NB, c = tid / LU_NB;
+        S.L11[tid] = (i < nb && c < i) ? Ap[i + (long)c * lda] : cd{0.0, 0.0};
+    }
+    if (tid == 0) {
+        // rows touched by the panel's exchanges: trow[t] now holds what was tsrc[t]
+        int cnt = nb;
+        for (int k = 0; k < nb; ++k) {
+            trow[k] = k;
+            tsrc[k] = k;
+        }
+        for (int k = 0; k < nb; ++k) {
+            const int p = S.piv[k];
+            int b = -1;
+            for (int t = 0; t < cnt; ++t)
+                if (trow[t] == p) b = t;
+            if (b < 0) {
+                b = cnt++;
+                trow[b] = p;
+                tsrc[b] = p;
+            }
+            const int x = tsrc[k];
+            tsrc[k] = tsrc[b];
+            tsrc[b] = x;
+        }
+        for (int k = 0; k < nb; ++k) S.src[k] = tsrc[k];
+        int np = 0;
+        for (int t = nb; t < cnt; ++t)
+            if (tsrc[t] != trow[t]) {
+                S.pdst[np] = trow[t];
+                S.psrc[np] = tsrc[t];
+                ++np;
+            }
+        S.npairs = np;
+    }
+}
+
+// The other columns of a full panel (nb = LU_NB), one column per thread: the panel's row permutation
+// as a gather (loads of every source row first, then the stores; the list of moved rows below the
+// block is padded with row 0, whose slot is rewritten last) and, right of the panel, U12 = L11^{-1} A12
+// in registers.  Straight-line code: every index is static.
+DEV void lu_columns_full(int n, int ks, cd* A, int lda, const LuPanelShared& S) {
+    const int ke = ks + LU_NB;
+    for (int c = threadIdx.x; c < n - LU_NB; c += NT) {
+        keep_loads_in_loop();
+        const int ci = c < ks ? c : c + LU_NB;
+        cd* col = A + ks + (long)ci * lda;
+        cd x[LU_NB], y[LU_NB];
+#pragma unroll
+        for (int k = 0; k < LU_NB; ++k) x[k] = ldc(col + S.src[k]);
+#pragma unroll
+        for (int j = 0; j < LU_NB; ++j) y[j] = ldc(col + S.psrc[j]);
+#pragma unroll
+        for (int j = 0; j < LU_NB; ++j) stc(col + S.pdst[j], y[j]);
+        if (ci >= ke) {
+#pragma unroll
+            for (int i = 1; i < LU_NB; ++i) {
+                keep_loads_in_loop();
+                cd s = x[i];
+#pragma unroll
+                for (int k = 0; k < i; ++k) s = s - S.L11[i + k * LU_NB] * x[k];
+                x[i] = s;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < LU_NB; ++k) stc(col + k, x[k]);
+    }
+}
+
+// The same for a last panel narrower than LU_NB: the exchanges one at a time, then the solve.
+DEV void lu_columns_partial(int n, int ks, int nb, cd* A, int lda, const LuPanelShared& S) {
+    const int ke = ks + nb;
+    for (int c = threadIdx.x; c < n - nb; c += NT) {
+        const int ci = c < ks ? c : c + nb;
+        cd* col = A + ks + (long)ci * lda;
+        for (int k = 0; k < nb; ++k) {
+            const int p = S.piv[k];
+            if (p != k) {
+                const cd t = col[k];
+                col[k] = col[p];
+                col[p] = t;
+            }
+        }
+        if (ci >= ke)
+            for (int i = 1; i < nb; ++i) {
+                cd s = col[i];
+                for (int k = 0; k < i; ++k) s = s - S.L11[i + k * LU_NB] * col[k];
+                col[i] = s;
+            }
+    }
+}
+
+// P A = L U in place, right-looking with panels of LU_NB columns and partial pivoting (largest
+// modulus, first on ties).  Per panel: the panel factorization (in registers for up to 2 NT rows),
+// then one pass over the other columns that applies the panel's row permutation and, right of the
+// panel, solves U12 = L11^{-1} A12; then the trailing MFMA update.
+DEV int wg_lu(int n, cd* A, int lda, int* piv) {
+    __shared__ LuPanelShared S;
+    __shared__ int s_trow[2 * LU_NB], s_tsrc[2 * LU_NB];
+    const int tid = threadIdx.x;
+    if (tid == 0) S.sing = 0;
+    for (int ks = 0; ks < n; ks += LU_NB) {
+        const int m = n - ks;
+        const int nb = min(LU_NB, m);
+        const int ke = ks + nb;
+        cd* Ap = A + ks + (long)ks * lda;
+        if (tid == 0) S.npairs = 0;
+        if (tid < LU_NB) {
+            S.pdst[tid] = 0;
+            S.psrc[tid] = 0;
+        }
+        __syncthreads();
+        if (m <= NT) lu_panel_regs<1>(m, nb, Ap, lda, S);
+        else if (m <= 2 * NT) lu_panel_regs<2>(m, nb, Ap, lda, S);
+        else lu_panel_global(m, nb, Ap, lda, S, s_trow, s_tsrc);
+        __syncthreads();
+        if (tid < nb) piv[ks + tid] = ks + S.piv[tid];
+        if (nb == LU_NB) lu_columns_full(n, ks, A, lda, S);
+        else lu_columns_partial(n, ks, nb, A, lda, S);
+        __syncthreads();
+        if (ke < n)
+            wg_gemm<GEMM_SUB, false, false, true>(n - ke, n - ke, nb, A + ke + (long)ks * lda, lda, A + ks + (long)ke * lda,
+                                                  lda, A + ke + (long)ke * lda, lda);
+    }
+    __syncthreads();
+    const int sing = S.sing;
+    __syncthreads();   // every thread has read the flag before a later call clears it
+    return sing;
+}
+
+// Row permutation of wg_lu's pivots applied to B (n x nrhs): staged through LDS in column blocks.
+constexpr int SOLVE_STAGE_WORDS = 1024;
+constexpr int SOLVE_MAX_PERM = 2048;
+
+// Solve A X = B in place from wg_lu's factors; B is n x nrhs with leading dimension ldb.
+// Blocked by LU_NB: each diagonal block is staged in LDS and solved in registers (one right-hand side
+// per thread), the off-diagonal updates are MFMA products (VALU products for fewer than 8 right-hand
+// sides).  The substitution order within a block is that of plain forward / backward substitution.
+DEV void wg_lu_solve(int n, const cd* LU, int lda, const int* piv, cd* B, int ldb, int nrhs) {
+    __shared__ int s_perm[SOLVE_MAX_PERM];
+    __shared__ cd s_stage[SOLVE_STAGE_WORDS];
+    __shared__ cd s_T[LU_NB * LU_NB];
+    const int tid = threadIdx.x;
+    // row exchanges
+    if (n <= SOLVE_MAX_PERM && n <= SOLVE_STAGE_WORDS) {
+        for (int i = tid; i < n; i += NT) s_perm[i] = piv[i];
+        __syncthreads();
+        if (tid == 0) {
+            // the exchanges applied in order to the identity: cur[i] = original row now at row i
+            int* cur = reinterpret_cast<int*>(s_stage);
+            for (int i = 0; i < n; ++i) cur[i] = i;
+            for (int k = 0; k < n; ++k) {
+                const int p = s_perm[k];
+                if (p != k) {
+                    const int t = cur[k];
+                    cur[k] = cur[p];
+                    cur[p] = t;
                 }
             }
+            for (int i = 0; i < n; ++i) s_perm[i] = cur[i];
+        }
+        __syncthreads();
+        const int cc = SOLVE_STAGE_WORDS / n;
+        for (int c0 = 0; c0 < nrhs; c0 += cc) {
+            const int w = min(cc, nrhs - c0) * n;
+            for (int e = tid; e < w; e += NT) s_stage[e] = B[s_perm[e % n] + (long)(c0 + e / n) * ldb];
             __syncthreads();
-            const int rows = n - j - 1, cols = ke - j - 1;
-            for (int e = tid; e < rows * cols; e += NT) {
-                const int i = j + 1 + e % rows, c = j + 1 + e / rows;
-                cd* col = A + (long)c * lda;
-                col[i] = col[i] - cj[i] * col[j];
-            }
+            for (int e = tid; e < w; e += NT) B[e % n + (long)(c0 + e / n) * ldb] = s_stage[e];
             __syncthreads();
         }
-        // the panel's row exchanges on the columns outside it
-        for (int c = tid; c < n - (ke - ks); c += NT) {
-            const int col_index = c < ks ? c : c + (ke - ks);
-            cd* col = A + (long)col_index * lda;
-            for (int k = ks; k < ke; ++k) {
+    } else {
+        for (int c = tid; c < nrhs; c += NT) {
+            cd* col = B + (long)c * ldb;
+            for (int k = 0; k < n; ++k) {
                 const int p = piv[k];
                 if (p != k) {
                     const cd t = col[k];
@@ -300,64 +659,84 @@ DEV int wg_lu(int n, cd* A, int lda, int* piv) {
             }
         }
         __syncthreads();
-        if (ke < n) {
-            // U12 = L11^{-1} A12 (unit lower, one column per thread)
-            for (int c = ke + tid; c < n; c += NT) {
-                cd* col = A + (long)c * lda;
-                for (int i = ks + 1; i < ke; ++i) {
+    }
+    // forward substitution with the unit-lower factor
+    for (int kb = 0; kb < n; kb += LU_NB) {
+        const int ke = min(kb + LU_NB, n), nb = ke - kb;
+        {
+            const int i = tid % LU_NB, k = tid / LU_NB;
+            s_T[tid] = (i < nb && k < i) ? LU[kb + i + (long)(kb + k) * lda] : cd{0.0, 0.0};
+        }
+        __syncthreads();
+        if (nb == LU_NB) {
+            for (int c = tid; c < nrhs; c += NT) {
+                keep_loads_in_loop();
+                cd* col = B + kb + (long)c * ldb;
+                cd x[LU_NB];
+#pragma unroll
+                for (int k = 0; k < LU_NB; ++k) x[k] = ldc(col + k);
+#pragma unroll
+                for (int i = 1; i < LU_NB; ++i) {
+                    keep_loads_in_loop();
+                    cd s = x[i];
+#pragma unroll
+                    for (int k = 0; k < i; ++k) s = s - s_T[i + k * LU_NB] * x[k];
+                    x[i] = s;
+                }
+#pragma unroll
+                for (int k = 1; k < LU_NB; ++k) stc(col + k, x[k]);
+            }
+        } else {
+            for (int c = tid; c < nrhs; c += NT) {
+                cd* col = B + kb + (long)c * ldb;
+                for (int i = 1; i < nb; ++i) {
                     cd s = col[i];
-                    for (int k = ks; k < i; ++k) s = s - A[i + (long)k * lda] * col[k];
+                    for (int k = 0; k < i; ++k) s = s - s_T[i + k * LU_NB] * col[k];
                     col[i] = s;
                 }
             }
-            __syncthreads();
-            wg_gemm<GEMM_SUB, false>(n - ke, n - ke, ke - ks, A + ke + (long)ks * lda, lda, A + ks + (long)ke * lda, lda,
-                                     A + ke + (long)ke * lda, lda);
-        }
-    }
-    return singular;
-}
-
-// Solve A X = B in place from wg_lu's factors; B is n x nrhs with leading dimension ldb.
-DEV void wg_lu_solve(int n, const cd* LU, int lda, const int* piv, cd* B, int ldb, int nrhs) {
-    const int tid = threadIdx.x;
-    for (int c = tid; c < nrhs; c += NT) {
-        cd* col = B + (long)c * ldb;
-        for (int k = 0; k < n; ++k) {
-            const int p = piv[k];
-            if (p != k) {
-                const cd t = col[k];
-                col[k] = col[p];
-                col[p] = t;
-            }
-        }
-    }
-    __syncthreads();
-    for (int kb = 0; kb < n; kb += LU_NB) {
-        const int ke = min(kb + LU_NB, n);
-        for (int c = tid; c < nrhs; c += NT) {
-            cd* col = B + (long)c * ldb;
-            for (int i = kb + 1; i < ke; ++i) {
-                cd s = col[i];
-                for (int k = kb; k < i; ++k) s = s - LU[i + (long)k * lda] * col[k];
-                col[i] = s;
-            }
         }
         __syncthreads();
-        if (ke < n) wg_gemm<GEMM_SUB, false>(n - ke, nrhs, ke - kb, LU + ke + (long)kb * lda, lda, B + kb, ldb, B + ke, ldb);
+        if (ke < n) wg_update(n - ke, nrhs, nb, LU + ke + (long)kb * lda, lda, B + kb, ldb, B + ke, ldb);
     }
+    // backward substitution with the upper factor
     const int last = ((n - 1) / LU_NB) * LU_NB;
     for (int kb = last; kb >= 0; kb -= LU_NB) {
-        const int ke = min(kb + LU_NB, n);
-        for (int c = tid; c < nrhs; c += NT) {
-            cd* col = B + (long)c * ldb;
-            for (int i = ke - 1; i >= kb; --i) {
-                cd s = col[i];
-                for (int k = i + 1; k < ke; ++k) s = s - LU[i + (long)k * lda] * col[k];
-                col[i] = cdiv(s, LU[i + (long)i * lda]);
+        const int ke = min(kb + LU_NB, n), nb = ke - kb;
+        {
+            const int i = tid % LU_NB, k = tid / LU_NB;
+            s_T[tid] = (i < nb && k < nb && k >= i) ? LU[kb + i + (long)(kb + k) * lda] : cd{0.0, 0.0};
+        }
+        __syncthreads();
+        if (nb == LU_NB) {
+            for (int c = tid; c < nrhs; c += NT) {
+                keep_loads_in_loop();
+                cd* col = B + kb + (long)c * ldb;
+                cd x[LU_NB];
+#pragma unroll
+                for (int k = 0; k < LU_NB; ++k) x[k] = ldc(col + k);
+#pragma unroll
+                for (int i = LU_NB - 1; i >= 0; --i) {
+                    keep_loads_in_loop();
+                    cd s = x[i];
+#pragma unroll
+                    for (int k = i + 1; k < LU_NB; ++k) s = s - s_T[i + k * LU_NB] * x[k];
+                    x[i] = cdiv(s, s_T[i + i * LU_NB]);
+                }
+#pragma unroll
+                for (int k = 0; k < LU_NB; ++k) stc(col + k, x[k]);
+            }
+        } else {
+            for (int c = tid; c < nrhs; c += NT) {
+                cd* col = B + kb + (long)c * ldb;
+                for (int i = nb - 1; i >= 0; --i) {
+                    cd s = col[i];
+                    for (int k = i + 1; k < nb; ++k) s = s - s_T[i + k * LU_NB] * col[k];
+                    col[i] = cdiv(s, s_T[i + i * LU_NB]);
+                }
             }
         }
         __syncthreads();
-        if (kb > 0) wg_gemm<GEMM_SUB, false>(kb, nrhs, ke - kb, LU + (long)kb * lda, lda, B + kb, ldb, B, ldb);
+        if (kb > 0) wg_update(kb, nrhs, nb, LU + (long)kb * lda, lda, B + kb, ldb, B, ldb);
     }
 }
