@@ -556,7 +556,7 @@ __global__ __launch_bounds__(NT) void k_modes_conical(const Inst1D* __restrict__
 }
 
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(NT) void k_smatrix_conical(const Inst1D* __restrict__ insts,
+__global__ SMAT_KERNEL_BOUNDS void k_smatrix_conical(const Inst1D* __restrict__ insts,
                                                         const double* __restrict__ stacks,
                                                         const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                                         const int* __restrict__ task_info, Out1D out) {

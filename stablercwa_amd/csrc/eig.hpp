@@ -62,7 +62,10 @@ struct EigScratch {
 // inside one wave: the owner lane's values are broadcast with readlane, with no LDS round trip and no
 // workgroup barrier per rotation.  Requires n <= 64 LC; the caller orders memory afterwards.
 template <int LC, int RD, bool ROWMAJOR = false>
-DEV void wave_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c, double* rot_ci, double* rot_s) {
+DEV void wave_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c, double* rot_ci, double* rot_s,
+                         int cend = -1) {
+    // columns l .. cend-1 are rotated (all of them by default; the workgroup solver defers the rest)
+    if (cend < 0) cend = n;
     const int lane = lane_id();
     auto h = [&](int i, int j) -> cd& { return ROWMAJOR ? H[i * n + j] : H[i + j * n]; };
     cd carry[LC];
@@ -70,7 +73,7 @@ DEV void wave_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c
 #pragma unroll
     for (int q = 0; q < LC; ++q) {
         const int c = l + lane + 64 * q;
-        if (c < n) {
+        if (c < cend) {
             carry[q] = h(l, c);
 #pragma unroll
             for (int u = 0; u < RD; ++u)
@@ -87,7 +90,7 @@ DEV void wave_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c
             for (int q = 0; q < LC; ++q) {
                 const int c = l + lane + 64 * q;
                 cur[q] = ring[q][u];
-                if (i + RD <= en && c >= i + RD - 1 && c < n) ring[q][u] = h(i + RD, c);
+                if (i + RD <= en && c >= i + RD - 1 && c < cend) ring[q][u] = h(i + RD, c);
             }
             // G_i from the owner column i-1: (h(i-1,i-1), h(i,i-1)) broadcast from its lane
             const int owner = i - 1 - l;
@@ -129,7 +132,7 @@ DEV void wave_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c
 #pragma unroll
             for (int q = 0; q < LC; ++q) {
                 const int c = l + lane + 64 * q;
-                if (c >= i && c < n) {
+                if (c >= i && c < cend) {
                     const cd yv = carry[q];
                     const cd z = cur[q];
                     h(i - 1, c) = cd{cr * yv.x + ci * yv.y + sn * z.x, cr * yv.y - ci * yv.x + sn * z.y};
@@ -141,7 +144,7 @@ DEV void wave_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c
 #pragma unroll
     for (int q = 0; q < LC; ++q) {
         const int c = l + lane + 64 * q;
-        if (c >= en && c < n) h(en, c) = carry[q];
+        if (c >= en && c < cend) h(en, c) = carry[q];
     }
 }
 
@@ -150,7 +153,8 @@ DEV void wave_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c
 // every thread applies it to its own columns.  Each column's next RD rows are loaded ahead into
 // registers.  H is row-major.  For large orders, where one wave would hold too many columns.
 template <int CPT, int RD>
-DEV void block_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c, double* rot_ci, double* rot_s) {
+DEV void block_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c, double* rot_ci, double* rot_s,
+                          int cend) {
     const int tid = threadIdx.x;
     auto h = [&](int i, int j) -> cd& { return H[i * n + j]; };
     cd carry[CPT];
@@ -158,7 +162,7 @@ DEV void block_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
         const int c = l + tid + q * NT;
-        if (c < n) {
+        if (c < cend) {
             carry[q] = h(l, c);
 #pragma unroll
             for (int u = 0; u < RD; ++u)
@@ -175,7 +179,7 @@ DEV void block_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_
             for (int q = 0; q < CPT; ++q) {
                 const int c = l + tid + q * NT;
                 cur[q] = ring[q][u];
-                if (i + RD <= en && c >= i + RD - 1 && c < n) ring[q][u] = h(i + RD, c);
+                if (i + RD <= en && c >= i + RD - 1 && c < cend) ring[q][u] = h(i + RD, c);
             }
             // owner of column i-1 forms rotation G_i from (h(i-1,i-1), h(i,i-1))
             const int owner = i - 1 - l;
@@ -210,7 +214,7 @@ DEV void block_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_
 #pragma unroll
             for (int q = 0; q < CPT; ++q) {
                 const int c = l + tid + q * NT;
-                if (c >= i && c < n) {
+                if (c >= i && c < cend) {
                     const cd yv = carry[q];
                     const cd z = cur[q];
                     h(i - 1, c) = cd{cr * yv.x + ci * yv.y + sn * z.x, cr * yv.y - ci * yv.x + sn * z.y};
@@ -222,7 +226,133 @@ DEV void block_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_
 #pragma unroll
     for (int q = 0; q < CPT; ++q) {
         const int c = l + tid + q * NT;
-        if (c >= en && c < n) h(en, c) = carry[q];
+        if (c >= en && c < cend) h(en, c) = carry[q];
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Deferred updates of the QR sweeps.
+//
+// A sweep on the active window [l, en] only has to update H(l..en, l..en) before the next sweep can
+// start (its shift, deflation test and rotations read nothing else).  The rest of its work -- the
+// right rotations on Z and on the rows of H above l, the left rotations on the columns of H right of
+// en, and the phase of column / row en there -- is queued (rotations and windows in LDS) and applied
+// every few sweeps in one streaming pass: each row of Z, row of H above a window or column of H right
+// of a window passes its elements through all queued sweeps in order, as a wavefront of one running
+// value per sweep, so every element is loaded and stored once per pass instead of once per sweep.
+// The sweeps are applied in their original order with the original arithmetic, so the result is the
+// same as applying each sweep in full at once.  A pass is due when the queue is full, when the next
+// window reaches above the top of a queued one (only then could the window touch queued work), and
+// before the eigenvalue loop ends.
+
+// Sweeps the queue holds (tuned): four cut the deferred traffic fourfold; more stages need more
+// registers than the occupancy-tuned eigen kernels have (each stage holds two running values).
+__host__ __device__ constexpr int eig_queue_sweeps(int QC) { return QC <= 8 ? SRCWA_EIG_QUEUE : SRCWA_EIG_QUEUE_LARGE; }
+
+struct EigQueue {
+    double* rot;       // [3 KQ][stride]: per queued sweep s, rows 3s, 3s+1, 3s+2 = c, ci, s of rotation j
+    int* lo;           // window of each queued sweep
+    int* hi;
+    double* phase;     // [2 KQ] phase of column en (1 when none)
+    int stride;        // n + 1
+};
+
+// One stream (a row for right rotations, a column for left rotations): elements k0..k1 at base[k * step]
+// pass through the queued sweeps whose bit is set in act, in order, as a systolic pipeline.  Stage s
+// holds the running value of sweep s; at step k it takes the element its predecessor emitted at step
+// k - 1 (element k - 2s; stage 0 takes the loaded element k) and emits the element before it.  All
+// stages of one step are independent, so their rotations overlap.  Outside its window a stage rotates
+// by the identity (c = 1, s = 0: emit the held value, hold the new one), i.e. it only delays.
+template <int KQ, bool LEFT>
+DEV void eig_stream(cd* base, long step, int nq, unsigned act, const EigQueue& q) {
+    // the windows are the same for every lane (scalar registers); act says which apply to this stream
+    int lo[KQ], hi[KQ];
+    int k0 = 1 << 30, k1 = -1;
+#pragma unroll
+    for (int s = 0; s < KQ; ++s) {
+        lo[s] = __builtin_amdgcn_readfirstlane(s < nq ? q.lo[s] : (1 << 30));
+        hi[s] = __builtin_amdgcn_readfirstlane(s < nq ? q.hi[s] : -2);
+        if (s >= nq) act &= ~(1u << s);
+        if ((act >> s) & 1u) {
+            k0 = min(k0, lo[s]);
+            k1 = max(k1, hi[s]);
+        }
+    }
+    if (k1 < 0) return;
+    constexpr int D = 4;                // elements loaded ahead (a shift register of loads in flight)
+    constexpr int LAG = 2 * KQ - 1;     // the last stage emits element k - LAG at step k
+    cd pend[KQ], carry[KQ], ahead[D];
+#pragma unroll
+    for (int s = 0; s < KQ; ++s) {
+        pend[s] = cd{0.0, 0.0};
+        carry[s] = cd{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < D; ++u) ahead[u] = k0 + u <= k1 ? ldc(base + (k0 + u) * step) : cd{0.0, 0.0};
+    for (int k = k0; k <= k1 + LAG; ++k) {
+        keep_loads_in_loop();     // one step's rotation coefficients at a time
+        const cd x = ahead[0];
+#pragma unroll
+        for (int u = 0; u + 1 < D; ++u) ahead[u] = ahead[u + 1];
+        ahead[D - 1] = k + D <= k1 ? ldc(base + (k + D) * step) : cd{0.0, 0.0};
+        // last stage first, so each stage still reads its predecessor's output of the previous step
+#pragma unroll
+        for (int s = KQ - 1; s >= 0; --s) {
+            const int j = k - 2 * s;
+            const bool on = (act >> s) & 1u;
+            const bool rot = on && j > lo[s] && j <= hi[s];
+            const int jc = min(max(j, 0), q.stride - 1);
+            const double* r = q.rot + 3 * s * q.stride + jc;
+            const double cr = rot ? r[0] : 1.0, ci = rot ? r[q.stride] : 0.0, sn = rot ? r[2 * q.stride] : 0.0;
+            const cd a = pend[s], b = s == 0 ? x : carry[s - 1];
+            cd out;
+            if (LEFT) {
+                out = cd{cr * a.x + ci * a.y + sn * b.x, cr * a.y - ci * a.x + sn * b.y};
+                pend[s] = cd{cr * b.x - ci * b.y - sn * a.x, cr * b.y + ci * b.x - sn * a.y};
+            } else {
+                out = cd{cr * a.x - ci * a.y + sn * b.x, cr * a.y + ci * a.x + sn * b.y};
+                pend[s] = cd{cr * b.x + ci * b.y - sn * a.x, cr * b.y - ci * b.x - sn * a.y};
+            }
+            if (on && j - 1 == hi[s]) {
+                // the end of sweep s's chain: its phase (none is stored as exactly 1)
+                const cd ph{q.phase[2 * s], q.phase[2 * s + 1]};
+                if (ph.y != 0.0) out = LEFT ? conj(ph) * out : out * ph;
+            }
+            carry[s] = out;
+        }
+        const int jo = k - LAG;
+        if (jo >= k0 && jo <= k1) stc(base + jo * step, carry[KQ - 1]);
+    }
+}
+
+// Apply the nq queued sweeps: every row of Z, the rows of H above the queued windows and the columns of
+// H right of them.  H is row-major (X), Z column-major.  Ends with a barrier.
+template <int KQ>
+DEV void eig_flush(int n, cd* __restrict__ HR, cd* __restrict__ Z, int nq, const EigQueue& q) {
+    __syncthreads();
+    int lmax = 0, emin = n;
+    for (int s = 0; s < nq; ++s) {
+        lmax = max(lmax, q.lo[s]);
+        emin = min(emin, q.hi[s]);
+    }
+    const int nrow = lmax, ncol = n - 1 - emin;
+    for (int t = threadIdx.x; t < n + nrow + ncol; t += NT) {
+        unsigned act = 0;
+        if (t < n + nrow) {
+            // right rotations: a row of Z (column-major) or a row of H above the windows (row-major)
+            const bool zrow = t < n;
+            const int r = zrow ? t : t - n;
+            for (int s = 0; s < nq; ++s)
+                if (zrow || r < q.lo[s]) act |= 1u << s;
+            eig_stream<KQ, false>(zrow ? Z + r : HR + (long)r * n, zrow ? n : 1, nq, act, q);
+        } else {
+            // left rotations: a column of H right of the windows
+            const int c = emin + 1 + (t - n - nrow);
+            for (int s = 0; s < nq; ++s)
+                if (c > q.hi[s]) act |= 1u << s;
+            eig_stream<KQ, true>(HR + c, n, nq, act, q);
+        }
     }
     __syncthreads();
 }
@@ -387,34 +517,50 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
     // the threads; the right rotations of H walk each row from one thread.  Z stays column-major.
     for (int e = tid; e < n * n; e += NT) X[e] = H[(e / n) + (e % n) * n];
     __syncthreads();
-    __shared__ double rot_c[EIG_MAX_N + 1], rot_ci[EIG_MAX_N + 1], rot_s[EIG_MAX_N + 1];
+    constexpr int KQ = eig_queue_sweeps(QC);
+    __shared__ double s_qrot[3 * KQ * (64 * QC + 1)];
+    __shared__ int s_qlo[KQ], s_qhi[KQ];
+    __shared__ double s_qphase[2 * KQ];
     __shared__ int s_info;
+    const EigQueue queue{s_qrot, s_qlo, s_qhi, s_qphase, n + 1};
+    int nq = 0, qtop = 0;          // queued sweeps; largest top row l among them
     cd t{0.0, 0.0};
     int en = n - 1, its = 0, info = 0;
     {
     cd* HR = X;
     auto h = [&](int i, int j) -> cd& { return HR[i * n + j]; };
-    while (en >= 0) {
-        int lc = 0;
-        for (int l = 1 + tid; l <= en; l += NT)
-            if (abs1(h(l, l - 1)) <= eps * (abs1(h(l - 1, l - 1)) + abs1(h(l, l)))) lc = l;
-        const int l = block_max_int(lc);
-        if (l == en) {
-            __syncthreads();
-            if (tid == 0) {
-                const cd val = h(en, en) + t;
-                w[en] = val;
-                h(en, en) = val;
+    while (true) {
+        int l = 0;
+        bool sweep = false;
+        if (en >= 0) {
+            int lc = 0;
+            for (int i = 1 + tid; i <= en; i += NT)
+                if (abs1(h(i, i - 1)) <= eps * (abs1(h(i - 1, i - 1)) + abs1(h(i, i)))) lc = i;
+            l = block_max_int(lc);
+            if (l == en) {
+                __syncthreads();
+                if (tid == 0) {
+                    const cd val = h(en, en) + t;
+                    w[en] = val;
+                    h(en, en) = val;
+                }
+                __syncthreads();
+                --en;
+                its = 0;
+                continue;
             }
-            __syncthreads();
-            --en;
-            its = 0;
-            continue;
+            if (its == max_iter) info = en + 1;
+            else sweep = true;
         }
-        if (its == max_iter) {
-            info = en + 1;
-            break;
+        // the queue is applied when full, when the next window reaches above a queued one (only then
+        // could it touch pending work), and once the iteration ends (the only call site, so the pass
+        // is compiled once)
+        if (nq > 0 && (!sweep || nq == KQ || l < qtop)) {
+            eig_flush<KQ>(n, HR, Z, nq, queue);
+            nq = 0;
+            qtop = 0;
         }
+        if (!sweep) break;
         cd sh;
         if (its == 10 || its == 20) {
             sh = cd{fabs(h(en, en - 1).x) + (en >= 2 ? fabs(h(en - 1, en - 2).x) : 0.0), 0.0};
@@ -433,6 +579,9 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
                 sh = d - scale2(cdiv(xv, ys + zz), e);
             }
         }
+        double* rot_c = s_qrot + 3 * nq * queue.stride;
+        double* rot_ci = rot_c + queue.stride;
+        double* rot_s = rot_ci + queue.stride;
         __syncthreads();
         for (int i = tid; i <= en; i += NT) h(i, i) = h(i, i) - sh;
         t = t + sh;
@@ -443,14 +592,15 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
         //     lane, rows of the row-major H loaded coalesced EIG_RING rows ahead) while the other waves
         //     wait at the barrier.  Larger orders: every thread owns columns and each rotation is
         //     published through LDS with a barrier.
+        //     Only columns l..en: the columns right of en are queued.
         if constexpr (QC <= 4) {
-            if (wave == 0) wave_left_sweep<QC, EIG_RING, true>(n, l, en, HR, rot_c, rot_ci, rot_s);
+            if (wave == 0) wave_left_sweep<QC, EIG_RING, true>(n, l, en, HR, rot_c, rot_ci, rot_s, en + 1);
         } else {
-            block_left_sweep<QC / 4, EIG_RING>(n, l, en, HR, rot_c, rot_ci, rot_s);
+            block_left_sweep<QC / 4, EIG_RING>(n, l, en, HR, rot_c, rot_ci, rot_s, en + 1);
         }
         __syncthreads();
 
-        // (2) phase that makes r(en, en) real
+        // (2) phase that makes r(en, en) real (row en right of en and column en above l are queued)
         const cd hee = h(en, en);
         const bool has_phase = hee.y != 0.0;
         cd phase{1.0, 0.0};
@@ -459,39 +609,43 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
             phase = unit_phase(hee, nrm);
             __syncthreads();
             if (tid == 0) h(en, en) = cd{nrm, 0.0};
-            for (int j = en + 1 + tid; j < n; j += NT) h(en, j) = conj(phase) * h(en, j);
         }
+        if (tid == 0) {
+            s_qlo[nq] = l;
+            s_qhi[nq] = en;
+            s_qphase[2 * nq] = phase.x;
+            s_qphase[2 * nq + 1] = phase.y;
+        }
+        ++nq;
+        qtop = max(qtop, l);
         __syncthreads();
 
-        // (3) right rotations: rows of H (0..en) and Z (0..n-1), one thread per row.  Each row loads
-        //     the next RB entries before rotating through them, so RB loads are in flight at once.
+        // (3) right rotations on the rows l..en of H, one thread per row (rows above l and Z are
+        //     queued).  Each row loads the next RB entries before rotating through them, so RB loads
+        //     are in flight at once.
         constexpr int RB = 8;
-        for (int r = tid; r < en + 1 + n; r += NT) {
-            const bool zrow = r > en;
-            const int row = zrow ? r - en - 1 : r;
-            cd* base = zrow ? Z : HR;
-            const int rs = zrow ? 1 : n, cs = zrow ? n : 1;     // Z column-major, H row-major
-            const int j0 = zrow ? l + 1 : max(l + 1, row);
-            cd a = base[row * rs + (j0 - 1) * cs];
+        for (int row = l + tid; row <= en; row += NT) {
+            cd* base = HR + (long)row * n;     // H row-major
+            const int j0 = max(l + 1, row);
+            cd a = base[j0 - 1];
             for (int jb = j0; jb <= en; jb += RB) {
                 cd bv[RB];
 #pragma unroll
                 for (int u = 0; u < RB; ++u)
-                    if (jb + u <= en) bv[u] = base[row * rs + (jb + u) * cs];
+                    if (jb + u <= en) bv[u] = base[jb + u];
 #pragma unroll
                 for (int u = 0; u < RB; ++u) {
                     const int j = jb + u;
                     if (j <= en) {
                         const double cr = rot_c[j], ci = rot_ci[j], sn = rot_s[j];
                         const cd b = bv[u];
-                        base[row * rs + (j - 1) * cs] =
-                            cd{cr * a.x - ci * a.y + sn * b.x, cr * a.y + ci * a.x + sn * b.y};
+                        base[j - 1] = cd{cr * a.x - ci * a.y + sn * b.x, cr * a.y + ci * a.x + sn * b.y};
                         a = cd{cr * b.x + ci * b.y - sn * a.x, cr * b.y - ci * b.x - sn * a.y};
                     }
                 }
             }
             if (has_phase) a = a * phase;
-            base[row * rs + en * cs] = a;
+            base[en] = a;
         }
         __syncthreads();
     }

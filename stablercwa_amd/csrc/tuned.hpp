@@ -29,5 +29,18 @@
 #define SRCWA_EIG_OCC_CROSSED SRCWA_EIG_OCC
 #endif
 
+// sweeps the eigen kernels' deferred-update queue holds, for orders up to 512 and above
+#ifndef SRCWA_EIG_QUEUE
+#define SRCWA_EIG_QUEUE 4
+#endif
+#ifndef SRCWA_EIG_QUEUE_LARGE
+#define SRCWA_EIG_QUEUE_LARGE 4
+#endif
+// minimum waves per SIMD the S-matrix kernels are built for
+#ifndef SRCWA_SMAT_OCC
+#define SRCWA_SMAT_OCC 2
+#endif
+
 // (a minimum of one wave per SIMD is the same as no bound)
 #define EIG_KERNEL_BOUNDS_OF(occ) __launch_bounds__(NT, ((occ) > 0 ? (occ) : 1))
+#define SMAT_KERNEL_BOUNDS __launch_bounds__(NT, SRCWA_SMAT_OCC)
