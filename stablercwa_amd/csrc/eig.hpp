@@ -264,7 +264,7 @@ struct EigQueue {
 // k - 1 (element k - 2s; stage 0 takes the loaded element k) and emits the element before it.  All
 // stages of one step are independent, so their rotations overlap.  Outside its window a stage rotates
 // by the identity (c = 1, s = 0: emit the held value, hold the new one), i.e. it only delays.
-template <int KQ, bool LEFT>
+template <int KQ, bool LEFT, bool ALL = false>
 DEV void eig_stream(cd* base, long step, int nq, unsigned act, const EigQueue& q) {
     // the windows are the same for every lane (scalar registers); act says which apply to this stream
     int lo[KQ], hi[KQ];
@@ -300,11 +300,17 @@ DEV void eig_stream(cd* base, long step, int nq, unsigned act, const EigQueue& q
 #pragma unroll
         for (int s = KQ - 1; s >= 0; --s) {
             const int j = k - 2 * s;
-            const bool on = (act >> s) & 1u;
-            const bool rot = on && j > lo[s] && j <= hi[s];
+            const bool on = s < nq && (ALL || ((act >> s) & 1u));
+            // a sweep's table holds the identity outside its window (and every index is clamped into
+            // the table), so only a stage that is unused or does not apply to this stream needs a select
             const int jc = min(max(j, 0), q.stride - 1);
             const double* r = q.rot + 3 * s * q.stride + jc;
-            const double cr = rot ? r[0] : 1.0, ci = rot ? r[q.stride] : 0.0, sn = rot ? r[2 * q.stride] : 0.0;
+            double cr = r[0], ci = r[q.stride], sn = r[2 * q.stride];
+            if (!on) {
+                cr = 1.0;
+                ci = 0.0;
+                sn = 0.0;
+            }
             const cd a = pend[s], b = s == 0 ? x : carry[s - 1];
             cd out;
             if (LEFT) {
@@ -345,7 +351,8 @@ DEV void eig_flush(int n, cd* __restrict__ HR, cd* __restrict__ Z, int nq, const
             const int r = zrow ? t : t - n;
             for (int s = 0; s < nq; ++s)
                 if (zrow || r < q.lo[s]) act |= 1u << s;
-            eig_stream<KQ, false>(zrow ? Z + r : HR + (long)r * n, zrow ? n : 1, nq, act, q);
+            if (zrow) eig_stream<KQ, false, true>(Z + r, n, nq, act, q);
+            else eig_stream<KQ, false>(HR + (long)r * n, 1, nq, act, q);
         } else {
             // left rotations: a column of H right of the windows
             const int c = emin + 1 + (t - n - nrow);
@@ -582,6 +589,13 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
         double* rot_c = s_qrot + 3 * nq * queue.stride;
         double* rot_ci = rot_c + queue.stride;
         double* rot_s = rot_ci + queue.stride;
+        // the identity outside this sweep's window (the left sweep fills l+1..en)
+        for (int j = tid; j <= n; j += NT)
+            if (j <= l || j > en) {
+                rot_c[j] = 1.0;
+                rot_ci[j] = 0.0;
+                rot_s[j] = 0.0;
+            }
         __syncthreads();
         for (int i = tid; i <= en; i += NT) h(i, i) = h(i, i) - sh;
         t = t + sh;

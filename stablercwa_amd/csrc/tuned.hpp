@@ -10,10 +10,10 @@
 #define SRCWA_EIG_OCC_PLANAR 3
 #endif
 #ifndef SRCWA_EIG_OCC_CONICAL
-#define SRCWA_EIG_OCC_CONICAL 0
+#define SRCWA_EIG_OCC_CONICAL 2
 #endif
 #ifndef SRCWA_EIG_OCC_CROSSED
-#define SRCWA_EIG_OCC_CROSSED 0
+#define SRCWA_EIG_OCC_CROSSED 2
 #endif
 #ifndef SRCWA_WAVE_EIG_MAX_N
 #define SRCWA_WAVE_EIG_MAX_N 64 // orders up to which one wave solves a matrix
