@@ -324,7 +324,10 @@ DEV void lu_panel_regs(int m, int nb, cd* Ap, int lda, LuPanelShared& S) {
         const int row = tid + r * NT;
         orig[r] = row;
 #pragma unroll
-        for (int c = 0; c < LU_NB; ++c) v[r][c] = (row < m && c < nb) ? ldc(Ap + row + (long)c * lda) : cd{0.0, 0.0};
+        for (int c = 0; c < LU_NB; ++c) {
+            const cd x = ldc(Ap + (row < m ? row : 0) + (long)(c < nb ? c : 0) * lda);
+            v[r][c] = (row < m && c < nb) ? x : cd{0.0, 0.0};
+        }
     }
     int sing = 0;
     // the column loop stays rolled; column jj of a register row is picked by unrolled selects, so the
@@ -406,7 +409,7 @@ DEV void lu_panel_regs(int m, int nb, cd* Ap, int lda, LuPanelShared& S) {
 #pragma unroll
                 for (int c = 0; c < LU_NB; ++c) {
                     if (c == jj) v[r][c] = l;
-                    else if (c > jj && c < nb) v[r][c] = v[r][c] - l * S.prow[c];
+                    else if (c > jj) v[r][c] = v[r][c] - l * S.prow[c];     // zeros past nb, never stored
                 }
             }
         }
@@ -417,8 +420,7 @@ DEV void lu_panel_regs(int m, int nb, cd* Ap, int lda, LuPanelShared& S) {
         const int row = tid + r * NT;
         if (row < m) {
 #pragma unroll
-            for (int c = 0; c < LU_NB; ++c)
-                if (c < nb) stc(Ap + row + (long)c * lda, v[r][c]);
+            for (int c = 0; c < LU_NB; ++c) stc(Ap + row + (long)(c < nb ? c : 0) * lda, c < nb ? v[r][c] : v[r][0]);
         }
         if (row < nb) {
 #pragma unroll
@@ -431,6 +433,120 @@ DEV void lu_panel_regs(int m, int nb, cd* Ap, int lda, LuPanelShared& S) {
         }
     }
     if (tid == 0 && sing) S.sing = 1;
+}
+
+// Panels of up to 64 RL rows, factored by wave 0 alone with the panel in its registers: lane l holds
+// rows l, l + 64, ... (all LU_NB columns).  Per column: pivot search by a wave reduction, the exchange of
+// the two rows and the broadcast of the pivot row through LDS, multipliers and the rank-1 update in
+// registers -- no workgroup barrier at all.  The column loop is unrolled, so every register index is
+// static.  Same pivots, multipliers and update order as lu_panel_regs; the caller's barrier publishes
+// the results to the other waves.
+template <int RL>
+DEV void lu_panel_wave(int m, int nb, cd* Ap, int lda, LuPanelShared& S) {
+    const int lane = lane_id();
+    cd v[RL][LU_NB];
+    int orig[RL];
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+        const int row = lane + 64 * q;
+        orig[q] = row;
+        // clamped addresses and selects instead of guarded loads: a guard on c < nb would turn the
+        // loop into one with a run-time trip count and the rows into private memory
+        const int rr = row < m ? row : 0;
+#pragma unroll
+        for (int c = 0; c < LU_NB; ++c) {
+            const cd x = ldc(Ap + rr + (long)(c < nb ? c : 0) * lda);
+            v[q][c] = (row < m && c < nb) ? x : cd{0.0, 0.0};
+        }
+    }
+    int sing = 0;
+#pragma unroll
+    for (int jj = 0; jj < LU_NB; ++jj) {
+        if (jj < nb) {
+            double best = -1.0;
+            int bi = m;
+#pragma unroll
+            for (int q = 0; q < RL; ++q) {
+                const int row = lane + 64 * q;
+                if (row >= jj && row < m) {
+                    const double a = cabs(v[q][jj]);
+                    if (a > best) {
+                        best = a;
+                        bi = row;
+                    }
+                }
+            }
+            wave_argmax(best, bi);
+            const int p = bi >= m ? jj : bi;
+#pragma unroll
+            for (int q = 0; q < RL; ++q) {
+                const int row = lane + 64 * q;
+                if (row == p) {
+#pragma unroll
+                    for (int c = 0; c < LU_NB; ++c) stc(&S.prow[c], v[q][c]);
+                    S.porig = orig[q];
+                }
+                if (row == jj && p != jj) {
+#pragma unroll
+                    for (int c = 0; c < LU_NB; ++c) stc(&S.jrow[c], v[q][c]);
+                    S.jorig = orig[q];
+                }
+            }
+            if (lane == 0) S.piv[jj] = p;
+            wave_sync();
+            const cd pivot = ldc(&S.prow[jj]);
+            const bool zero = is_zero(pivot);
+            sing |= zero;
+            const cd inv = zero ? cd{0.0, 0.0} : cdiv(cd{1.0, 0.0}, pivot);
+            const bool fin = finite(inv);
+#pragma unroll
+            for (int q = 0; q < RL; ++q) {
+                const int row = lane + 64 * q;
+                if (p != jj) {
+                    if (row == jj) {
+#pragma unroll
+                        for (int c = 0; c < LU_NB; ++c) v[q][c] = ldc(&S.prow[c]);
+                        orig[q] = S.porig;
+                    } else if (row == p) {
+#pragma unroll
+                        for (int c = 0; c < LU_NB; ++c) v[q][c] = ldc(&S.jrow[c]);
+                        orig[q] = S.jorig;
+                    }
+                }
+                if (row > jj && row < m) {
+                    const cd l = zero ? cd{0.0, 0.0} : (fin ? v[q][jj] * inv : cdiv(v[q][jj], pivot));
+                    // columns past nb hold zeros in every row (the pivot row included) and are never
+                    // stored, so they are updated without a guard; the loop has a fixed trip count so it
+                    // unrolls before the column loop does
+#pragma unroll
+                    for (int c = 0; c < LU_NB; ++c) {
+                        if (c == jj) v[q][c] = l;
+                        else if (c > jj) v[q][c] = v[q][c] - l * ldc(&S.prow[c]);
+                    }
+                }
+            }
+            wave_sync();     // every lane has read the exchange buffers before the next column writes them
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+        const int row = lane + 64 * q;
+        if (row < m) {
+            // past nb, column 0 is stored again with its own value
+#pragma unroll
+            for (int c = 0; c < LU_NB; ++c) stc(Ap + row + (long)(c < nb ? c : 0) * lda, c < nb ? v[q][c] : v[q][0]);
+        }
+        if (row < nb) {
+#pragma unroll
+            for (int c = 0; c < LU_NB; ++c) S.L11[row + c * LU_NB] = c < row ? v[q][c] : cd{0.0, 0.0};
+            S.src[row] = orig[q];
+        } else if (row < m && orig[q] != row) {
+            const int k = atomicAdd(&S.npairs, 1);
+            S.pdst[k] = row;
+            S.psrc[k] = orig[q];
+        }
+    }
+    if (lane == 0 && sing) S.sing = 1;
 }
 
 // Same outputs as lu_panel_regs for panels too tall for registers: one wave factors the panel in
@@ -569,7 +685,8 @@ DEV void lu_columns_partial(int n, int ks, int nb, cd* A, int lda, const LuPanel
 }
 
 // P A = L U in place, right-looking with panels of LU_NB columns and partial pivoting (largest
-// modulus, first on ties).  Per panel: the panel factorization (in registers for up to 2 NT rows),
+// modulus, first on ties).  Per panel: the panel factorization (in one wave's registers for up to 128
+// rows, in the workgroup's registers for up to 2 NT),
 // then one pass over the other columns that applies the panel's row permutation and, right of the
 // panel, solves U12 = L11^{-1} A12; then the trailing MFMA update.
 DEV int wg_lu(int n, cd* A, int lda, int* piv) {
@@ -588,7 +705,12 @@ DEV int wg_lu(int n, cd* A, int lda, int* piv) {
             S.psrc[tid] = 0;
         }
         __syncthreads();
-        if (m <= NT) lu_panel_regs<1>(m, nb, Ap, lda, S);
+        if (m <= 128) {
+            if (wave_id() == 0) {
+                if (m <= 64) lu_panel_wave<1>(m, nb, Ap, lda, S);
+                else lu_panel_wave<2>(m, nb, Ap, lda, S);
+            }
+        } else if (m <= NT) lu_panel_regs<1>(m, nb, Ap, lda, S);
         else if (m <= 2 * NT) lu_panel_regs<2>(m, nb, Ap, lda, S);
         else lu_panel_global(m, nb, Ap, lda, S, s_trow, s_tsrc);
         __syncthreads();
