@@ -218,15 +218,14 @@ DEV DOp op_add(SmatCtx& c, DOp a, DOp b) {
     }
     const int64_t NN = (int64_t)N * N;
     cd* m = c.bump.tmp(NN);
-    for (int64_t e = threadIdx.x; e < NN; e += NT) {
-        const int i = (int)(e % N), j = (int)(e / N);
-        cd v;
-        if (a.kind == OP_DENSE && b.kind == OP_DENSE) v = a.p[e] + b.p[e];
-        else if (a.kind == OP_DENSE) v = i == j ? a.p[e] + b.p[i] : a.p[e];
-        else v = i == j ? b.p[e] + a.p[i] : b.p[e];
-        m[e] = v;
-    }
-    __syncthreads();
+    if (a.kind == OP_DENSE && b.kind == OP_DENSE)
+        wg_map((int)NN, [&](int e) { return ldc(a.p + e) + ldc(b.p + e); }, [&](int e, cd v) { stc(m + e, v); });
+    else if (a.kind == OP_DENSE)
+        wg_map((int)NN, [&](int e) { const int i = e % N; return i == e / N ? ldc(a.p + e) + ldc(b.p + i) : ldc(a.p + e); },
+               [&](int e, cd v) { stc(m + e, v); });
+    else
+        wg_map((int)NN, [&](int e) { const int i = e % N; return i == e / N ? ldc(b.p + e) + ldc(a.p + i) : ldc(b.p + e); },
+               [&](int e, cd v) { stc(m + e, v); });
     return DOp{OP_DENSE, m};
 }
 
@@ -356,19 +355,14 @@ DEV void interface_matching(SmatCtx& c, const Region& up, const Region& low, cd*
             c.fail = 1;
             return;
         }
-        for (int64_t e = threadIdx.x; e < NN; e += NT) {
-            const int i = (int)(e % N);
-            mm[e] = cdiv(cd{1.0, 0.0}, low.w[i]) * up.W[e];
-            nn[e] = cdiv(cd{1.0, 0.0}, low.p[i]) * up.V[e];
-        }
-        __syncthreads();
+        wg_map((int)NN, [&](int e) { return cdiv(cd{1.0, 0.0}, ldc(low.w + e % N)) * ldc(up.W + e); },
+               [&](int e, cd v) { stc(mm + e, v); });
+        wg_map((int)NN, [&](int e) { return cdiv(cd{1.0, 0.0}, ldc(low.p + e % N)) * ldc(up.V + e); },
+               [&](int e, cd v) { stc(nn + e, v); });
     } else if (low.analytic) {
         wg_copy(N, N, up.W, N, mm, N);
-        for (int64_t e = threadIdx.x; e < NN; e += NT) {
-            const int i = (int)(e % N);
-            nn[e] = cdiv(cd{1.0, 0.0}, low.p[i]) * up.V[e];
-        }
-        __syncthreads();
+        wg_map((int)NN, [&](int e) { return cdiv(cd{1.0, 0.0}, ldc(low.p + e % N)) * ldc(up.V + e); },
+               [&](int e, cd v) { stc(nn + e, v); });
     } else if (up.analytic) {
         if (up.w != nullptr) wg_diag(N, up.w, mm, N);
         else wg_identity(N, mm, N);
@@ -382,12 +376,18 @@ DEV void interface_matching(SmatCtx& c, const Region& up, const Region& low, cd*
         dense_solve(c, low.V, nn, N);
     }
     if (c.fail) return;
-    for (int64_t e = threadIdx.x; e < NN; e += NT) {
-        const cd a = mm[e], b = nn[e];
-        mm[e] = 0.5 * (a + b);
-        nn[e] = 0.5 * (a - b);
-    }
-    __syncthreads();
+    struct Pair {
+        cd p, m;
+    };
+    wg_map((int)NN,
+           [&](int e) {
+               const cd a = ldc(mm + e), b = ldc(nn + e);
+               return Pair{0.5 * (a + b), 0.5 * (a - b)};
+           },
+           [&](int e, Pair v) {
+               stc(mm + e, v.p);
+               stc(nn + e, v.m);
+           });
     tp = mm;
     tm = nn;
 }
@@ -453,13 +453,13 @@ DEV void stack_on(SmatCtx& c, const SBlock& s, DOp r, DOp& g, DOp& rn) {
         g = op_mul(c, k, s.Tdd);
     } else {
         const int64_t NN = (int64_t)N * N;
-        for (int64_t e = threadIdx.x; e < NN; e += NT) {
-            const int i = (int)(e % N), j = (int)(e / N);
-            cd v = -k.p[e];
-            if (i == j) v = v + cd{1.0, 0.0};
-            k.p[e] = v;
-        }
-        __syncthreads();
+        wg_map((int)NN,
+               [&](int e) {
+                   cd v = -ldc(k.p + e);
+                   if (e % N == e / N) v = v + cd{1.0, 0.0};
+                   return v;
+               },
+               [&](int e, cd v) { stc(k.p + e, v); });
         cd* rhs = op_to_dense(c, s.Tdd);
         c.check();
         if (c.fail) return;
@@ -479,10 +479,7 @@ DEV DOp op_keep(SmatCtx& c, DOp a, cd* slot) {
     const int N = c.N;
     if (a.kind == OP_ZERO) return a;
     const int64_t cnt = a.kind == OP_DIAG ? N : (int64_t)N * N;
-    if (a.p != slot) {
-        for (int64_t e = threadIdx.x; e < cnt; e += NT) slot[e] = a.p[e];
-        __syncthreads();
-    }
+    if (a.p != slot) wg_map((int)cnt, [&](int e) { return ldc(a.p + e); }, [&](int e, cd v) { stc(slot + e, v); });
     return DOp{a.kind, slot};
 }
 
@@ -589,13 +586,13 @@ DEV void run_recursion(SmatCtx& c, int L, double k0, const double* thick, Region
         }
         __syncthreads();
     } else if (k.kind == OP_DENSE) {
-        for (int64_t e = threadIdx.x; e < NN; e += NT) {
-            const int i = (int)(e % N), j = (int)(e / N);
-            cd v = -k.p[e];
-            if (i == j) v = v + cd{1.0, 0.0};
-            k.p[e] = v;
-        }
-        __syncthreads();
+        wg_map((int)NN,
+               [&](int e) {
+                   cd v = -ldc(k.p + e);
+                   if (e % N == e / N) v = v + cd{1.0, 0.0};
+                   return v;
+               },
+               [&](int e, cd v) { stc(k.p + e, v); });
         dense_solve_inplace(c, k.p, down, nrhs);
         if (c.fail) return;
     }

@@ -234,47 +234,55 @@ DEV void wg_update(int m, int nrhs, int k, const cd* A, int lda, const cd* B, in
 
 // ------------------------------------------------------------------------------------------------
 // Elementwise helpers (all end with a barrier).
-
-DEV void wg_copy(int m, int n, const cd* src, int lds, cd* dst, int ldd) {
-    for (int e = threadIdx.x; e < m * n; e += NT) {
-        const int i = e % m, j = e / m;
-        dst[i + (long)j * ldd] = src[i + (long)j * lds];
+//
+// wg_map runs an elementwise pass with WG_MAP_U loads in flight per thread: ld(e) computes the new value
+// (or values) of element e from loads only, st(e, v) stores it.  Each group's loads are issued before any of its
+// stores, so a pass whose output may alias its input (which keeps the compiler from reordering) still
+// overlaps the memory latency of WG_MAP_U elements instead of paying it once per element.  Passes are
+// strictly elementwise: element e reads nothing another element of the pass writes.
+constexpr int WG_MAP_U = 8;
+template <class Load, class Store>
+DEV void wg_map(int count, Load ld, Store st) {
+    int e0 = threadIdx.x;
+    for (; e0 + (WG_MAP_U - 1) * NT < count; e0 += WG_MAP_U * NT) {
+        decltype(ld(0)) v[WG_MAP_U];
+#pragma unroll
+        for (int u = 0; u < WG_MAP_U; ++u) v[u] = ld(e0 + u * NT);
+#pragma unroll
+        for (int u = 0; u < WG_MAP_U; ++u) st(e0 + u * NT, v[u]);
     }
+    for (int e = e0; e < count; e += NT) st(e, ld(e));
     __syncthreads();
 }
+
+DEV void wg_copy(int m, int n, const cd* src, int lds, cd* dst, int ldd) {
+    wg_map(m * n, [&](int e) { return ldc(src + e % m + (long)(e / m) * lds); },
+           [&](int e, cd v) { stc(dst + e % m + (long)(e / m) * ldd, v); });
+}
 DEV void wg_fill(long count, cd* dst, cd v) {
-    for (long e = threadIdx.x; e < count; e += NT) dst[e] = v;
+    for (long e = threadIdx.x; e < count; e += NT) stc(dst + e, v);
     __syncthreads();
 }
 DEV void wg_identity(int n, cd* dst, int ld) {
     for (int e = threadIdx.x; e < n * n; e += NT) {
         const int i = e % n, j = e / n;
-        dst[i + (long)j * ld] = cd{i == j ? 1.0 : 0.0, 0.0};
+        stc(dst + i + (long)j * ld, cd{i == j ? 1.0 : 0.0, 0.0});
     }
     __syncthreads();
 }
 DEV void wg_diag(int n, const cd* d, cd* dst, int ld) {
-    for (int e = threadIdx.x; e < n * n; e += NT) {
-        const int i = e % n, j = e / n;
-        dst[i + (long)j * ld] = i == j ? d[i] : cd{0.0, 0.0};
-    }
-    __syncthreads();
+    wg_map(n * n, [&](int e) { const int i = e % n; return i == e / n ? ldc(d + i) : cd{0.0, 0.0}; },
+           [&](int e, cd v) { stc(dst + e % n + (long)(e / n) * ld, v); });
 }
 // dst(i, j) = d[i] * src(i, j)      (diagonal times dense)
 DEV void wg_scale_rows(int m, int n, const cd* d, const cd* src, int lds, cd* dst, int ldd) {
-    for (int e = threadIdx.x; e < m * n; e += NT) {
-        const int i = e % m, j = e / m;
-        dst[i + (long)j * ldd] = d[i] * src[i + (long)j * lds];
-    }
-    __syncthreads();
+    wg_map(m * n, [&](int e) { const int i = e % m; return ldc(d + i) * ldc(src + i + (long)(e / m) * lds); },
+           [&](int e, cd v) { stc(dst + e % m + (long)(e / m) * ldd, v); });
 }
 // dst(i, j) = src(i, j) * d[j]      (dense times diagonal)
 DEV void wg_scale_cols(int m, int n, const cd* src, int lds, const cd* d, cd* dst, int ldd) {
-    for (int e = threadIdx.x; e < m * n; e += NT) {
-        const int i = e % m, j = e / m;
-        dst[i + (long)j * ldd] = src[i + (long)j * lds] * d[j];
-    }
-    __syncthreads();
+    wg_map(m * n, [&](int e) { const int j = e / m; return ldc(src + e % m + (long)j * lds) * ldc(d + j); },
+           [&](int e, cd v) { stc(dst + e % m + (long)(e / m) * ldd, v); });
 }
 
 // ------------------------------------------------------------------------------------------------
