@@ -64,13 +64,14 @@ __global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ in
         fourier_coefficients(lv, period, 2 * M, false, coef);
         toeplitz(coef, N, S0);
         // A = Kx^2 - [[eps]]
-        for (int64_t e = threadIdx.x; e < NN; e += NT) {
-            const int i = (int)(e % N), c = (int)(e / N);
-            cd v = -S0[e];
-            if (i == c) v = v + cd{kx[i].x * kx[i].x, 0.0};
-            S0[e] = v;
-        }
-        __syncthreads();
+        wg_map((int)NN,
+               [&](int e) {
+                   const int i = e % N;
+                   cd v = -ldc(S0 + e);
+                   if (i == e / N) v = v + cd{kx[i].x * kx[i].x, 0.0};
+                   return v;
+               },
+               [&](int e, cd v) { stc(S0 + e, v); });
     } else {
         // S1 = [[eps]], S3 = [[1/eps]] (kept for V), S2 = diag(kx)
         fourier_coefficients(lv, period, 2 * M, false, coef);
@@ -81,13 +82,14 @@ __global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ in
         fail |= wg_lu(N, S1, N, piv);
         wg_lu_solve(N, S1, N, piv, S2, N, N);
         // S2 = Kx S2 - I
-        for (int64_t e = threadIdx.x; e < NN; e += NT) {
-            const int i = (int)(e % N), c = (int)(e / N);
-            cd v = kx[i].x * S2[e];
-            if (i == c) v = v - cd{1.0, 0.0};
-            S2[e] = v;
-        }
-        __syncthreads();
+        wg_map((int)NN,
+               [&](int e) {
+                   const int i = e % N;
+                   cd v = kx[i].x * ldc(S2 + e);
+                   if (i == e / N) v = v - cd{1.0, 0.0};
+                   return v;
+               },
+               [&](int e, cd v) { stc(S2 + e, v); });
         wg_copy(N, N, S3, N, S0, N);
         fail |= wg_lu(N, S0, N, piv);
         wg_lu_solve(N, S0, N, piv, S2, N, N);

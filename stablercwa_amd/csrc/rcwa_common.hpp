@@ -92,12 +92,7 @@ DEV void fourier_coefficients(const LayerView& lv, double period, int hmax, bool
 
 // [[f]](m, p) = c_{m-p}; c holds 2N-1 coefficients with c_0 at index N-1.
 DEV void toeplitz(const cd* c, int N, cd* S) {
-    const int64_t NN = (int64_t)N * N;
-    for (int64_t e = threadIdx.x; e < NN; e += NT) {
-        const int i = (int)(e % N), j = (int)(e / N);
-        S[e] = c[i - j + N - 1];
-    }
-    __syncthreads();
+    wg_map(N * N, [&](int e) { return ldc(c + (e % N) - (e / N) + N - 1); }, [&](int e, cd v) { stc(S + e, v); });
 }
 
 // Re q >= 0; propagating (Re q ~ 0) takes Im q <= 0 (rcwa1d.hpp:514-518).

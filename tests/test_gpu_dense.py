@@ -89,3 +89,46 @@ def test_zgeev_against_reference_residual(ctx, ref_lib):
         _, _, rinfo, rres = ref.eig(ref_lib, A)
         assert rinfo == 0
         assert res <= 4 * max(rres, 1e-16), (res, rres)
+
+
+def test_zgesv_every_panel_path(ctx):
+    """Orders on both sides of each LU panel form (one wave: up to 64 and 128 rows; the workgroup: up to 256
+    and 512 rows; global memory above) and of the last, narrower panel, with right-hand-side counts on
+    both sides of the MFMA / VALU substitution update; entries of equal modulus force pivot ties."""
+    from stablercwa_amd import rcwa
+    rng = np.random.default_rng(21)
+    cases = [(63, 1), (64, 64), (65, 8), (127, 7), (128, 9), (129, 129), (144, 3), (255, 1), (256, 256),
+             (257, 2), (511, 16), (512, 5), (513, 513), (600, 1)]
+    As, Bs = [], []
+    for n, r in cases:
+        A = crand(rng, n, n)
+        if n in (65, 129, 257):
+            A = (rng.integers(0, 2, (n, n)) * 2 - 1) + 1j * (rng.integers(0, 2, (n, n)) * 2 - 1)  # ties
+            A = A + 0.05 * np.eye(n)
+        As.append(A)
+        Bs.append(crand(rng, n, r))
+    out = rcwa.zgesv_batched(As, Bs, ctx)
+    for (n, r), A, B, (X, info) in zip(cases, As, Bs, out):
+        assert info == 0, n
+        berr = np.linalg.norm(A @ X - B) / (np.linalg.norm(A) * np.linalg.norm(X) + np.linalg.norm(B))
+        assert berr <= 1e-14, (n, r, berr)
+
+
+def test_zgeev_split_windows(ctx):
+    """Matrices whose QR iteration splits into separate windows (weakly coupled diagonal blocks), at orders
+    of each eigen-kernel instantiation: the queued off-window updates must land on the right rows and
+    columns whatever order the windows come in."""
+    from stablercwa_amd import rcwa
+    rng = np.random.default_rng(22)
+    mats = []
+    for n in (90, 200, 300, 530):
+        A = crand(rng, n, n)
+        cut = [0, n // 3, 2 * n // 3, n]
+        for a in range(3):
+            for b in range(3):
+                if a != b:
+                    A[cut[a]:cut[a + 1], cut[b]:cut[b + 1]] *= 1e-13
+        mats.append(A)
+    for A in mats:      # one call each: a batch runs the instantiation of its largest order
+        (w, V, info), = rcwa.zgeev_batched([A], ctx)
+        _check_eig(A, w, V, info, tol=1e-12)
