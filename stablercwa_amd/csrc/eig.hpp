@@ -54,6 +54,39 @@ struct EigScratch {
     cd* tmp;     // n
 };
 
+// Rotation G_i from the carried diagonal (xr + i xi) and the real sub-diagonal sub: c = x / |(x, sub)|,
+// s = sub / |(x, sub)| (complex cosine, real sine), and the new diagonal |(x, sub)|.  Operands far from 1
+// are scaled by a power of two first.  1/|(x, sub)| is the hardware reciprocal square root refined by two
+// Newton steps (y <- y + y (1 - s y^2) / 2, each doubling the correct bits): a dependent chain of about
+// ten FP64 operations instead of a correctly rounded sqrt and a division.  Both sweep forms use this one
+// function, so a matrix gets the same rotations whichever instantiation decomposes it.
+DEV double qr_rotation(double xr, double xi, double sub, double& cr, double& ci, double& sn) {
+    const double mm = fmax(fmax(fabs(xr), fabs(xi)), fabs(sub));
+    int e = 0;
+    if (!(mm > 1e-140 && mm < 1e140)) {
+        e = binary_exponent(mm);
+        xr = ldexp(xr, -e);
+        xi = ldexp(xi, -e);
+        sub = ldexp(sub, -e);
+    }
+    const double ss = fma(sub, sub, fma(xi, xi, xr * xr));
+    double inv = __builtin_amdgcn_rsq(ss);
+    inv = fma(0.5 * inv, fma(-ss * inv, inv, 1.0), inv);
+    inv = fma(0.5 * inv, fma(-ss * inv, inv, 1.0), inv);
+    const double nrm = ss * inv;
+    cr = xr * inv;
+    ci = xi * inv;
+    sn = sub * inv;
+    return e ? ldexp(nrm, e) : nrm;
+}
+
+// Left rotation of one column's rows (i-1, i): top = conj(c) y + s z, bottom = c z - s y.  Written with
+// explicit fused multiply-adds so both sweep forms evaluate it identically.
+DEV void rotate_left(double cr, double ci, double sn, cd y, cd z, cd& top, cd& bottom) {
+    top = cd{fma(sn, z.x, fma(ci, y.y, cr * y.x)), fma(sn, z.y, fma(-ci, y.x, cr * y.y))};
+    bottom = cd{fma(-sn, y.x, fma(-ci, z.y, cr * z.x)), fma(-sn, y.y, fma(ci, z.x, cr * z.y))};
+}
+
 // The left half of one QR sweep, run by a single wave: rotations G_{l+1..en} (complex cosine, real
 // sine, each formed from the diagonal carry of column i-1 and the real sub-diagonal h(i,i-1)) applied to
 // rows l..en of columns l..n-1, the rotations stored in rot_c / rot_ci / rot_s for the right half.
@@ -102,26 +135,10 @@ DEV void wave_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c
                     xc = carry[q];
                     zc = cur[q];
                 }
-            double xr = read_lane(xc.x, ol), xi = read_lane(xc.y, ol), sub = read_lane(zc.x, ol);
-            const double mm = fmax(fmax(fabs(xr), fabs(xi)), fabs(sub));
-            int e = 0;
-            if (!(mm > 1e-140 && mm < 1e140)) {
-                e = binary_exponent(mm);
-                xr = ldexp(xr, -e);
-                xi = ldexp(xi, -e);
-                sub = ldexp(sub, -e);
-            }
-            // 1/|(x, sub)|: the hardware reciprocal square root refined by two Newton steps
-            // (y <- y + y (1 - s y^2) / 2, each doubling the correct bits), a dependent chain of
-            // about ten FP64 operations instead of a correctly rounded sqrt and a division
-            const double ss = xr * xr + xi * xi + sub * sub;
-            double inv = __builtin_amdgcn_rsq(ss);
-            inv = fma(0.5 * inv, fma(-ss * inv, inv, 1.0), inv);
-            inv = fma(0.5 * inv, fma(-ss * inv, inv, 1.0), inv);
-            const double nrm = ss * inv;
-            const double cr = xr * inv, ci = xi * inv, sn = sub * inv;
+            double cr, ci, sn;
+            const double nrm = qr_rotation(read_lane(xc.x, ol), read_lane(xc.y, ol), read_lane(zc.x, ol), cr, ci, sn);
             if (lane == ol) {
-                h(i - 1, i - 1) = cd{e ? ldexp(nrm, e) : nrm, 0.0};
+                h(i - 1, i - 1) = cd{nrm, 0.0};
                 h(i, i - 1) = cd{0.0, 0.0};
             }
             if (lane == 0) {
@@ -133,10 +150,9 @@ DEV void wave_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c
             for (int q = 0; q < LC; ++q) {
                 const int c = l + lane + 64 * q;
                 if (c >= i && c < cend) {
-                    const cd yv = carry[q];
-                    const cd z = cur[q];
-                    h(i - 1, c) = cd{cr * yv.x + ci * yv.y + sn * z.x, cr * yv.y - ci * yv.x + sn * z.y};
-                    carry[q] = cd{cr * z.x - ci * z.y - sn * yv.x, cr * z.y + ci * z.x - sn * yv.y};
+                    cd top;
+                    rotate_left(cr, ci, sn, carry[q], cur[q], top, carry[q]);
+                    h(i - 1, c) = top;
                 }
             }
         }
@@ -192,22 +208,13 @@ DEV void block_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_
                         xc = carry[q];
                         zc = cur[q];
                     }
-                double xr = xc.x, xi = xc.y, sub = zc.x;
-                const double mm = fmax(fmax(fabs(xr), fabs(xi)), fabs(sub));
-                int e = 0;
-                if (!(mm > 1e-140 && mm < 1e140)) {
-                    e = binary_exponent(mm);
-                    xr = ldexp(xr, -e);
-                    xi = ldexp(xi, -e);
-                    sub = ldexp(sub, -e);
-                }
-                const double nrm = sqrt(xr * xr + xi * xi + sub * sub);
-                const double inv = 1.0 / nrm;
-                h(i - 1, i - 1) = cd{e ? ldexp(nrm, e) : nrm, 0.0};
+                double cr, ci, sn;
+                const double nrm = qr_rotation(xc.x, xc.y, zc.x, cr, ci, sn);
+                h(i - 1, i - 1) = cd{nrm, 0.0};
                 h(i, i - 1) = cd{0.0, 0.0};
-                rot_c[i] = xr * inv;
-                rot_ci[i] = xi * inv;
-                rot_s[i] = sub * inv;
+                rot_c[i] = cr;
+                rot_ci[i] = ci;
+                rot_s[i] = sn;
             }
             __syncthreads();
             const double cr = rot_c[i], ci = rot_ci[i], sn = rot_s[i];
@@ -215,10 +222,9 @@ DEV void block_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_
             for (int q = 0; q < CPT; ++q) {
                 const int c = l + tid + q * NT;
                 if (c >= i && c < cend) {
-                    const cd yv = carry[q];
-                    const cd z = cur[q];
-                    h(i - 1, c) = cd{cr * yv.x + ci * yv.y + sn * z.x, cr * yv.y - ci * yv.x + sn * z.y};
-                    carry[q] = cd{cr * z.x - ci * z.y - sn * yv.x, cr * z.y + ci * z.x - sn * yv.y};
+                    cd top;
+                    rotate_left(cr, ci, sn, carry[q], cur[q], top, carry[q]);
+                    h(i - 1, c) = top;
                 }
             }
         }

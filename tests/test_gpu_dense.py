@@ -132,3 +132,17 @@ def test_zgeev_split_windows(ctx):
     for A in mats:      # one call each: a batch runs the instantiation of its largest order
         (w, V, info), = rcwa.zgeev_batched([A], ctx)
         _check_eig(A, w, V, info, tol=1e-12)
+
+
+def test_zgeev_result_independent_of_batch(ctx):
+    """A batch runs the eigen instantiation of its largest order (one wave carries the QR rotation chain
+    up to order 256, the whole workgroup above); both compute the same rotations, so a matrix's
+    eigenpairs are bitwise the same alone and next to a larger matrix."""
+    from stablercwa_amd import rcwa
+    rng = np.random.default_rng(23)
+    A, B = crand(rng, 150, 150), crand(rng, 300, 300)
+    (w1, V1, i1), = rcwa.zgeev_batched([A], ctx)
+    (w2, V2, i2), _ = rcwa.zgeev_batched([A, B], ctx)
+    assert i1 == 0 and i2 == 0
+    assert np.array_equal(w1, w2)
+    assert np.array_equal(V1, V2)
