@@ -736,7 +736,8 @@ DEV int wg_lu(int n, cd* A, int lda, int* piv) {
     return sing;
 }
 
-// Row permutation of wg_lu's pivots applied to B (n x nrhs): staged through LDS in column blocks.
+// Row permutation of wg_lu's pivots applied to B (n x nrhs): the rows that move are staged through LDS in
+// column blocks.
 constexpr int SOLVE_STAGE_WORDS = 1024;
 constexpr int SOLVE_MAX_PERM = 2048;
 
@@ -746,6 +747,8 @@ constexpr int SOLVE_MAX_PERM = 2048;
 // sides).  The substitution order within a block is that of plain forward / backward substitution.
 DEV void wg_lu_solve(int n, const cd* LU, int lda, const int* piv, cd* B, int ldb, int nrhs) {
     __shared__ int s_perm[SOLVE_MAX_PERM];
+    __shared__ int s_moved[SOLVE_MAX_PERM];
+    __shared__ int s_nmoved;
     __shared__ cd s_stage[SOLVE_STAGE_WORDS];
     __shared__ cd s_T[LU_NB * LU_NB];
     const int tid = threadIdx.x;
@@ -754,7 +757,8 @@ DEV void wg_lu_solve(int n, const cd* LU, int lda, const int* piv, cd* B, int ld
         for (int i = tid; i < n; i += NT) s_perm[i] = piv[i];
         __syncthreads();
         if (tid == 0) {
-            // the exchanges applied in order to the identity: cur[i] = original row now at row i
+            // the exchanges applied in order to the identity: cur[i] = original row now at row i; only
+            // the rows that end up holding another row move
             int* cur = reinterpret_cast<int*>(s_stage);
             for (int i = 0; i < n; ++i) cur[i] = i;
             for (int k = 0; k < n; ++k) {
@@ -765,16 +769,24 @@ DEV void wg_lu_solve(int n, const cd* LU, int lda, const int* piv, cd* B, int ld
                     cur[p] = t;
                 }
             }
-            for (int i = 0; i < n; ++i) s_perm[i] = cur[i];
+            int nm = 0;
+            for (int i = 0; i < n; ++i) {
+                s_perm[i] = cur[i];
+                if (cur[i] != i) s_moved[nm++] = i;
+            }
+            s_nmoved = nm;
         }
         __syncthreads();
-        const int cc = SOLVE_STAGE_WORDS / n;
-        for (int c0 = 0; c0 < nrhs; c0 += cc) {
-            const int w = min(cc, nrhs - c0) * n;
-            for (int e = tid; e < w; e += NT) s_stage[e] = B[s_perm[e % n] + (long)(c0 + e / n) * ldb];
-            __syncthreads();
-            for (int e = tid; e < w; e += NT) B[e % n + (long)(c0 + e / n) * ldb] = s_stage[e];
-            __syncthreads();
+        const int nm = s_nmoved;
+        if (nm > 0) {
+            const int cc = SOLVE_STAGE_WORDS / nm;
+            for (int c0 = 0; c0 < nrhs; c0 += cc) {
+                const int w = min(cc, nrhs - c0) * nm;
+                wg_map(w, [&](int e) { return ldc(B + s_perm[s_moved[e % nm]] + (long)(c0 + e / nm) * ldb); },
+                       [&](int e, cd v) { s_stage[e] = v; });
+                wg_map(w, [&](int e) { return s_stage[e]; },
+                       [&](int e, cd v) { stc(B + s_moved[e % nm] + (long)(c0 + e / nm) * ldb, v); });
+            }
         }
     } else {
         for (int c = tid; c < nrhs; c += NT) {
