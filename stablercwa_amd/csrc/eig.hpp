@@ -370,6 +370,63 @@ DEV void eig_flush(int n, cd* __restrict__ HR, cd* __restrict__ Z, int nq, const
     __syncthreads();
 }
 
+// Householder reflector applied to columns j0..n-1 of M (rows r0 .. r0+len-1, column-major, leading
+// dimension ld): d_j = sum_k conj(u_k) M(r0+k, j), then M(r0+k, j) += t(d_j) u_k.  One thread per column,
+// so no reduction crosses lanes; each pass loads eight entries at a time (the column is back in the
+// caches for the second pass).  u is in LDS.
+template <class TFun>
+DEV void reflect_columns(cd* M, long ld, int r0, int len, int j0, int n, const cd* u, TFun tfun) {
+    constexpr int B = 8;
+    for (int j = j0 + threadIdx.x; j < n; j += NT) {
+        cd* a = M + r0 + (long)j * ld;
+        // B partial sums (entry k goes to sum k mod B), added pairwise at the end: about as accurate as
+        // a tree reduction over the column, and B independent chains
+        double pr[B], pi[B];
+#pragma unroll
+        for (int b = 0; b < B; ++b) pr[b] = pi[b] = 0.0;
+        int k = 0;
+        for (; k + B <= len; k += B) {
+            cd av[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) av[b] = ldc(a + k + b);
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const cd uk = u[k + b];
+                pr[b] += uk.x * av[b].x + uk.y * av[b].y;
+                pi[b] += uk.x * av[b].y - uk.y * av[b].x;
+            }
+        }
+        for (int b = 0; k < len; ++k, ++b) {
+            const cd uk = u[k], ak = ldc(a + k);
+            // the tail (fewer than B entries) continues the same B sums
+#pragma unroll
+            for (int q = 0; q < B; ++q)
+                if (q == b) {
+                    pr[q] += uk.x * ak.x + uk.y * ak.y;
+                    pi[q] += uk.x * ak.y - uk.y * ak.x;
+                }
+        }
+#pragma unroll
+        for (int w = B / 2; w > 0; w >>= 1)
+#pragma unroll
+            for (int b = 0; b < w; ++b) {
+                pr[b] += pr[b + w];
+                pi[b] += pi[b + w];
+            }
+        const cd t = tfun(pr[0], pi[0]);
+        k = 0;
+        for (; k + B <= len; k += B) {
+            cd av[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) av[b] = ldc(a + k + b);
+#pragma unroll
+            for (int b = 0; b < B; ++b) stc(a + k + b, av[b] + t * u[k + b]);
+        }
+        for (; k < len; ++k) stc(a + k, ldc(a + k) + t * u[k]);
+    }
+    __syncthreads();
+}
+
 // Eigen-decompose the n x n matrix A (column-major, leading dimension n) in place.
 //   in : A = the matrix (overwritten)
 //   out: A = right eigenvectors (unit 2-norm columns) when info == 0; w = eigenvalues
@@ -386,6 +443,7 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
     cd* H = A;
     cd* ort = s.ort;
     auto h = [&](int i, int j) -> cd& { return H[i + (long)j * n]; };
+    __shared__ cd s_u[64 * QC];     // the current Householder vector
 
     // ---- 1. Householder reduction to Hessenberg form (CORTH)
     for (int m = 1; m < n - 1; ++m) {
@@ -421,28 +479,35 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
             if (tid == 0) ort[m] = om * (1.0 + g);
         }
         __syncthreads();
-        // left: columns m..n-1, one wave per column
+        // the reflector in LDS for both sides
         const int len = n - m;
-        for (int j = m + wave; j < n; j += NWAVE) {
-            cd* aj = H + (long)j * n;
-            double sr = 0.0, si = 0.0;
-            for (int i = lane; i < len; i += 64) {
-                const cd u = ort[m + i], a = aj[m + i];
-                sr += u.x * a.x + u.y * a.y;
-                si += u.x * a.y - u.y * a.x;
-            }
-            sr = wave_sum(sr);
-            si = wave_sum(si);
-            const cd t = cd{-sr / hh, -si / hh};
-            for (int i = lane; i < len; i += 64) aj[m + i] = aj[m + i] + t * ort[m + i];
-        }
+        for (int i = tid; i < len; i += NT) s_u[i] = ort[m + i];
         __syncthreads();
-        // right: rows 0..n-1, one thread per row
+        // left: columns m..n-1, one thread per column
+        reflect_columns(H, n, m, len, m, n, s_u, [&](double sr, double si) { return cd{-sr / hh, -si / hh}; });
+        // right: rows 0..n-1, one thread per row (entries of a row are n apart, coalesced across rows)
         for (int i = tid; i < n; i += NT) {
+            constexpr int B = 8;
             cd acc{0.0, 0.0};
-            for (int j = n - 1; j >= m; --j) acc = acc + ort[j] * h(i, j);
+            int j = n - 1;
+            for (; j - B + 1 >= m; j -= B) {
+                cd hv[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) hv[b] = ldc(H + i + (long)(j - b) * n);
+#pragma unroll
+                for (int b = 0; b < B; ++b) acc = acc + s_u[j - b - m] * hv[b];
+            }
+            for (; j >= m; --j) acc = acc + s_u[j - m] * ldc(H + i + (long)j * n);
             acc = cdiv(acc, hh);
-            for (int j = m; j < n; ++j) h(i, j) = h(i, j) - conj(ort[j]) * acc;
+            j = m;
+            for (; j + B <= n; j += B) {
+                cd hv[B];
+#pragma unroll
+                for (int b = 0; b < B; ++b) hv[b] = ldc(H + i + (long)(j + b) * n);
+#pragma unroll
+                for (int b = 0; b < B; ++b) stc(H + i + (long)(j + b) * n, hv[b] - conj(s_u[j + b - m]) * acc);
+            }
+            for (; j < n; ++j) stc(H + i + (long)j * n, ldc(H + i + (long)j * n) - conj(s_u[j - m]) * acc);
         }
         __syncthreads();
         if (tid == 0) {
@@ -454,7 +519,6 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
 
     // ---- 2. Z = P_1 ... P_{n-2}
     wg_identity(n, Z, n);
-    cd* v = s.tmp;
     for (int i = n - 2; i >= 1; --i) {
         const cd oi = ort[i];
         const cd sub0 = h(i, i - 1);
@@ -464,22 +528,9 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
         const cd v0 = scale2(oi, -e);
         const double minus_h = sub.x * v0.x + sub.y * v0.y;
         const int len = n - i;
-        for (int k = tid; k < len; k += NT) v[k] = k == 0 ? v0 : scale2(h(i + k, i - 1), -e);
+        for (int k = tid; k < len; k += NT) s_u[k] = k == 0 ? v0 : scale2(h(i + k, i - 1), -e);
         __syncthreads();
-        for (int j = i + wave; j < n; j += NWAVE) {
-            cd* zj = Z + (long)j * n + i;
-            double sr = 0.0, si = 0.0;
-            for (int k = lane; k < len; k += 64) {
-                const cd u = v[k], a = zj[k];
-                sr += u.x * a.x + u.y * a.y;
-                si += u.x * a.y - u.y * a.x;
-            }
-            sr = wave_sum(sr);
-            si = wave_sum(si);
-            const cd t = cd{sr / minus_h, si / minus_h};
-            for (int k = lane; k < len; k += 64) zj[k] = zj[k] + t * v[k];
-        }
-        __syncthreads();
+        reflect_columns(Z, n, i, len, i, n, s_u, [&](double sr, double si) { return cd{sr / minus_h, si / minus_h}; });
     }
 
     // ---- 3. clear below the sub-diagonal; make the sub-diagonal real (D^H H D, Z D)
