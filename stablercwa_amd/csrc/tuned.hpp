@@ -1,5 +1,7 @@
-// Compile-time tuning constants of the eigen kernels (override with -D at build time; the values
-// here are the ones measured best on MI355X by stablercwa_amd/tune.py).
+// Compile-time tuning constants of the eigen and S-matrix kernels (override with -D at build time).  The
+// values here are the ones measured best on MI355X: ring depth and eigen occupancy by
+// stablercwa_amd/tune.py, the queue depth and the conical / crossed / S-matrix occupancies by A/B builds
+// of the same constants on the benchmark batches.
 #pragma once
 
 #ifndef SRCWA_EIG_RING
