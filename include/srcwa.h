@@ -124,6 +124,12 @@ int srcwa_plan2d_bytes(srcwa_context* ctx, int n_stacks, const double* stacks, c
  * only (SRCWA_EUNSUPPORTED otherwise).  A replayed plan reports only the total in srcwa_plan_stage_ms
  * (the stage entries are -1). */
 int srcwa_plan_set_graph(srcwa_plan* plan, int enable);
+/* Iteration limit of every layer eigensolve of the plan: QR sweeps per eigenvalue before the solve
+ * reports eig_info = the index (1-based) of the eigenvalue that did not converge, with NaN results
+ * (EigOptions::max_iter, eig_types.hpp; default 30, as the reference).  A captured graph is captured
+ * again.  Lets a caller reproduce the reference's non-convergence contract
+ * (tests/test_rcwa1d.cpp:1169-1181). */
+int srcwa_plan_set_max_iter(srcwa_plan* plan, int max_iter);
 /* Launch all chunks of the plan on the context stream (asynchronous). */
 int srcwa_plan_execute(srcwa_plan* plan);
 /* 1 when the last execute has finished, 0 while it runs, a negative code on a device error. */

@@ -121,6 +121,24 @@ int guarded(F&& f) {
 
 }  // namespace
 
+namespace {
+// Options of the *_ex solves: max_iter != 30 routes every layer eigensolve through eig_qr with that
+// iteration limit (the reference tests' own idiom for an unconverged eigensolve, tests/test_rcwa1d.cpp:1169-1181,
+// tests/test_rcwa1d_conical.cpp:554-563); diag = 1 sets diagnostics.  counts = {eig_info, n_eig, n_dense_solve,
+// n_schur}; metrics = {max_eig_residual, max_solve_backward_error}.
+template <class Opt>
+void set_ex_options(Opt& opt, int max_iter, int diag) {
+    opt.diagnostics = diag != 0;
+    if (max_iter != 30)
+        opt.custom_eig = [max_iter](const stablercwa::Matrix& a) {
+            stablercwa::EigOptions e;
+            e.max_iter = max_iter;
+            return stablercwa::eig_qr(a, e);
+        };
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* ref_last_error() { return g_error.c_str(); }
@@ -200,6 +218,81 @@ int ref_solve2d(const double* stack, double wavelength, double theta_deg, double
         put(Tp, r.Tp);
         *status = static_cast<int>(r.status);
         *eig_info = r.eig_info;
+    });
+}
+
+int ref_solve1d_ex(const double* stack, double wavelength, double theta_deg, int M, int pol, int max_iter, int diag,
+                   double* R, double* T, double* DEr, double* DEt, int* counts, double* metrics) {
+    return guarded([&] {
+        rcwa::Options opt;
+        set_ex_options(opt, max_iter, diag);
+        const rcwa::Result r = rcwa::solve(decode1d(stack), wavelength, theta_deg, M,
+                                           pol ? rcwa::Polarization::TM : rcwa::Polarization::TE, opt);
+        put(R, r.R);
+        put(T, r.T);
+        std::memcpy(DEr, r.DEr.data(), r.DEr.size() * sizeof(double));
+        std::memcpy(DEt, r.DEt.data(), r.DEt.size() * sizeof(double));
+        counts[0] = r.eig_info;
+        counts[1] = r.n_eig;
+        counts[2] = r.n_dense_solve;
+        counts[3] = 0;
+        metrics[0] = r.max_eig_residual;
+        metrics[1] = r.max_solve_backward_error;
+    });
+}
+
+int ref_solve_conical_ex(const double* stack, double wavelength, double theta_deg, double phi_deg, int M,
+                         int max_iter, int diag, double* Jr, double* Jt, double* DEr, double* DEt, int* counts,
+                         double* metrics) {
+    return guarded([&] {
+        rcwa::Options opt;
+        set_ex_options(opt, max_iter, diag);
+        const rcwa::ConicalResult r = rcwa::solve_conical(decode1d(stack), wavelength, theta_deg, phi_deg, M, opt);
+        const std::size_t n = r.Jr.size();
+        for (std::size_t j = 0; j < n; ++j) {
+            const cplx a[4] = {r.Jr[j].ss, r.Jr[j].sp, r.Jr[j].ps, r.Jr[j].pp};
+            const cplx b[4] = {r.Jt[j].ss, r.Jt[j].sp, r.Jt[j].ps, r.Jt[j].pp};
+            for (int c = 0; c < 4; ++c) {
+                Jr[8 * j + 2 * c] = a[c].real();
+                Jr[8 * j + 2 * c + 1] = a[c].imag();
+                Jt[8 * j + 2 * c] = b[c].real();
+                Jt[8 * j + 2 * c + 1] = b[c].imag();
+            }
+            const double er[4] = {r.DEr[j].ss, r.DEr[j].sp, r.DEr[j].ps, r.DEr[j].pp};
+            const double et[4] = {r.DEt[j].ss, r.DEt[j].sp, r.DEt[j].ps, r.DEt[j].pp};
+            std::memcpy(DEr + 4 * j, er, sizeof er);
+            std::memcpy(DEt + 4 * j, et, sizeof et);
+        }
+        counts[0] = r.eig_info;
+        counts[1] = r.n_eig;
+        counts[2] = r.n_dense_solve;
+        counts[3] = r.n_schur;
+        metrics[0] = r.max_eig_residual;
+        metrics[1] = r.max_solve_backward_error;
+    });
+}
+
+int ref_solve2d_ex(const double* stack, double wavelength, double theta_deg, double phi_deg, int M1, int M2,
+                   const double* jones, int max_iter, int diag, double* DEr, double* DEt, double* Rs, double* Rp,
+                   double* Ts, double* Tp, int* status, int* counts, double* metrics) {
+    return guarded([&] {
+        rcwa::Options2D opt;
+        set_ex_options(opt, max_iter, diag);
+        const rcwa::Jones2D j{cplx{jones[0], jones[1]}, cplx{jones[2], jones[3]}};
+        const rcwa::Result2D r = rcwa::solve2d(decode2d(stack), wavelength, theta_deg, phi_deg, M1, M2, j, opt);
+        std::memcpy(DEr, r.DEr.data(), r.DEr.size() * sizeof(double));
+        std::memcpy(DEt, r.DEt.data(), r.DEt.size() * sizeof(double));
+        put(Rs, r.Rs);
+        put(Rp, r.Rp);
+        put(Ts, r.Ts);
+        put(Tp, r.Tp);
+        *status = static_cast<int>(r.status);
+        counts[0] = r.eig_info;
+        counts[1] = r.n_eig;
+        counts[2] = r.n_dense_solve;
+        counts[3] = 0;
+        metrics[0] = r.max_eig_residual;
+        metrics[1] = 0.0;
     });
 }
 

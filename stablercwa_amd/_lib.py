@@ -42,6 +42,7 @@ PROTOTYPES = {
                                           _vp, _vp, _vp, ctypes.c_int64, _vp]),
     "srcwa_plan_execute": (ctypes.c_int, [_vp]),
     "srcwa_plan_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "srcwa_plan_set_max_iter": (ctypes.c_int, [_vp, ctypes.c_int]),
     "srcwa_plan_poll": (ctypes.c_int, [_vp]),
     "srcwa_pool_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.POINTER(_vp)]),
     "srcwa_pool_destroy": (ctypes.c_int, [_vp]),

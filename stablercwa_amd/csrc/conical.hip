@@ -103,6 +103,7 @@ DEV LayerView layer_at(const double* st, int index) {
 }
 
 // ------------------------------------------------------------------------------------------------
+#if SRCWA_MAIN_TU
 __global__ __launch_bounds__(NT) void k_setup_conical(const Inst1D* __restrict__ insts, const double* __restrict__ stacks,
                                                       const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                                       double* __restrict__ geometry) {
@@ -162,8 +163,10 @@ __global__ __launch_bounds__(NT) void k_setup_conical(const Inst1D* __restrict__
     }
     __syncthreads();
 }
+#endif
 
 // ------------------------------------------------------------------------------------------------
+#if SRCWA_MAIN_TU
 __global__ __launch_bounds__(NT) void k_assemble_conical(const Inst1D* __restrict__ insts,
                                                          const LayerTask* __restrict__ tasks,
                                                          const double* __restrict__ stacks,
@@ -213,6 +216,8 @@ __global__ __launch_bounds__(NT) void k_assemble_conical(const Inst1D* __restric
     bad = block_max_int(bad);
     if (threadIdx.x == 0) task_info[4 * blockIdx.x] = (fail || bad) ? -1 : 0;
 }
+#endif
+
 
 // ------------------------------------------------------------------------------------------------
 template <int QC>
@@ -555,7 +560,26 @@ __global__ __launch_bounds__(NT) void k_modes_conical(const Inst1D* __restrict__
     if (threadIdx.x == 0) task_info[4 * t + 3] = bad ? -1 : 0;
 }
 
+// QcLaunch::what: 0 = k_eig_conical, 1 = k_eig_conical_wave, 2 = k_modes_conical
+SRCWA_QC_DISPATCH(conical_qc, QcLaunch)
+#ifdef SRCWA_QC_ONLY
+void SRCWA_QC_FN(conical_qc)(const QcLaunch& a) {
+    constexpr int QC = SRCWA_QC_ONLY;
+    const Inst1D* insts = static_cast<const Inst1D*>(a.insts);
+    if (a.what == 0) {
+        k_eig_conical<QC><<<a.grid, NT, 0, a.stream>>>(insts, a.tasks, a.layouts, a.ws, a.task_info, a.max_iter);
+    } else if (a.what == 1) {
+        if constexpr (QC <= 4)
+            k_eig_conical_wave<QC><<<a.grid, NT, 0, a.stream>>>(a.n_items, insts, a.tasks, a.layouts, a.ws,
+                                                                a.task_info, a.max_iter);
+    } else {
+        k_modes_conical<QC><<<a.grid, NT, 0, a.stream>>>(insts, a.tasks, a.stacks, a.layouts, a.ws, a.task_info);
+    }
+}
+#endif
+
 // ------------------------------------------------------------------------------------------------
+#if SRCWA_IN_PART(1)
 __global__ SMAT_KERNEL_BOUNDS void k_smatrix_conical(const Inst1D* __restrict__ insts,
                                                         const double* __restrict__ stacks,
                                                         const int64_t* __restrict__ layouts, cd* __restrict__ ws,
@@ -673,30 +697,58 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix_conical(const Inst1D* __restrict__ 
         out.eig_info[blockIdx.x] = eig_info;
     }
 }
+void smatrix_conical_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out) {
+    k_smatrix_conical<<<grid, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
+}
+#endif
 
 // ------------------------------------------------------------------------------------------------
+// the S-matrix kernel is compiled in its own unit
+void smatrix_conical_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out);
+
+#if SRCWA_MAIN_TU
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
                     int part, hipEvent_t* ev) {
     // part 1: setup, assembly, eigensolves (events ev[0..2]); part 2: modes, S-matrix, efficiencies
     // (events ev[3], ev[4]); the host may run the parts of different chunks on two streams
+    QcLaunch a{};
+    a.stream = s;
+    a.insts = insts;
+    a.tasks = tasks;
+    a.stacks = stacks;
+    a.layouts = layouts;
+    a.ws = ws;
+    a.task_info = task_info;
+    a.max_iter = max_iter;
     if (part & 1) {
         if (ev) hipEventRecord(ev[0], s);
         k_setup_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);
         if (n_tasks > 0) k_assemble_conical<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
         if (ev) hipEventRecord(ev[1], s);
         // each matrix goes to the wave or the workgroup eigensolver by its own order
-        if (n_tasks > 0 && min_n <= WAVE_EIG_MAX_N)
-            EIG_WAVE_LAUNCH(WAVE_EIG_MAX_N, k_eig_conical_wave, 2 * n_tasks, s, insts, tasks, layouts, ws, task_info,
-                            max_iter);
-        if (n_tasks > 0 && max_n > WAVE_EIG_MAX_N)
-            EIG_LAUNCH(max_n, k_eig_conical, 2 * n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
+        if (n_tasks > 0 && min_n <= WAVE_EIG_MAX_N) {
+            a.what = 1;
+            a.n_items = 2 * n_tasks;
+            a.grid = (2 * n_tasks + NWAVE - 1) / NWAVE;
+            conical_qc(wave_columns_per_lane(WAVE_EIG_MAX_N), a);
+        }
+        if (n_tasks > 0 && max_n > WAVE_EIG_MAX_N) {
+            a.what = 0;
+            a.grid = 2 * n_tasks;
+            conical_qc(eig_columns_per_thread(max_n), a);
+        }
         if (ev) hipEventRecord(ev[2], s);
     }
     if (part & 2) {
         if (ev) hipEventRecord(ev[3], s);
-        if (n_tasks > 0) EIG_LAUNCH(2 * max_n, k_modes_conical, n_tasks, s, insts, tasks, stacks, layouts, ws, task_info);
-        k_smatrix_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
+        if (n_tasks > 0) {
+            a.what = 2;
+            a.grid = n_tasks;
+            conical_qc(eig_columns_per_thread(2 * max_n), a);
+        }
+        smatrix_conical_launch(n_inst, s, insts, stacks, layouts, ws, task_info, out);
         if (ev) hipEventRecord(ev[4], s);
     }
 }
+#endif

@@ -3,11 +3,14 @@
 #include "batch.hpp"
 #include "eig.hpp"
 
+#if SRCWA_MAIN_TU
 __global__ __launch_bounds__(NT) void k_zgemm_batched(const DenseItem* __restrict__ items, cd* arena) {
     const DenseItem it = items[blockIdx.x];
     wg_gemm<GEMM_SET, false>(it.n, it.nrhs, it.k, arena + it.a, it.n, arena + it.b, it.k, arena + it.c, it.n);
 }
+#endif
 
+#if SRCWA_MAIN_TU
 // A is overwritten by its LU factors, B by the solution; scratch holds the pivots.
 __global__ __launch_bounds__(NT) void k_zgesv_batched(const DenseItem* __restrict__ items, cd* arena, int* info) {
     const DenseItem it = items[blockIdx.x];
@@ -16,6 +19,8 @@ __global__ __launch_bounds__(NT) void k_zgesv_batched(const DenseItem* __restric
     if (!sing) wg_lu_solve(it.n, arena + it.a, it.n, piv, arena + it.b, it.n, it.nrhs);
     if (threadIdx.x == 0) info[blockIdx.x] = sing;
 }
+#endif
+
 
 // A is overwritten by the eigenvectors; scratch: Z, X (n x n each), ort, tmp (n each).
 template <int QC>
@@ -30,6 +35,14 @@ __global__ __launch_bounds__(NT) void k_zgeev_batched(const DenseItem* __restric
     if (threadIdx.x == 0) info[blockIdx.x] = r;
 }
 
+SRCWA_QC_DISPATCH(dense_qc, QcLaunch)
+#ifdef SRCWA_QC_ONLY
+void SRCWA_QC_FN(dense_qc)(const QcLaunch& a) {
+    k_zgeev_batched<SRCWA_QC_ONLY><<<a.grid, NT, 0, a.stream>>>(a.items, a.ws, a.info, a.max_iter);
+}
+#endif
+
+#if SRCWA_MAIN_TU
 void launch_zgemm_batched(hipStream_t s, int count, const DenseItem* items, cd* arena) {
     k_zgemm_batched<<<count, NT, 0, s>>>(items, arena);
 }
@@ -38,5 +51,13 @@ void launch_zgesv_batched(hipStream_t s, int count, const DenseItem* items, cd* 
 }
 void launch_zgeev_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info, int max_iter,
                           int max_n) {
-    EIG_LAUNCH(max_n, k_zgeev_batched, count, s, items, arena, info, max_iter);
+    QcLaunch a{};
+    a.grid = count;
+    a.stream = s;
+    a.items = items;
+    a.ws = arena;
+    a.info = info;
+    a.max_iter = max_iter;
+    dense_qc(eig_columns_per_thread(max_n), a);
 }
+#endif

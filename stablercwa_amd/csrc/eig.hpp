@@ -37,17 +37,59 @@ constexpr int EIG_RING = SRCWA_EIG_RING;    // rows each column prefetches ahead
 // for 1, 2, 4, 8 and 16 so small matrices do not pay the registers of the largest.
 inline int eig_columns_per_thread(int n) { return n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 512 ? 8 : 16; }
 
-// Launches KERNEL<QC> for the QC that matrices of order up to max_n need.
-#define EIG_LAUNCH(max_n, KERNEL, GRID, STREAM, ...)                                      \
-    do {                                                                                  \
-        switch (eig_columns_per_thread(max_n)) {                                          \
-            case 1: KERNEL<1><<<(GRID), NT, 0, (STREAM)>>>(__VA_ARGS__); break;           \
-            case 2: KERNEL<2><<<(GRID), NT, 0, (STREAM)>>>(__VA_ARGS__); break;           \
-            case 4: KERNEL<4><<<(GRID), NT, 0, (STREAM)>>>(__VA_ARGS__); break;           \
-            case 8: KERNEL<8><<<(GRID), NT, 0, (STREAM)>>>(__VA_ARGS__); break;           \
-            default: KERNEL<16><<<(GRID), NT, 0, (STREAM)>>>(__VA_ARGS__); break;         \
+// Translation-unit split.  The QC-templated kernels of a family dominate its compile time, so each
+// family source is compiled once without SRCWA_QC_ONLY (its other kernels and its host launcher, the
+// "main" unit) and once per QC in {1, 2, 4, 8, 16} with SRCWA_QC_ONLY = QC (that QC's template kernels
+// only); the units build in parallel.  The main unit reaches the per-QC units through plain host
+// functions NAME_q1 .. NAME_q16 that take one argument struct, declared by SRCWA_QC_DISPATCH together
+// with the dispatcher NAME(qc, args).
+// The heaviest non-template kernels (the S-matrix recursions) likewise sit in a unit of their own,
+// "part 1" (-DSRCWA_PART_ONLY=1), reached through a host function.
+#define SRCWA_CAT2(a, b) a##b
+#define SRCWA_CAT(a, b) SRCWA_CAT2(a, b)
+#if defined(SRCWA_QC_ONLY)
+#define SRCWA_UNIT_PART (-1)
+#define SRCWA_QC_FN(NAME) SRCWA_CAT(NAME##_q, SRCWA_QC_ONLY)
+#elif defined(SRCWA_PART_ONLY)
+#define SRCWA_UNIT_PART SRCWA_PART_ONLY
+#else
+#define SRCWA_UNIT_PART 0
+#endif
+#define SRCWA_MAIN_TU (SRCWA_UNIT_PART == 0)
+#define SRCWA_IN_PART(k) (SRCWA_UNIT_PART == (k))
+#define SRCWA_QC_DISPATCH(NAME, ARGS)                                                     \
+    void NAME##_q1(const ARGS&);                                                          \
+    void NAME##_q2(const ARGS&);                                                          \
+    void NAME##_q4(const ARGS&);                                                          \
+    void NAME##_q8(const ARGS&);                                                          \
+    void NAME##_q16(const ARGS&);                                                         \
+    inline void NAME(int qc, const ARGS& a) {                                             \
+        switch (qc) {                                                                     \
+            case 1: NAME##_q1(a); break;                                                  \
+            case 2: NAME##_q2(a); break;                                                  \
+            case 4: NAME##_q4(a); break;                                                  \
+            case 8: NAME##_q8(a); break;                                                  \
+            default: NAME##_q16(a); break;                                                \
         }                                                                                 \
-    } while (0)
+    }
+
+// Arguments of a family's QC-templated launches (which kernel, grid, stream and the kernel arguments;
+// each family uses the fields its kernels take).
+struct QcLaunch {
+    int what;                 // family-specific kernel selector
+    int grid;
+    hipStream_t stream;
+    int n_items;
+    const void* insts;        // Inst1D* or Inst2D*
+    const struct LayerTask* tasks;
+    const double* stacks;
+    const int64_t* layouts;
+    cd* ws;
+    int* task_info;
+    int max_iter;
+    const struct DenseItem* items;
+    int* info;
+};
 
 struct EigScratch {
     cd* ort;     // n
@@ -70,6 +112,12 @@ DEV double qr_rotation(double xr, double xi, double sub, double& cr, double& ci,
         sub = ldexp(sub, -e);
     }
     const double ss = fma(sub, sub, fma(xi, xi, xr * xr));
+#if SRCWA_QR_EXACT_ROTATION
+    const double nrm = sqrt(ss);
+    cr = xr / nrm;
+    ci = xi / nrm;
+    sn = sub / nrm;
+#else
     double inv = __builtin_amdgcn_rsq(ss);
     inv = fma(0.5 * inv, fma(-ss * inv, inv, 1.0), inv);
     inv = fma(0.5 * inv, fma(-ss * inv, inv, 1.0), inv);
@@ -77,6 +125,7 @@ DEV double qr_rotation(double xr, double xi, double sub, double& cr, double& ci,
     cr = xr * inv;
     ci = xi * inv;
     sn = sub * inv;
+#endif
     return e ? ldexp(nrm, e) : nrm;
 }
 

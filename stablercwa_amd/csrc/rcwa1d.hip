@@ -17,6 +17,7 @@
 #include "rcwa_common.hpp"
 
 // ------------------------------------------------------------------------------------------------
+#if SRCWA_MAIN_TU
 __global__ __launch_bounds__(NT) void k_setup1d(const Inst1D* __restrict__ insts, const double* __restrict__ stacks,
                                                 const int64_t* __restrict__ layouts, cd* __restrict__ ws) {
     const Inst1D in = insts[blockIdx.x];
@@ -39,8 +40,10 @@ __global__ __launch_bounds__(NT) void k_setup1d(const Inst1D* __restrict__ insts
     }
     __syncthreads();
 }
+#endif
 
 // ------------------------------------------------------------------------------------------------
+#if SRCWA_MAIN_TU
 __global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                                    const double* __restrict__ stacks, const int64_t* __restrict__ layouts,
                                                    cd* __restrict__ ws, int* __restrict__ task_info) {
@@ -100,6 +103,8 @@ __global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ in
     bad = block_max_int(bad);
     if (threadIdx.x == 0) task_info[blockIdx.x] = (fail || bad) ? -1 : 0;
 }
+#endif
+
 
 // ------------------------------------------------------------------------------------------------
 template <int QC>
@@ -144,7 +149,22 @@ __global__ __launch_bounds__(NT) void k_eig1d_wave(int n_tasks, const Inst1D* __
     if (lane_id() == 0) task_info[t] = info;
 }
 
+// QcLaunch::what: 0 = k_eig1d (workgroup per matrix), 1 = k_eig1d_wave (wave per matrix)
+SRCWA_QC_DISPATCH(planar_qc, QcLaunch)
+#ifdef SRCWA_QC_ONLY
+void SRCWA_QC_FN(planar_qc)(const QcLaunch& a) {
+    constexpr int QC = SRCWA_QC_ONLY;
+    const Inst1D* insts = static_cast<const Inst1D*>(a.insts);
+    if (a.what == 0)
+        k_eig1d<QC><<<a.grid, NT, 0, a.stream>>>(insts, a.tasks, a.layouts, a.ws, a.task_info, a.max_iter);
+    else if constexpr (QC <= 4)
+        k_eig1d_wave<QC><<<a.grid, NT, 0, a.stream>>>(a.n_items, insts, a.tasks, a.layouts, a.ws, a.task_info,
+                                                      a.max_iter);
+}
+#endif
+
 // ------------------------------------------------------------------------------------------------
+#if SRCWA_MAIN_TU
 __global__ __launch_bounds__(NT) void k_modes1d(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                                 const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                                 const int* __restrict__ task_info) {
@@ -172,8 +192,10 @@ __global__ __launch_bounds__(NT) void k_modes1d(const Inst1D* __restrict__ insts
         wg_scale_cols(N, N, V, N, basis, V, N);
     }
 }
+#endif
 
 // ------------------------------------------------------------------------------------------------
+#if SRCWA_IN_PART(1)
 __global__ SMAT_KERNEL_BOUNDS void k_smatrix1d(const Inst1D* __restrict__ insts, const double* __restrict__ stacks,
                                                   const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                                   const int* __restrict__ task_info, Out1D out) {
@@ -282,8 +304,16 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix1d(const Inst1D* __restrict__ insts,
         out.eig_info[blockIdx.x] = eig_info;
     }
 }
+void smatrix1d_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out) {
+    k_smatrix1d<<<grid, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
+}
+#endif
 
 // ------------------------------------------------------------------------------------------------
+// the S-matrix kernel is compiled in its own unit
+void smatrix1d_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out);
+
+#if SRCWA_MAIN_TU
 // host launchers
 void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
@@ -297,16 +327,32 @@ void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, 
         if (ev) hipEventRecord(ev[1], s);
         // each matrix goes to the wave or the workgroup eigensolver by its own order, so a result does not
         // depend on what else is in the batch
-        if (n_tasks > 0 && min_n <= WAVE_EIG_MAX_N)
-            EIG_WAVE_LAUNCH(WAVE_EIG_MAX_N, k_eig1d_wave, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
-        if (n_tasks > 0 && max_n > WAVE_EIG_MAX_N)
-            EIG_LAUNCH(max_n, k_eig1d, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
+        QcLaunch a{};
+        a.stream = s;
+        a.n_items = n_tasks;
+        a.insts = insts;
+        a.tasks = tasks;
+        a.layouts = layouts;
+        a.ws = ws;
+        a.task_info = task_info;
+        a.max_iter = max_iter;
+        if (n_tasks > 0 && min_n <= WAVE_EIG_MAX_N) {
+            a.what = 1;
+            a.grid = (n_tasks + NWAVE - 1) / NWAVE;
+            planar_qc(wave_columns_per_lane(WAVE_EIG_MAX_N), a);
+        }
+        if (n_tasks > 0 && max_n > WAVE_EIG_MAX_N) {
+            a.what = 0;
+            a.grid = n_tasks;
+            planar_qc(eig_columns_per_thread(max_n), a);
+        }
         if (ev) hipEventRecord(ev[2], s);
     }
     if (part & 2) {
         if (ev) hipEventRecord(ev[3], s);
         if (n_tasks > 0) k_modes1d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
-        k_smatrix1d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
+        smatrix1d_launch(n_inst, s, insts, stacks, layouts, ws, task_info, out);
         if (ev) hipEventRecord(ev[4], s);
     }
 }
+#endif

@@ -90,6 +90,7 @@ DEV void homogeneous_modes2d(cd n, const cd* geo, int N, cd* q, cd* vb) {
 }
 
 // ------------------------------------------------------------------------------------------------
+#if SRCWA_MAIN_TU
 __global__ __launch_bounds__(NT) void k_setup2d(const Inst2D* __restrict__ insts, const double* __restrict__ stacks,
                                                 const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                                 double* __restrict__ geometry) {
@@ -148,6 +149,7 @@ __global__ __launch_bounds__(NT) void k_setup2d(const Inst2D* __restrict__ insts
     }
     __syncthreads();
 }
+#endif
 
 // ------------------------------------------------------------------------------------------------
 // Operators.
@@ -281,6 +283,7 @@ struct Ops2DScratch {
     const double** rows;
 };
 
+#if SRCWA_MAIN_TU
 __global__ __launch_bounds__(NT) void k_ops2d(const Inst2D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                               const double* __restrict__ stacks, const int64_t* __restrict__ layouts,
                                               cd* __restrict__ ws, int* __restrict__ task_info) {
@@ -419,6 +422,7 @@ __global__ __launch_bounds__(NT) void k_ops2d(const Inst2D* __restrict__ insts, 
     bad = block_max_int(bad) | block_max_int(fail);
     if (threadIdx.x == 0) task_info[4 * blockIdx.x] = bad ? -1 : 0;
 }
+#endif
 
 // ------------------------------------------------------------------------------------------------
 struct Slots2D {
@@ -443,6 +447,7 @@ DEV Slots2D slots2d(cd* ws, const int64_t* lay, int L, int j, int N) {
     return s;
 }
 
+#if SRCWA_MAIN_TU
 // Ei = Eps^-1, P = [pc1 | pc2], Q12 = Cy - Kx^2, Q21 = Ky^2 - Cx, H = P Q (rcwa2d.hpp:412-474)
 __global__ __launch_bounds__(NT) void k_pq2d(const Inst2D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                              const int64_t* __restrict__ layouts, cd* __restrict__ ws,
@@ -505,6 +510,8 @@ __global__ __launch_bounds__(NT) void k_pq2d(const Inst2D* __restrict__ insts, c
     bad = block_max_int(bad) | fail;
     if (threadIdx.x == 0) task_info[4 * blockIdx.x] = bad ? -1 : 0;
 }
+#endif
+
 
 template <int QC>
 __global__ EIG_KERNEL_BOUNDS_OF(SRCWA_EIG_OCC_CROSSED) void k_eig2d(const Inst2D* __restrict__ insts, const LayerTask* __restrict__ tasks,
@@ -541,6 +548,21 @@ __global__ __launch_bounds__(NT) void k_eig2d_wave(int n_tasks, const Inst2D* __
     if (lane_id() == 0) task_info[4 * t + 1] = info;
 }
 
+// QcLaunch::what: 0 = k_eig2d (workgroup per matrix), 1 = k_eig2d_wave (wave per matrix)
+SRCWA_QC_DISPATCH(crossed_qc, QcLaunch)
+#ifdef SRCWA_QC_ONLY
+void SRCWA_QC_FN(crossed_qc)(const QcLaunch& a) {
+    constexpr int QC = SRCWA_QC_ONLY;
+    const Inst2D* insts = static_cast<const Inst2D*>(a.insts);
+    if (a.what == 0)
+        k_eig2d<QC><<<a.grid, NT, 0, a.stream>>>(insts, a.tasks, a.layouts, a.ws, a.task_info, a.max_iter);
+    else if constexpr (QC <= 4)
+        k_eig2d_wave<QC><<<a.grid, NT, 0, a.stream>>>(a.n_items, insts, a.tasks, a.layouts, a.ws, a.task_info,
+                                                      a.max_iter);
+}
+#endif
+
+#if SRCWA_MAIN_TU
 // q = branch(sqrt(w)), V = (Q W) / q into Z (rcwa2d.hpp:499-533)
 __global__ __launch_bounds__(NT) void k_modes2d(const Inst2D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                                 const int64_t* __restrict__ layouts, cd* __restrict__ ws,
@@ -577,6 +599,7 @@ __global__ __launch_bounds__(NT) void k_modes2d(const Inst2D* __restrict__ insts
     bad = block_max_int(bad);
     if (threadIdx.x == 0) task_info[4 * blockIdx.x + 2] = bad ? -1 : 0;
 }
+#endif
 
 // ------------------------------------------------------------------------------------------------
 // 2 x 2 block-diagonal operators (a11, a12, a21, a22 of length Nh) and the recursion state.
@@ -695,6 +718,7 @@ struct Region2D {
     cd* V;
 };
 
+#if SRCWA_IN_PART(1)
 __global__ SMAT_KERNEL_BOUNDS void k_smatrix2d(const Inst2D* __restrict__ insts, const double* __restrict__ stacks,
                                                   const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                                   const int* __restrict__ task_info, Out1D out) {
@@ -1172,8 +1196,16 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix2d(const Inst2D* __restrict__ insts,
         out.eig_info[blockIdx.x] = eig_info;
     }
 }
+void smatrix2d_launch(int grid, hipStream_t s, const Inst2D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out) {
+    k_smatrix2d<<<grid, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
+}
+#endif
 
 // ------------------------------------------------------------------------------------------------
+// the S-matrix kernel is compiled in its own unit
+void smatrix2d_launch(int grid, hipStream_t s, const Inst2D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out);
+
+#if SRCWA_MAIN_TU
 void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
                     int part, hipEvent_t* ev) {
@@ -1188,16 +1220,32 @@ void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts,
         }
         if (ev) hipEventRecord(ev[1], s);
         // each matrix goes to the wave or the workgroup eigensolver by its own order
-        if (n_tasks > 0 && 2 * min_n <= WAVE_EIG_MAX_N)
-            EIG_WAVE_LAUNCH(WAVE_EIG_MAX_N, k_eig2d_wave, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
-        if (n_tasks > 0 && 2 * max_n > WAVE_EIG_MAX_N)
-            EIG_LAUNCH(2 * max_n, k_eig2d, n_tasks, s, insts, tasks, layouts, ws, task_info, max_iter);
+        QcLaunch a{};
+        a.stream = s;
+        a.n_items = n_tasks;
+        a.insts = insts;
+        a.tasks = tasks;
+        a.layouts = layouts;
+        a.ws = ws;
+        a.task_info = task_info;
+        a.max_iter = max_iter;
+        if (n_tasks > 0 && 2 * min_n <= WAVE_EIG_MAX_N) {
+            a.what = 1;
+            a.grid = (n_tasks + NWAVE - 1) / NWAVE;
+            crossed_qc(wave_columns_per_lane(WAVE_EIG_MAX_N), a);
+        }
+        if (n_tasks > 0 && 2 * max_n > WAVE_EIG_MAX_N) {
+            a.what = 0;
+            a.grid = n_tasks;
+            crossed_qc(eig_columns_per_thread(2 * max_n), a);
+        }
         if (ev) hipEventRecord(ev[2], s);
     }
     if (part & 2) {
         if (ev) hipEventRecord(ev[3], s);
         if (n_tasks > 0) k_modes2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
-        k_smatrix2d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
+        smatrix2d_launch(n_inst, s, insts, stacks, layouts, ws, task_info, out);
         if (ev) hipEventRecord(ev[4], s);
     }
 }
+#endif

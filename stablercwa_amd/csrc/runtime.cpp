@@ -388,8 +388,16 @@ struct srcwa_plan {
     hipGraph_t graph = nullptr;      // captured launch sequence (srcwa_plan_set_graph)
     hipGraphExec_t graph_exec = nullptr;
     bool timed = false;
+    int max_iter = 30;               // QR sweeps per eigenvalue (EigOptions::max_iter, eig_types.hpp)
     double flops[2] = {0.0, 0.0};
     ~srcwa_plan() {
+        // uploads and launches still queued on the context's streams may touch the buffers below, which
+        // return to a shared pool (another plan may get them at once) when the members are destroyed
+        if (ctx) {
+            (void)hipSetDevice(ctx->device);
+            for (hipStream_t st : {ctx->stream, ctx->back})
+                if (st) (void)hipStreamSynchronize(st);
+        }
         for (std::vector<hipEvent_t>* v : {&ev, &front_done, &back_done})
             for (hipEvent_t& e : *v)
                 if (e) (void)hipEventDestroy(e);
@@ -470,14 +478,15 @@ void enqueue(srcwa_plan* plan, bool timing) {
         const int ni = ch.i1 - ch.i0, nt = ch.t1 - ch.t0;
         if (plan->kind == 1)
             launch_planar(st, ni, nt, plan->insts.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p, plan->layouts.p,
-                          ws, plan->task_info.p + ch.t0, out, 30, plan->min_n, plan->max_n, part, ev);
+                          ws, plan->task_info.p + ch.t0, out, plan->max_iter, plan->min_n, plan->max_n, part, ev);
         else if (plan->kind == 2)
             launch_conical(st, ni, nt, plan->insts.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p, plan->layouts.p,
-                           ws, plan->task_info.p + 4 * ch.t0, out, 30, plan->min_n, plan->max_n, part, ev);
+                           ws, plan->task_info.p + 4 * ch.t0, out, plan->max_iter, plan->min_n, plan->max_n, part,
+                           ev);
         else
             launch_crossed(st, ni, nt, plan->insts2d.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p,
-                           plan->layouts.p, ws, plan->task_info.p + 4 * ch.t0, out, 30, plan->min_n, plan->max_n,
-                           part, ev);
+                           plan->layouts.p, ws, plan->task_info.p + 4 * ch.t0, out, plan->max_iter, plan->min_n,
+                           plan->max_n, part, ev);
         hip_check(hipGetLastError(), "kernel launch");
     };
     for (size_t c = 0; c < nc; ++c) {
@@ -929,6 +938,20 @@ int srcwa_plan_set_graph(srcwa_plan* plan, int enable) {
         hip_check(hipStreamEndCapture(st, &plan->graph), "hipStreamEndCapture");
         hip_check(hipGraphInstantiate(&plan->graph_exec, plan->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     });
+}
+
+int srcwa_plan_set_max_iter(srcwa_plan* plan, int max_iter) {
+    if (!plan) {
+        g_error = "null plan";
+        return SRCWA_EINVAL;
+    }
+    if (max_iter < 1 || max_iter > 100000) {
+        g_error = "srcwa_plan_set_max_iter: max_iter must lie in 1..100000";
+        return SRCWA_EINVAL;
+    }
+    plan->max_iter = max_iter;
+    // a captured graph holds the old limit in its kernel arguments: capture again
+    return plan->graph_exec ? srcwa_plan_set_graph(plan, 1) : SRCWA_OK;
 }
 
 int srcwa_plan_sync(srcwa_plan* plan) {

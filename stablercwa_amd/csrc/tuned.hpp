@@ -43,6 +43,17 @@
 #define SRCWA_SMAT_OCC 2
 #endif
 
+// 1: wg_lu and wg_lu_solve are called, not inlined, at each of their call sites (one copy per kernel
+// instead of one per call site: a fraction of the compile time and code size of the S-matrix kernels)
+#ifndef SRCWA_LU_OUTLINE
+#define SRCWA_LU_OUTLINE 0
+#endif
+// 1: the QR rotation divides by a correctly rounded sqrt; 0: hardware reciprocal square root refined by
+// two Newton steps
+#ifndef SRCWA_QR_EXACT_ROTATION
+#define SRCWA_QR_EXACT_ROTATION 0
+#endif
+
 // (a minimum of one wave per SIMD is the same as no bound)
 #define EIG_KERNEL_BOUNDS_OF(occ) __launch_bounds__(NT, ((occ) > 0 ? (occ) : 1))
 #define SMAT_KERNEL_BOUNDS __launch_bounds__(NT, SRCWA_SMAT_OCC)

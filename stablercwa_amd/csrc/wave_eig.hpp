@@ -28,17 +28,6 @@ constexpr int WAVE_RING = 4;
 // Columns per lane for matrices of order up to n (wave kernels are instantiated for 1, 2 and 4).
 inline int wave_columns_per_lane(int n) { return n <= 64 ? 1 : n <= 128 ? 2 : 4; }
 
-// Launches KERNEL<QC> with one wave per item (NITEMS items, four per workgroup) when every matrix has
-// order <= WAVE_EIG_MAX_N; the kernels take the item count as their first argument.
-#define EIG_WAVE_LAUNCH(max_n, KERNEL, NITEMS, STREAM, ...)                                              \
-    do {                                                                                                 \
-        const int grid_ = ((NITEMS) + NWAVE - 1) / NWAVE;                                                \
-        switch (wave_columns_per_lane(max_n)) {                                                          \
-            case 1: KERNEL<1><<<grid_, NT, 0, (STREAM)>>>((NITEMS), __VA_ARGS__); break;                 \
-            case 2: KERNEL<2><<<grid_, NT, 0, (STREAM)>>>((NITEMS), __VA_ARGS__); break;                 \
-            default: KERNEL<4><<<grid_, NT, 0, (STREAM)>>>((NITEMS), __VA_ARGS__); break;                \
-        }                                                                                                \
-    } while (0)
 
 // Eigen-decomposition of the n x n matrix A by the calling wave (all 64 lanes must call it).
 //   QC   : columns per lane in the QR sweep (n <= 64 QC)

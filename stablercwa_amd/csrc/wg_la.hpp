@@ -15,6 +15,13 @@
 #pragma once
 
 #include "cplx.hpp"
+#include "tuned.hpp"
+
+#if SRCWA_LU_OUTLINE
+#define LU_FN __device__ __noinline__
+#else
+#define LU_FN DEV
+#endif
 
 constexpr int NT = 256;          // threads per workgroup for every wg_* routine
 constexpr int NWAVE = NT / 64;
@@ -697,7 +704,7 @@ DEV void lu_columns_partial(int n, int ks, int nb, cd* A, int lda, const LuPanel
 // rows, in the workgroup's registers for up to 2 NT),
 // then one pass over the other columns that applies the panel's row permutation and, right of the
 // panel, solves U12 = L11^{-1} A12; then the trailing MFMA update.
-DEV int wg_lu(int n, cd* A, int lda, int* piv) {
+LU_FN int wg_lu(int n, cd* A, int lda, int* piv) {
     __shared__ LuPanelShared S;
     __shared__ int s_trow[2 * LU_NB], s_tsrc[2 * LU_NB];
     const int tid = threadIdx.x;
@@ -745,7 +752,7 @@ constexpr int SOLVE_MAX_PERM = 2048;
 // Blocked by LU_NB: each diagonal block is staged in LDS and solved in registers (one right-hand side
 // per thread), the off-diagonal updates are MFMA products (VALU products for fewer than 8 right-hand
 // sides).  The substitution order within a block is that of plain forward / backward substitution.
-DEV void wg_lu_solve(int n, const cd* LU, int lda, const int* piv, cd* B, int ldb, int nrhs) {
+LU_FN void wg_lu_solve(int n, const cd* LU, int lda, const int* piv, cd* B, int ldb, int nrhs) {
     __shared__ int s_perm[SOLVE_MAX_PERM];
     __shared__ int s_moved[SOLVE_MAX_PERM];
     __shared__ int s_nmoved;

@@ -78,6 +78,10 @@ class PlanarBatch:
         """Capture the launch sequence into a HIP graph that execute() replays (one-chunk plans)."""
         _lib.check(self.lib.srcwa_plan_set_graph(self.handle, 1 if enable else 0))
 
+    def set_max_iter(self, max_iter: int) -> None:
+        """QR sweeps per eigenvalue of every layer eigensolve (EigOptions::max_iter; default 30)."""
+        _lib.check(self.lib.srcwa_plan_set_max_iter(self.handle, int(max_iter)))
+
     def sync(self) -> None:
         _lib.check(self.lib.srcwa_plan_sync(self.handle))
 

@@ -91,6 +91,27 @@ class Stack:
                 out += [float(int(layer.fourier_method)), float(layer.fft_samples)]
         return np.asarray(out, dtype=np.float64)
 
+    @staticmethod
+    def from_flat(rec) -> "Stack":
+        """Inverse of flat(): the stack a flat planar record describes."""
+        p = [float(x) for x in rec]
+        st = Stack(complex(p[1], p[2]), [], complex(p[3], p[4]), p[0])
+        k = 6
+        for _ in range(int(p[5])):
+            d, kind = p[k], int(p[k + 1])
+            k += 2
+            if kind == 0:
+                st.layers.append(Layer.uniform(d, complex(p[k], p[k + 1])))
+                k += 2
+            else:
+                K = int(p[k])
+                edges = p[k + 1:k + K + 2]
+                vals = [complex(p[k + K + 2 + 2 * i], p[k + K + 3 + 2 * i]) for i in range(K)]
+                k += 1 + (K + 1) + 2 * K
+                st.layers.append(Layer.grating(d, edges, vals, FourierMethod(int(p[k])), int(p[k + 1])))
+                k += 2
+        return st
+
 
 @dataclass
 class Result:
@@ -200,6 +221,32 @@ class Stack2D:
                         v = complex(v)
                         out += [v.real, v.imag]
         return np.asarray(out, dtype=np.float64)
+
+    @staticmethod
+    def from_flat(rec) -> "Stack2D":
+        """Inverse of flat(): the crossed-grating stack a flat record describes."""
+        p = [float(x) for x in rec]
+        st = Stack2D(p[2], complex(p[3], p[4]), [], p[0], p[1])
+        k = 6
+        for _ in range(int(p[5])):
+            d, kind = p[k], int(p[k + 1])
+            k += 2
+            if kind == 0:
+                st.layers.append(Layer2D.uniform(d, complex(p[k], p[k + 1])))
+                k += 2
+                continue
+            K = int(p[k])
+            y_edges = p[k + 1:k + K + 2]
+            k += K + 2
+            rows = []
+            for _ in range(K):
+                Kx = int(p[k])
+                xe = p[k + 1:k + Kx + 2]
+                vals = [complex(p[k + Kx + 2 + 2 * i], p[k + Kx + 3 + 2 * i]) for i in range(Kx)]
+                rows.append(XProfile2D(xe, vals))
+                k += 1 + (Kx + 1) + 2 * Kx
+            st.layers.append(Layer2D.patterned(d, y_edges, rows))
+        return st
 
 
 def jones_from_angles_2d(psi_deg: float = 90.0, delta_deg: float = 0.0):

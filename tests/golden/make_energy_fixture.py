@@ -1,7 +1,7 @@
-"""Writes tests/golden/planar_energy_ref.json: the reference solver's own energy defect |R+T-1| over
-the benchmark sweep (grating on a 20-layer mirror, M = 60, every 4th of the 4096 wavelengths, TE and
-TM), computed with oracle/_ref.  A lossless stack's defect is pure rounding; this fixes the level the
-GPU path is held to.
+"""Writes tests/golden/planar_energy_ref.json and planar_energy_ref.npz: the reference solver's own
+energy defect |R+T-1| over the benchmark sweep (grating on a 20-layer mirror, M = 60, all 4096
+wavelengths, TE and TM), computed with oracle/_ref.  A lossless stack's defect is pure rounding; this
+fixes the level the GPU path is held to, point by point (the .npz holds every point's defect).
 
     python tests/golden/make_energy_fixture.py
 """
@@ -25,17 +25,20 @@ def main():
         n = 2.1 if k % 2 == 0 else 1.46
         layers.append(Layer.uniform(1.053 / (4 * n), n))
     st = Stack(1.0, layers, 1.46, 0.574)
-    lams = np.linspace(0.95, 1.15, 4096)[::4]
+    lams = np.linspace(0.95, 1.15, 4096)
     lib = ref.load()
-    out = {}
+    out, per_point = {}, {}
     for pol, name in ((0, "TE"), (1, "TM")):
         der, det = ref.sweep1d(lib, st.flat(), lams, np.full(lams.size, 66.36), 60, pol, os.cpu_count() or 1)
         e = np.abs(der.sum(1) + det.sum(1) - 1.0)
+        per_point[name] = e
         out[name] = dict(max=float(e.max()), median=float(np.median(e)), p99=float(np.percentile(e, 99)),
                          points=int(lams.size))
     with open(os.path.join(HERE, "planar_energy_ref.json"), "w") as f:
         json.dump(dict(source="reference rcwa::sweep (oracle/_ref), grating + 20-layer mirror, M=60, theta 66.36",
                        **out), f, indent=1)
+    np.savez_compressed(os.path.join(HERE, "planar_energy_ref.npz"), wavelength=lams, TE=per_point["TE"],
+                        TM=per_point["TM"])
     print(out)
 
 

@@ -9,10 +9,12 @@
 * each result is bitwise the same alone and inside a mixed batch
 * error conventions: invalid input raises ValueError; FFT layers are rejected explicitly
 """
+import os
+
 import numpy as np
 import pytest
 
-from conftest import load_golden, max_abs_diff
+from conftest import GOLDEN, load_golden, max_abs_diff
 
 pytestmark = pytest.mark.gpu
 
@@ -107,14 +109,17 @@ def test_full_size_sweep_energy_and_sample_parity(ctx):
     der = res["DEr"].reshape(-1, N)
     det = res["DEt"].reshape(-1, N)
     rt = der.sum(axis=1) + det.sum(axis=1)
-    # lossless: |R+T-1| is rounding only.  Held to the reference solver's own defect on this sweep
-    # (planar_energy_ref.json: max 3.7e-12, median 5.3e-13 over 1024 wavelengths per polarization).
+    # lossless: |R+T-1| is rounding only.  Held to the reference solver's own defect on the same 8192
+    # solves (planar_energy_ref.npz, every point): maximum, 99th percentile and median of the GPU's
+    # defect at or below the reference's, per polarization
     e = np.abs(rt - 1.0)
-    ref = load_golden("planar_energy_ref.json")
-    ref_max = max(ref["TE"]["max"], ref["TM"]["max"])
-    ref_med = max(ref["TE"]["median"], ref["TM"]["median"])
-    assert np.max(e) <= 1.5 * ref_max, (np.max(e), ref_max)
-    assert np.median(e) <= 2.0 * ref_med, (np.median(e), ref_med)
+    z = np.load(os.path.join(GOLDEN, "planar_energy_ref.npz"), allow_pickle=False)
+    assert np.array_equal(z["wavelength"], lams)
+    for k, pol in enumerate(("TE", "TM")):
+        g, r = e[4096 * k:4096 * (k + 1)], z[pol]
+        assert np.max(g) <= np.max(r), (pol, np.max(g), np.max(r))
+        assert np.percentile(g, 99) <= np.percentile(r, 99), (pol, np.percentile(g, 99), np.percentile(r, 99))
+        assert np.median(g) <= np.median(r), (pol, np.median(g), np.median(r))
     flat = st.flat()
     for idx in (0, 1111, 4095, 4096, 6000, 8191):
         pol = probs[idx][4]
