@@ -89,6 +89,8 @@ struct QcLaunch {
     int max_iter;
     const struct DenseItem* items;
     int* info;
+    cd* wbase;                // dense eigensolves: base of the eigenvalue offsets (DenseItem::w)
+    cd* sbase;                // dense eigensolves: base of the scratch offsets (DenseItem::scratch)
 };
 
 struct EigScratch {
@@ -98,10 +100,11 @@ struct EigScratch {
 
 // Rotation G_i from the carried diagonal (xr + i xi) and the real sub-diagonal sub: c = x / |(x, sub)|,
 // s = sub / |(x, sub)| (complex cosine, real sine), and the new diagonal |(x, sub)|.  Operands far from 1
-// are scaled by a power of two first.  1/|(x, sub)| is the hardware reciprocal square root refined by two
-// Newton steps (y <- y + y (1 - s y^2) / 2, each doubling the correct bits): a dependent chain of about
-// ten FP64 operations instead of a correctly rounded sqrt and a division.  Both sweep forms use this one
-// function, so a matrix gets the same rotations whichever instantiation decomposes it.
+// are scaled by a power of two first.  By default (SRCWA_QR_EXACT_ROTATION) the norm is a correctly
+// rounded sqrt and c, s are correctly rounded quotients, as the reference's rotation (eig_qr.hpp:460-463);
+// the alternative, the hardware reciprocal square root refined by two Newton steps, is 5 % faster but its
+// rotations are less exactly unitary and the eigen-pair residuals come out 20 % higher.  Both sweep forms
+// use this one function, so a matrix gets the same rotations whichever instantiation decomposes it.
 DEV double qr_rotation(double xr, double xi, double sub, double& cr, double& ci, double& sn) {
     const double mm = fmax(fmax(fabs(xr), fabs(xi)), fabs(sub));
     int e = 0;

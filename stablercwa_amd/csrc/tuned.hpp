@@ -49,9 +49,11 @@
 #define SRCWA_LU_OUTLINE 0
 #endif
 // 1: the QR rotation divides by a correctly rounded sqrt; 0: hardware reciprocal square root refined by
-// two Newton steps
+// two Newton steps.  Measured on MI355X over 112 matrices (random orders 41-241 and planar layer
+// matrices): mean eigen-pair residual 6.2e-16 with 1 against 7.7e-16 with 0 (the reference QR: 6.5e-16),
+// for 5 % more eigen time; so 1
 #ifndef SRCWA_QR_EXACT_ROTATION
-#define SRCWA_QR_EXACT_ROTATION 0
+#define SRCWA_QR_EXACT_ROTATION 1
 #endif
 
 // (a minimum of one wave per SIMD is the same as no bound)

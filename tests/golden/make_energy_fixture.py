@@ -1,7 +1,8 @@
 """Writes tests/golden/planar_energy_ref.json and planar_energy_ref.npz: the reference solver's own
 energy defect |R+T-1| over the benchmark sweep (grating on a 20-layer mirror, M = 60, all 4096
-wavelengths, TE and TM), computed with oracle/_ref.  A lossless stack's defect is pure rounding; this
-fixes the level the GPU path is held to, point by point (the .npz holds every point's defect).
+wavelengths, TE and TM), computed with oracle/_ref, and (keys M150_*) over a 512-wavelength sweep of
+the same stack at M = 150 (N = 301).  A lossless stack's defect is pure rounding; this fixes the level
+the GPU path is held to, as a distribution over the same points (the .npz holds every point's defect).
 
     python tests/golden/make_energy_fixture.py
 """
@@ -37,8 +38,12 @@ def main():
     with open(os.path.join(HERE, "planar_energy_ref.json"), "w") as f:
         json.dump(dict(source="reference rcwa::sweep (oracle/_ref), grating + 20-layer mirror, M=60, theta 66.36",
                        **out), f, indent=1)
-    np.savez_compressed(os.path.join(HERE, "planar_energy_ref.npz"), wavelength=lams, TE=per_point["TE"],
-                        TM=per_point["TM"])
+    lams150 = np.linspace(0.95, 1.15, 512)
+    for pol, name in ((0, "TE"), (1, "TM")):
+        der, det = ref.sweep1d(lib, st.flat(), lams150, np.full(lams150.size, 66.36), 150, pol, os.cpu_count() or 1)
+        per_point["M150_" + name] = np.abs(der.sum(1) + det.sum(1) - 1.0)
+    np.savez_compressed(os.path.join(HERE, "planar_energy_ref.npz"), wavelength=lams, M150_wavelength=lams150,
+                        **per_point)
     print(out)
 
 

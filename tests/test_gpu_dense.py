@@ -77,18 +77,35 @@ def test_zgeev_matrices(ctx):
         _check_eig(A, w, V, info)
 
 
-def test_zgeev_against_reference_residual(ctx, ref_lib):
-    """The GPU eigen-pair residual stays at or below a small multiple of the reference QR's."""
+def test_zgeev_residual_not_worse_than_reference(ctx, ref_lib):
+    """The eigen-pair residual max_j ||A v_j - l_j v_j|| / (||A||_F ||v_j||) (eig_residual, reference
+    eig.hpp:38-69) over 80 random matrices (orders 41, 121, 241) and 32 of the reference's own planar
+    layer matrices (mirror grating, M = 60, TE and TM): the GPU's residuals are not worse than the
+    reference QR's on the same matrices (conftest.not_worse_than), and none exceeds 2e-15."""
+    import ctypes
     from oracle import ref
     from stablercwa_amd import rcwa
-    rng = np.random.default_rng(4)
-    mats = [crand(rng, n, n) for n in (30, 121, 241)]
-    out = rcwa.zgeev_batched(mats, ctx)
-    for A, (w, V, info) in zip(mats, out):
-        res = _check_eig(A, w, V, info)
-        _, _, rinfo, rres = ref.eig(ref_lib, A)
-        assert rinfo == 0
-        assert res <= 4 * max(rres, 1e-16), (res, rres)
+    from stablercwa_amd.workloads import mirror_stack
+    from conftest import not_worse_than
+    rng = np.random.default_rng(123)
+    groups = [[crand(rng, n, n) for _ in range(k)] for n, k in ((41, 32), (121, 32), (241, 16))]
+    f = mirror_stack().flat()
+    layer = []
+    for pol in (0, 1):
+        for lam in np.linspace(0.95, 1.15, 16):
+            A = np.zeros(2 * 121 * 121)
+            assert ref_lib.ref_layer_matrix(ctypes.c_void_p(f.ctypes.data), 0, ctypes.c_double(lam),
+                                            ctypes.c_double(66.36), 60, pol, ctypes.c_void_p(A.ctypes.data)) == 0
+            layer.append(A.view(np.complex128).reshape((121, 121), order="F").copy())
+    groups.append(layer)
+    gpu, refr = [], []
+    for mats in groups:
+        for A, (w, V, info) in zip(mats, rcwa.zgeev_batched(mats, ctx)):
+            gpu.append(_check_eig(A, w, V, info, tol=2e-15))
+            _, _, rinfo, rres = ref.eig(ref_lib, A)
+            assert rinfo == 0
+            refr.append(rres)
+    not_worse_than(gpu, refr, "eig residual")
 
 
 def test_zgesv_every_panel_path(ctx):

@@ -8,8 +8,9 @@ compiled from its headers (tests/golden/large_ref.npz, written by tests/golden/m
 * the served mix at its stated orders (planar M up to 150, conical up to 80, crossed up to 9): every
   served result against the reference, and bitwise against the same request solved alone
 
-Order efficiencies agree to 1e-10 absolute (the plain FP64 form).  For lossless planar stacks R + T - 1 is
-held to max(1e-12, the reference's own defect on the same case) plus the sums' own rounding.
+Order efficiencies agree to 1e-10 absolute (the plain FP64 form).  The energy defect of lossless planar
+stacks at M = 150 is held to the reference's distribution in test_gpu_planar.py (one point against one
+point compares rounding noise).
 """
 import json
 import os
@@ -49,14 +50,6 @@ def _jones(c):
     return (complex(j[0], j[1]), complex(j[2], j[3]))
 
 
-def _energy_ok(der, det, ref_der, ref_det):
-    """|R+T-1| of a lossless planar solve at most max(1e-12, the reference's own defect), plus the rounding
-    of the efficiency sums themselves (one unit roundoff per summed order: 2N eps)."""
-    gpu = abs(np.sum(der) + np.sum(det) - 1.0)
-    ref = abs(np.sum(ref_der) + np.sum(ref_det) - 1.0)
-    return gpu <= max(1e-12, ref) + (der.size + det.size) * 2.2e-16, (gpu, ref)
-
-
 def test_planar_M150_against_reference(ctx):
     from stablercwa_amd import rcwa
     from stablercwa_amd.types import Stack
@@ -71,9 +64,6 @@ def test_planar_M150_against_reference(ctx):
         assert res["status"][i] == 0 and r.eig_info == c["eig_info"] == 0, c["name"]
         err = max(np.max(np.abs(r.DEr - c["DEr"])), np.max(np.abs(r.DEt - c["DEt"])))
         assert err <= TOL, (c["name"], err)
-        if c["lossless"]:
-            ok, vals = _energy_ok(r.DEr, r.DEt, c["DEr"], c["DEt"])
-            assert ok, (c["name"], vals)
     b.close()
 
 

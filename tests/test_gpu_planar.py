@@ -14,7 +14,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, load_golden, max_abs_diff
+from conftest import GOLDEN, load_golden, max_abs_diff, not_worse_than
 
 pytestmark = pytest.mark.gpu
 
@@ -110,8 +110,8 @@ def test_full_size_sweep_energy_and_sample_parity(ctx):
     det = res["DEt"].reshape(-1, N)
     rt = der.sum(axis=1) + det.sum(axis=1)
     # lossless: |R+T-1| is rounding only.  Held to the reference solver's own defect on the same 8192
-    # solves (planar_energy_ref.npz, every point): maximum, 99th percentile and median of the GPU's
-    # defect at or below the reference's, per polarization
+    # solves (planar_energy_ref.npz, every point), per polarization: maximum, 99th percentile and median
+    # at or below the reference's, and a distribution not worse than the reference's
     e = np.abs(rt - 1.0)
     z = np.load(os.path.join(GOLDEN, "planar_energy_ref.npz"), allow_pickle=False)
     assert np.array_equal(z["wavelength"], lams)
@@ -120,6 +120,7 @@ def test_full_size_sweep_energy_and_sample_parity(ctx):
         assert np.max(g) <= np.max(r), (pol, np.max(g), np.max(r))
         assert np.percentile(g, 99) <= np.percentile(r, 99), (pol, np.percentile(g, 99), np.percentile(r, 99))
         assert np.median(g) <= np.median(r), (pol, np.median(g), np.median(r))
+        not_worse_than(g, r, pol)
     flat = st.flat()
     for idx in (0, 1111, 4095, 4096, 6000, 8191):
         pol = probs[idx][4]
@@ -195,3 +196,21 @@ def test_invalid_input_raises(ctx):
         rcwa.solve(ok, 1.0, 90.0, 5, rcwa.Polarization.TE, ctx)
     with pytest.raises(ValueError):
         rcwa.solve(Stack(complex(1.0, 0.1), ok.layers, 1.5, 0.6), 1.0, 10.0, 5, rcwa.Polarization.TE, ctx)
+
+
+def test_M150_sweep_energy_not_worse_than_reference(ctx):
+    """The mirror grating at M = 150 (N = 301, the served mix's largest planar order), 512 wavelengths x
+    TE/TM: the reference's own |R+T-1| exceeds 1e-12 on 80 % of these points (max 1.9e-11,
+    planar_energy_ref.npz), so the GPU is held to the reference's distribution on the same points."""
+    from stablercwa_amd import rcwa
+    z = np.load(os.path.join(GOLDEN, "planar_energy_ref.npz"), allow_pickle=False)
+    lams = z["M150_wavelength"]
+    b = rcwa.PlanarBatch([_mirror_stack()], [(0, float(l), 66.36, 150, pol) for pol in (0, 1) for l in lams], ctx)
+    b.execute()
+    res = b.results()
+    b.close()
+    assert np.all(res["status"] == 0)
+    n = lams.size
+    e = np.abs(res["DEr"].reshape(-1, 301).sum(1) + res["DEt"].reshape(-1, 301).sum(1) - 1.0)
+    for k, pol in enumerate(("TE", "TM")):
+        not_worse_than(e[n * k:n * (k + 1)], z["M150_" + pol], pol)
