@@ -177,6 +177,19 @@ int srcwa_zgesv_batched(srcwa_context* ctx, int count, const int* n, const int* 
 int srcwa_zgemm_batched(srcwa_context* ctx, int count, const int* m, const int* n, const int* k, const double* A,
                         const double* B, double* C);
 
+/* srcwa_zgeev_batched without any device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  A, w, V and info are DEVICE pointers the caller owns, items packed back to back
+ * as in srcwa_zgeev_batched (A_i, V_i: n_i x n_i column-major complex; w_i: n_i complex; info: count
+ * ints); n is a host array of the orders.  A is read (V may equal A: then it is decomposed in place).
+ * The scratch (item descriptors and each item's Z, X, ort, tmp: 16 (2 n_i^2 + 2 n_i) bytes, rounded up)
+ * comes from a 64 MiB device array the library holds on every GPU it runs on; a batch that needs more
+ * returns SRCWA_ENOMEM and writes nothing.  Nothing in the scratch has to start at zero.  Synchronous
+ * on the context's stream.  Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (from several threads or contexts) race and must be serialised by the caller. */
+int srcwa_zgeev_batched_noalloc(srcwa_context* ctx, int count, const int* n, const double* A, double* w, double* V,
+                                int* info);
+
 /* ---------------------------------------------------------------------------------------------
  * Request server with continuous batching (no counterpart in the reference, whose solves are
  * synchronous calls; it serves the reference's solve/sweep requests to many callers at once).

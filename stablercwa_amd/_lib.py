@@ -71,6 +71,7 @@ PROTOTYPES = {
     "srcwa_zgeev_batched": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
     "srcwa_zgesv_batched": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "srcwa_zgemm_batched": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "srcwa_zgeev_batched_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

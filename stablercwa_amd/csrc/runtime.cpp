@@ -17,6 +17,7 @@
 
 #include "../../include/srcwa.h"
 #include "batch.hpp"
+#include "context.hpp"
 #include "cplx.hpp"
 
 // kernel launchers (rcwa1d.hip, dense.hip)
@@ -70,6 +71,8 @@ int guarded(F&& f) {
 void invalid(const std::string& m) { throw Failure{SRCWA_EINVAL, m}; }
 
 }  // namespace
+
+void srcwa_set_last_error(const std::string& msg) { g_error = msg; }
 
 // A device arena carved first-fit into 256-byte aligned blocks.  Plans created on a context that has
 // a pool take every device buffer from it, so many plans can come and go (a request server) without
@@ -351,13 +354,6 @@ PlanarCost planar_cost(int N, const std::vector<bool>& general, int pol, bool co
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------
-struct srcwa_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipStream_t back = nullptr;     // second stream, created for the first plan with several chunks
-    srcwa_pool* pool = nullptr;     // plans created on this context draw device memory from it
-};
-
 struct PlanChunk {
     int i0, i1, t0, t1;
 };
@@ -515,6 +511,8 @@ int srcwa_context_create(int device, srcwa_context** out) {
         hip_check(hipGetDeviceCount(&count), "hipGetDeviceCount");
         if (device < 0 || device >= count) invalid("srcwa_context_create: no such device");
         hip_check(hipSetDevice(device), "hipSetDevice");
+        // the no-allocation entry point's scratch array, located once per device (dense.hip)
+        hip_check(static_cast<hipError_t>(dense_scratch_setup(device)), "hipGetSymbolAddress (scratch)");
         auto* c = new srcwa_context;
         c->device = device;
         const hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
