@@ -58,6 +58,15 @@ int srcwa_pool_destroy(srcwa_pool* pool);
 int64_t srcwa_pool_free_bytes(srcwa_pool* pool);
 int srcwa_context_set_pool(srcwa_context* ctx, srcwa_pool* pool);
 
+/* Flags of the plans created on a context from now on (they are fixed at plan creation).
+ * SRCWA_PLAN_DIAGNOSTICS: the reference's Options::diagnostics (rcwa1d.hpp:120, 141-163): every layer
+ *   eigensolve's eigen-pair residual max_j ||A v_j - l_j v_j|| / (||A||_F ||v_j||) (eig.hpp:38-69) and every
+ *   dense solve's backward error ||A X - B||_F / (||A||_F ||X||_F + ||B||_F) (lu.hpp:452-459), reduced per
+ *   solve by srcwa_plan_diagnostics; the plan holds copies of the layer matrices (more device memory)
+ *   and each dense solve does one more product. */
+enum { SRCWA_PLAN_DIAGNOSTICS = 1 };
+int srcwa_context_set_plan_flags(srcwa_context* ctx, int flags);
+
 /* ---------------------------------------------------------------------------------------------
  * Batched planar (TE/TM) solves.  Replaces rcwa::solve / rcwa::sweep (rcwa1d.hpp:1077-1091): each
  * problem is (stack, wavelength, theta_deg, M, pol) with pol 0 = TE, 1 = TM, and yields N = 2M+1
@@ -138,6 +147,15 @@ int srcwa_plan_poll(srcwa_plan* plan);
 int srcwa_plan_sync(srcwa_plan* plan);
 /* Copy results to host buffers (implies sync).  Any pointer may be NULL. */
 int srcwa_plan_results(srcwa_plan* plan, double* R, double* T, double* DEr, double* DEt, int* status, int* eig_info);
+/* Per-solve diagnostics of the last execute, 5 doubles per problem (Result::max_eig_residual,
+ * max_solve_backward_error, n_eig, n_dense_solve; rcwa1d.hpp:159-162, ConicalResult, Result2D; and
+ * ConicalResult::n_schur, rcwa1d_conical.hpp:175):
+ * [0] largest eigen-pair residual of the solve's eigensolves (NaN when one did not converge; 0 without
+ * SRCWA_PLAN_DIAGNOSTICS), [1] largest backward error of its dense solves (0 without the flag, and for
+ * crossed gratings, whose Result2D has no such field), [2] dense eigensolves, [3] dense linear solves,
+ * counted as the reference counts them (a conical layer in the Schur basis adds one eigensolve), [4] conical
+ * layers propagated in the Schur basis (0 for planar and crossed plans). */
+int srcwa_plan_diagnostics(srcwa_plan* plan, double* out);
 /* Per-order geometry of the last execute of a conical or crossed plan (8 doubles per order, at 8 x the
  * order offset).  Conical: kx, kzI (re, im), kzII (re, im), flux_r, flux_t, phi_m in degrees
  * (ConicalResult::kx, kzI, kzII, flux_r, flux_t, phi_m_deg, rcwa1d_conical.hpp:105-112).  Crossed: Kx, Ky,

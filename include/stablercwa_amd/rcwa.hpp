@@ -27,6 +27,7 @@
 #include <map>
 #include <mutex>
 #include <stdexcept>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -53,17 +54,34 @@ inline void check(int code) {
     throw std::runtime_error(msg);
 }
 
-// One context (HIP stream) per device for the whole process; contexts live until the process ends.
-inline srcwa_context* context(int device) {
+// One context (HIP stream) per device and plan-flag set for the whole process (the flags are fixed per
+// context, so concurrent callers with different options never share one); contexts live until the
+// process ends.
+inline srcwa_context* context(int device, int plan_flags = 0) {
     static std::mutex lock;
-    static std::map<int, srcwa_context*> contexts;
+    static std::map<std::pair<int, int>, srcwa_context*> contexts;
     std::lock_guard<std::mutex> guard(lock);
-    auto it = contexts.find(device);
+    const auto key = std::make_pair(device, plan_flags);
+    auto it = contexts.find(key);
     if (it != contexts.end()) return it->second;
     srcwa_context* ctx = nullptr;
     check(srcwa_context_create(device, &ctx));
-    contexts[device] = ctx;
+    if (plan_flags) check(srcwa_context_set_plan_flags(ctx, plan_flags));
+    contexts[key] = ctx;
     return ctx;
+}
+
+// Per-solve diagnostics of a plan's last execute (srcwa_plan_diagnostics) and its stage times.
+struct PlanReport {
+    std::vector<double> diag;     // 5 per problem
+    double ms[4] = {0, 0, 0, 0};  // setup + assembly, eigensolves, modes + S-matrix, total
+};
+inline PlanReport report(srcwa_plan* p, int problems) {
+    PlanReport r;
+    r.diag.resize(5 * static_cast<std::size_t>(problems));
+    check(srcwa_plan_diagnostics(p, r.diag.data()));
+    check(srcwa_plan_stage_ms(p, r.ms));
+    return r;
 }
 
 // RAII owner of a launch plan.
@@ -152,7 +170,7 @@ struct Options {
     EigMethod eig = EigMethod::QR;
     LinearSolver linear = LinearSolver::LUPartial;
     bool homogeneous_fast_path = true;
-    bool diagnostics = false;          // accepted; the GPU path reports no residual diagnostics
+    bool diagnostics = false;          // eigen-pair residuals and solve backward errors, computed on the GPU
     bool parallel_layers = false;      // accepted; layers always run concurrently on the GPU
     std::function<EigResult(const Matrix&)> custom_eig;
     std::function<Matrix(const Matrix&, const Matrix&)> custom_multiply;
@@ -444,6 +462,22 @@ inline void flat_stack(const Stack2D& s, std::vector<double>& out) {
     }
 }
 
+// Result fields from the plan report.  A batched plan runs every problem at once, so the stage times
+// are shared out evenly: timing.eig is the eigensolve stage, timing.linear the modes and S-matrix
+// stage (its dense solves and the products between them), timing.product stays 0.
+template <class R>
+void fill_diagnostics(R& r, const PlanReport& rep, int i, int problems) {
+    const double* d = rep.diag.data() + 5 * static_cast<std::size_t>(i);
+    r.max_eig_residual = d[0];
+    r.max_solve_backward_error = d[1];
+    r.n_eig = static_cast<int>(d[2]);
+    r.n_dense_solve = static_cast<int>(d[3]);
+    if (rep.ms[1] >= 0) {
+        r.timing.eig = rep.ms[1] * 1e-3 / problems;
+        r.timing.linear = rep.ms[2] * 1e-3 / problems;
+    }
+}
+
 inline std::vector<Result> planar_batch(const Stack& stack, const std::vector<Point>& points, int M, Polarization pol,
                                         const Options& options) {
     require_gpu_options(options, true);
@@ -461,9 +495,11 @@ inline std::vector<Result> planar_batch(const Stack& stack, const std::vector<Po
         th[i] = points[i].theta_deg;
     }
     Plan plan;
-    check(srcwa_plan1d_create(context(options.device), 1, rec.data(), &off, P, idx.data(), wl.data(), th.data(),
-                              Ms.data(), pols.data(), options.max_workspace_bytes, &plan.p));
+    check(srcwa_plan1d_create(context(options.device, options.diagnostics ? SRCWA_PLAN_DIAGNOSTICS : 0), 1, rec.data(),
+                              &off, P, idx.data(), wl.data(), th.data(), Ms.data(), pols.data(),
+                              options.max_workspace_bytes, &plan.p));
     plan.run();
+    const PlanReport rep = report(plan.p, P);
     const std::int64_t orders = srcwa_plan_orders(plan.p);
     std::vector<double> R(2 * orders), T(2 * orders), DEr(orders), DEt(orders);
     std::vector<int> status(P), info(P);
@@ -481,6 +517,7 @@ inline std::vector<Result> planar_batch(const Stack& stack, const std::vector<Po
         r.DEr.assign(DEr.begin() + a, DEr.begin() + a + N);
         r.DEt.assign(DEt.begin() + a, DEt.begin() + a + N);
         r.eig_info = info[i];
+        fill_diagnostics(r, rep, i, P);
         r.timing.total = total;
     }
     return out;
@@ -504,9 +541,11 @@ inline std::vector<ConicalResult> conical_batch(const Stack& stack, const std::v
         ph[i] = points[i].phi_deg;
     }
     Plan plan;
-    check(srcwa_plan_conical_create(context(options.device), 1, rec.data(), &off, P, idx.data(), wl.data(), th.data(),
-                                    ph.data(), Ms.data(), options.max_workspace_bytes, &plan.p));
+    check(srcwa_plan_conical_create(context(options.device, options.diagnostics ? SRCWA_PLAN_DIAGNOSTICS : 0), 1,
+                                    rec.data(), &off, P, idx.data(), wl.data(), th.data(), ph.data(), Ms.data(),
+                                    options.max_workspace_bytes, &plan.p));
     plan.run();
+    const PlanReport rep = report(plan.p, P);
     const std::int64_t orders = srcwa_plan_orders(plan.p);
     std::vector<double> J(8 * orders), Jt(8 * orders), DEr(4 * orders), DEt(4 * orders), geo(8 * orders);
     std::vector<int> status(P), info(P);
@@ -550,6 +589,8 @@ inline std::vector<ConicalResult> conical_batch(const Stack& stack, const std::v
             r.DEt[j] = JonesEfficiency{et[0], et[1], et[2], et[3]};
         }
         r.eig_info = info[i];
+        fill_diagnostics(r, rep, i, P);
+        r.n_schur = static_cast<int>(rep.diag[5 * i + 4]);
         r.timing.total = total;
     }
     return out;
@@ -577,9 +618,17 @@ inline std::vector<ConicalResult> sweep_conical(const Stack& stack, const std::v
     return detail::conical_batch(stack, points, M, options);
 }
 
+// The reference's near-field capture (rcwa2d.hpp:939-950).  Declared so that reference callers that pass
+// the capture argument compile; this build computes no near fields and throws Unsupported when one is
+// requested.
+namespace detail2d {
+struct FieldCapture2D;
+}
+
 inline Result2D solve2d(const Stack2D& stack, double wavelength, double theta_deg, double phi_deg, int M1, int M2,
-                        const Jones2D& jones = jones_from_angles_2d(90.0), const Options2D& options = {}) {
+                        const Jones2D& jones, const Options2D& options = {}, detail2d::FieldCapture2D* capture = nullptr) {
     detail::require_gpu_options(options);
+    if (capture) throw Unsupported("rcwa2d: near-field capture is not provided by this build");
     const auto t0 = detail::Clock::now();
     std::vector<double> rec;
     detail::flat_stack(stack, rec);
@@ -587,9 +636,11 @@ inline Result2D solve2d(const Stack2D& stack, double wavelength, double theta_de
     const int idx = 0, fz = options.factorization == Factorization2D::Li ? 0 : 1;
     const double jv[4] = {jones[0].real(), jones[0].imag(), jones[1].real(), jones[1].imag()};
     detail::Plan plan;
-    detail::check(srcwa_plan2d_create(detail::context(options.device), 1, rec.data(), &off, 1, &idx, &wavelength,
-                                      &theta_deg, &phi_deg, &M1, &M2, jv, &fz, options.max_workspace_bytes, &plan.p));
+    detail::check(srcwa_plan2d_create(detail::context(options.device, options.diagnostics ? SRCWA_PLAN_DIAGNOSTICS : 0),
+                                      1, rec.data(), &off, 1, &idx, &wavelength, &theta_deg, &phi_deg, &M1, &M2, jv,
+                                      &fz, options.max_workspace_bytes, &plan.p));
     plan.run();
+    const detail::PlanReport rep = detail::report(plan.p, 1);
     const std::int64_t N = srcwa_plan_orders(plan.p);
     std::vector<double> R(8 * N), T(8 * N), DEr(N), DEt(N), geo(8 * N);
     int status = 0, info = 0;
@@ -647,6 +698,14 @@ inline Result2D solve2d(const Stack2D& stack, double wavelength, double theta_de
         r.prop_t[i] = r.kzII[i].real() > 1e-12;
     }
     r.eig_info = info;
+    r.max_eig_residual = rep.diag[0];
+    r.n_eig = static_cast<int>(rep.diag[2]);
+    r.n_dense_solve = static_cast<int>(rep.diag[3]);
+    if (rep.ms[0] >= 0) {
+        r.timing.operators = rep.ms[0] * 1e-3;
+        r.timing.eig = rep.ms[1] * 1e-3;
+        r.timing.linear = rep.ms[2] * 1e-3;
+    }
     r.timing.total = detail::seconds_since(t0);
     return r;
 }

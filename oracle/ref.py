@@ -64,3 +64,50 @@ def eig(lib, A):
     if rc != 0:
         raise ValueError(lib.ref_last_error().decode())
     return w.view(np.complex128), V.view(np.complex128).reshape((n, n), order="F"), info.value, res.value
+
+
+def solve1d_ex(lib, stack_flat, wavelength, theta, M, pol, max_iter=30, diagnostics=True):
+    """ref_solve1d_ex: the reference's solve with an eigen iteration limit and Options::diagnostics;
+    returns DEr, DEt and eig_info, n_eig, n_dense_solve, max_eig_residual, max_solve_backward_error."""
+    N = 2 * M + 1
+    f = np.ascontiguousarray(stack_flat, dtype=np.float64)
+    R = np.zeros(2 * N); T = np.zeros(2 * N); DEr = np.zeros(N); DEt = np.zeros(N)
+    counts = np.zeros(4, np.int32); met = np.zeros(2)
+    rc = lib.ref_solve1d_ex(_p(f), ctypes.c_double(wavelength), ctypes.c_double(theta), M, pol, max_iter,
+                            int(diagnostics), _p(R), _p(T), _p(DEr), _p(DEt), _p(counts), _p(met))
+    if rc != 0:
+        raise ValueError(lib.ref_last_error().decode())
+    return dict(DEr=DEr, DEt=DEt, eig_info=int(counts[0]), n_eig=int(counts[1]), n_dense_solve=int(counts[2]),
+                max_eig_residual=float(met[0]), max_solve_backward_error=float(met[1]))
+
+
+def solve_conical_ex(lib, stack_flat, wavelength, theta, phi, M, max_iter=30, diagnostics=True):
+    """ref_solve_conical_ex (as solve1d_ex, plus n_schur)."""
+    N = 2 * M + 1
+    f = np.ascontiguousarray(stack_flat, dtype=np.float64)
+    Jr = np.zeros(8 * N); Jt = np.zeros(8 * N); DEr = np.zeros(4 * N); DEt = np.zeros(4 * N)
+    counts = np.zeros(4, np.int32); met = np.zeros(2)
+    rc = lib.ref_solve_conical_ex(_p(f), ctypes.c_double(wavelength), ctypes.c_double(theta), ctypes.c_double(phi),
+                                  M, max_iter, int(diagnostics), _p(Jr), _p(Jt), _p(DEr), _p(DEt), _p(counts),
+                                  _p(met))
+    if rc != 0:
+        raise ValueError(lib.ref_last_error().decode())
+    return dict(DEr=DEr, DEt=DEt, eig_info=int(counts[0]), n_eig=int(counts[1]), n_dense_solve=int(counts[2]),
+                n_schur=int(counts[3]), max_eig_residual=float(met[0]), max_solve_backward_error=float(met[1]))
+
+
+def solve2d_ex(lib, stack_flat, wavelength, theta, phi, M1, M2, jones, max_iter=30, diagnostics=True):
+    """ref_solve2d_ex: status, eig_info, n_eig, n_dense_solve, max_eig_residual."""
+    N = (2 * M1 + 1) * (2 * M2 + 1)
+    f = np.ascontiguousarray(stack_flat, dtype=np.float64)
+    j = np.ascontiguousarray(jones, dtype=np.float64)
+    DEr = np.zeros(N); DEt = np.zeros(N)
+    Rs, Rp, Ts, Tp = (np.zeros(2 * N) for _ in range(4))
+    st = ctypes.c_int(); counts = np.zeros(4, np.int32); met = np.zeros(2)
+    rc = lib.ref_solve2d_ex(_p(f), ctypes.c_double(wavelength), ctypes.c_double(theta), ctypes.c_double(phi), M1,
+                            M2, _p(j), max_iter, int(diagnostics), _p(DEr), _p(DEt), _p(Rs), _p(Rp), _p(Ts), _p(Tp),
+                            ctypes.byref(st), _p(counts), _p(met))
+    if rc != 0:
+        raise ValueError(lib.ref_last_error().decode())
+    return dict(DEr=DEr, DEt=DEt, status=st.value, eig_info=int(counts[0]), n_eig=int(counts[1]),
+                n_dense_solve=int(counts[2]), max_eig_residual=float(met[0]))

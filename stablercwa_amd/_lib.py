@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsrcwa_amd.so")
 
 OK, EINVAL, EDEVICE, ENOMEM, EUNSUPPORTED = 0, -1, -2, -3, -4
+PLAN_DIAGNOSTICS = 1
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -48,6 +49,8 @@ PROTOTYPES = {
     "srcwa_pool_destroy": (ctypes.c_int, [_vp]),
     "srcwa_pool_free_bytes": (ctypes.c_int64, [_vp]),
     "srcwa_context_set_pool": (ctypes.c_int, [_vp, _vp]),
+    "srcwa_context_set_plan_flags": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "srcwa_plan_diagnostics": (ctypes.c_int, [_vp, _vp]),
     "srcwa_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                            ctypes.POINTER(_vp)]),
     "srcwa_server_submit": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_int64)]),
@@ -116,12 +119,14 @@ def ptr(a: np.ndarray) -> ctypes.c_void_p:
 class Context:
     """A device context: one HIP stream on one GPU."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, plan_flags: int = 0):
         lib = load()
         h = _vp()
         check(lib.srcwa_context_create(device, ctypes.byref(h)))
         self.handle = h
         self.device = device
+        if plan_flags:
+            check(lib.srcwa_context_set_plan_flags(h, int(plan_flags)))
 
     def close(self) -> None:
         if self.handle:

@@ -15,10 +15,28 @@ enum : int {
     ST_SCHUR = 4,          // conical: a degenerate layer needs Schur-basis propagation (not provided)
 };
 
+// Plan flags (srcwa_context_set_plan_flags), carried per instance.
+enum : int {
+    PLAN_DIAGNOSTICS = 1,      // eigen-pair residuals and solve backward errors (Options::diagnostics)
+};
+
+// Per-layer-task diagnostics, reduced per instance by the S-matrix kernels.
+struct TaskDiag {
+    double eig_res;            // largest eigen-pair residual of the task's eigensolves (NaN: no vectors)
+    double berr;               // largest backward error of the task's dense solves
+    int n_solve;               // dense linear solves of the task (assembly)
+    int n_eig;                 // dense eigensolves of the task
+};
+
+// Per-instance diagnostics record (srcwa_plan_diagnostics): max_eig_residual, max_solve_backward_error,
+// n_eig, n_dense_solve, n_schur.
+constexpr int DIAG_WORDS = 5;
+
 // One planar (TE/TM) or conical instance.
 struct Inst1D {
     int M, N, pol, L;          // pol: 0 TE, 1 TM, 2 conical
     int first_task, n_tasks;   // layer tasks (grating layers) of this instance
+    int flags;                 // PLAN_* flags
     double wavelength, theta_deg, phi_deg;
     int64_t stack_off;         // offset of the flat stack record (doubles)
     int64_t layout_off;        // offset of the layout table (int64)
@@ -76,6 +94,7 @@ __host__ __device__ inline int x2_layout_size(int L) { return 1 + 2 * (L + 2) + 
 struct Inst2D {
     int M1, M2, N, L;
     int first_task, n_tasks;
+    int flags;                 // PLAN_* flags
     int laurent;               // 0: Li factorisation, 1: Laurent rule
     double wavelength, theta_deg, phi_deg;
     double jones[4];           // Es.re, Es.im, Ep.re, Ep.im
@@ -91,6 +110,9 @@ struct Out1D {
     int* status;
     int* eig_info;
     double* geometry;   // conical / crossed: GEO_WORDS doubles per order (null for planar plans)
+    double* diag;       // DIAG_WORDS per instance
+    TaskDiag* task_diag;   // per layer task of the launch chunk
+    double* item_res;      // conical: the eigen residual of each of a task's two eigensolves (2 per task)
 };
 
 // Per-order geometry written by the conical and crossed setup kernels:

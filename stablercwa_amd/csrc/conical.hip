@@ -171,7 +171,7 @@ __global__ __launch_bounds__(NT) void k_assemble_conical(const Inst1D* __restric
                                                          const LayerTask* __restrict__ tasks,
                                                          const double* __restrict__ stacks,
                                                          const int64_t* __restrict__ layouts, cd* __restrict__ ws,
-                                                         int* __restrict__ task_info) {
+                                                         int* __restrict__ task_info, TaskDiag* __restrict__ task_diag) {
     const LayerTask tk = tasks[blockIdx.x];
     const Inst1D in = insts[tk.inst];
     const double* st = stacks + in.stack_off;
@@ -182,6 +182,11 @@ __global__ __launch_bounds__(NT) void k_assemble_conical(const Inst1D* __restric
     const ConicalGeo g = geo_of(ws, lay, N);
     const ConicalSlots s = slots_of(ws + lay[conical_slot(L, tk.layer)], N);
     cd* coef = s.lam1;                         // lambda1 .. tmp2 hold the 4M+1 coefficients
+    // with diagnostics: D0, D1 keep A and B C for the eigen residuals, D2, D3 are solve scratch
+    const bool diag = in.flags & PLAN_DIAGNOSTICS;
+    cd* D0 = ws + lay[conical_slot(L, tk.layer)] + conical_slot_words(N);
+    cd *D1 = D0 + NN, *D2 = D0 + 2 * NN, *D3 = D0 + 3 * NN;
+    double berr = 0.0;
     int fail = 0;
     fourier_coefficients(lv, st[0], 2 * M, false, coef);
     toeplitz(coef, N, s.A);                    // [[eps]]
@@ -189,11 +194,17 @@ __global__ __launch_bounds__(NT) void k_assemble_conical(const Inst1D* __restric
     fail |= wg_lu(N, s.Z1, N, s.piv);
     wg_identity(N, s.IE, N);
     if (!fail) wg_lu_solve(N, s.Z1, N, s.piv, s.IE, N, N);          // [[eps]]^-1
+    if (diag && !fail) {
+        wg_identity(N, D2, N);
+        berr = max_keep_nan(berr, wg_backward_error(N, s.A, s.IE, D2, N, D3));
+    }
     fourier_coefficients(lv, st[0], 2 * M, true, coef);
     toeplitz(coef, N, s.Z2);                   // [[1/eps]]
+    if (diag) wg_copy(N, N, s.Z2, N, D0, N);
     fail |= wg_lu(N, s.Z2, N, s.piv);
     wg_identity(N, s.C, N);
     if (!fail) wg_lu_solve(N, s.Z2, N, s.piv, s.C, N, N);           // C = [[1/eps]]^-1
+    if (diag && !fail) berr = max_keep_nan(berr, wg_backward_error(N, D0, s.C, D2, N, D3));
     // B = Kx [[eps]]^-1 Kx - I, then B C
     for (int64_t e = threadIdx.x; e < NN; e += NT) {
         const int i = (int)(e % N), j = (int)(e / N);
@@ -214,7 +225,46 @@ __global__ __launch_bounds__(NT) void k_assemble_conical(const Inst1D* __restric
     int bad = 0;
     for (int64_t e = threadIdx.x; e < NN; e += NT) bad |= !finite(s.A[e]) || !finite(s.BC[e]);
     bad = block_max_int(bad);
-    if (threadIdx.x == 0) task_info[4 * blockIdx.x] = (fail || bad) ? -1 : 0;
+    if (diag) {
+        wg_copy(N, N, s.A, N, D0, N);
+        wg_copy(N, N, s.BC, N, D1, N);
+    }
+    if (threadIdx.x == 0) {
+        task_info[4 * blockIdx.x] = (fail || bad) ? -1 : 0;
+        // two dense solves (the inverses of [[eps]] and [[1/eps]], rcwa1d_conical.hpp:302-336), two
+        // eigensolves (A and B C); a Schur-basis layer adds one more, counted from its modes status
+        task_diag[blockIdx.x] = TaskDiag{0.0, berr, 2, 2};
+    }
+}
+
+// task_diag[t].eig_res = the larger residual of the task's two eigensolves
+__global__ __launch_bounds__(NT) void k_fold_res(int n_tasks, const double* __restrict__ item_res,
+                                                 TaskDiag* __restrict__ task_diag) {
+    const int t = blockIdx.x * NT + threadIdx.x;
+    if (t < n_tasks) task_diag[t].eig_res = max_keep_nan(item_res[2 * t], item_res[2 * t + 1]);
+}
+
+// Eigen-pair residuals of a layer's two eigensolves (A and B C) from the matrices kept in D0 and D1
+// (diagnostics only); blockIdx.x = 2 task + which, Z1 / Z2 are scratch (free until the modes kernel).
+__global__ __launch_bounds__(NT) void k_eigres_conical(const Inst1D* __restrict__ insts,
+                                                       const LayerTask* __restrict__ tasks,
+                                                       const int64_t* __restrict__ layouts, cd* __restrict__ ws,
+                                                       const int* __restrict__ task_info,
+                                                       double* __restrict__ item_res) {
+    const int t = blockIdx.x >> 1, which = blockIdx.x & 1;
+    const LayerTask tk = tasks[t];
+    const Inst1D in = insts[tk.inst];
+    if (!(in.flags & PLAN_DIAGNOSTICS)) return;
+    const int N = in.N;
+    const int64_t NN = (int64_t)N * N;
+    const int64_t* lay = layouts + in.layout_off;
+    cd* base = ws + lay[conical_slot(in.L, tk.layer)];
+    const ConicalSlots s = slots_of(base, N);
+    cd* D = base + conical_slot_words(N) + which * NN;
+    double res = __builtin_nan("");
+    if (task_info[4 * t] == 0 && task_info[4 * t + 1 + which] == 0)
+        res = wg_eig_residual(N, D, which ? s.BC : s.A, which ? s.lam2 : s.lam1, which ? s.Z2 : s.Z1);
+    if (threadIdx.x == 0) item_res[blockIdx.x] = res;
 }
 #endif
 
@@ -596,11 +646,26 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix_conical(const Inst1D* __restrict__ 
     double* DEt = out.DEt + 4 * in.out_off;
 
     int eig_info = 0, failed = 0;
+    double max_res = 0.0, max_berr = 0.0;
+    int n_solve = 0, n_eig = 0, n_schur = 0;
+    const bool diag = in.flags & PLAN_DIAGNOSTICS;
+    SmatCtx c;
+    c.N = n;
+    c.fail = 0;
+    c.diag = diag;
     for (int t = 0; t < in.n_tasks; ++t) {
         const int* ti = task_info + 4 * (in.first_task + t);
         if (ti[0] < 0 || ti[3] < 0) failed = 1;
         for (int w = 1; w <= 2; ++w)
             if (ti[w] != 0 && eig_info == 0) eig_info = ti[w];
+        const TaskDiag d = out.task_diag[in.first_task + t];
+        n_solve += d.n_solve;
+        n_eig += d.n_eig + (ti[3] == 3 ? 1 : 0);
+        n_schur += ti[3] == 3 ? 1 : 0;
+        if (diag) {
+            max_berr = max_keep_nan(max_berr, d.berr);
+            max_res = max_keep_nan(max_res, d.eig_res);
+        }
     }
     int status = (failed || eig_info) ? ST_NUMERICAL : ST_OK;
     // a uniform layer with an exactly grazing order fails (pure film stacks: unexcited orders excepted)
@@ -623,10 +688,7 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix_conical(const Inst1D* __restrict__ 
             if (threadIdx.x == 0) s_thick[j] = lv.d;
         }
         __syncthreads();
-        SmatCtx c;
-        c.N = n;
         c.bump = Bump(ws, in.ws_lo, in.ws_hi);
-        c.fail = 0;
         auto region = [&](int r) -> Region {
             Region g;
             g.fzb = false;
@@ -695,6 +757,12 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix_conical(const Inst1D* __restrict__ 
     if (threadIdx.x == 0) {
         out.status[blockIdx.x] = status;
         out.eig_info[blockIdx.x] = eig_info;
+        double* dg = out.diag + (int64_t)DIAG_WORDS * blockIdx.x;
+        dg[0] = max_res;
+        dg[1] = max_keep_nan(max_berr, c.max_berr);
+        dg[2] = n_eig;
+        dg[3] = n_solve + c.n_solve;
+        dg[4] = n_schur;
     }
 }
 void smatrix_conical_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out) {
@@ -709,7 +777,8 @@ void smatrix_conical_launch(int grid, hipStream_t s, const Inst1D* insts, const 
 #if SRCWA_MAIN_TU
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    int part, hipEvent_t* ev) {
+                    int part, int flags, hipEvent_t* ev) {
+    const bool diag = flags & PLAN_DIAGNOSTICS;
     // part 1: setup, assembly, eigensolves (events ev[0..2]); part 2: modes, S-matrix, efficiencies
     // (events ev[3], ev[4]); the host may run the parts of different chunks on two streams
     QcLaunch a{};
@@ -724,7 +793,8 @@ void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts,
     if (part & 1) {
         if (ev) hipEventRecord(ev[0], s);
         k_setup_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);
-        if (n_tasks > 0) k_assemble_conical<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
+        if (n_tasks > 0)
+            k_assemble_conical<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info, out.task_diag);
         if (ev) hipEventRecord(ev[1], s);
         // each matrix goes to the wave or the workgroup eigensolver by its own order
         if (n_tasks > 0 && min_n <= WAVE_EIG_MAX_N) {
@@ -737,6 +807,10 @@ void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts,
             a.what = 0;
             a.grid = 2 * n_tasks;
             conical_qc(eig_columns_per_thread(max_n), a);
+        }
+        if (n_tasks > 0 && diag) {
+            k_eigres_conical<<<2 * n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info, out.item_res);
+            k_fold_res<<<(n_tasks + NT - 1) / NT, NT, 0, s>>>(n_tasks, out.item_res, out.task_diag);
         }
         if (ev) hipEventRecord(ev[2], s);
     }

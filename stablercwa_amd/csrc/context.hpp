@@ -13,6 +13,7 @@ struct srcwa_context {
     hipStream_t stream = nullptr;
     hipStream_t back = nullptr;     // second stream, created for the first plan with several chunks
     srcwa_pool* pool = nullptr;     // plans created on this context draw device memory from it
+    int plan_flags = 0;             // SRCWA_PLAN_* flags of plans created on this context
 };
 
 // Message of the calling thread's last failed entry point (srcwa_last_error); runtime.cpp.

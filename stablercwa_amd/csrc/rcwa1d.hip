@@ -46,7 +46,8 @@ __global__ __launch_bounds__(NT) void k_setup1d(const Inst1D* __restrict__ insts
 #if SRCWA_MAIN_TU
 __global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                                    const double* __restrict__ stacks, const int64_t* __restrict__ layouts,
-                                                   cd* __restrict__ ws, int* __restrict__ task_info) {
+                                                   cd* __restrict__ ws, int* __restrict__ task_info,
+                                                   TaskDiag* __restrict__ task_diag) {
     const LayerTask tk = tasks[blockIdx.x];
     const Inst1D in = insts[tk.inst];
     const double* st = stacks + in.stack_off;
@@ -62,6 +63,11 @@ __global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ in
     cd *S0 = base, *S1 = base + NN, *S2 = base + 2 * NN, *S3 = base + 3 * NN;
     cd* coef = base + 4 * NN;                 // lambda + ort hold the 4M+1 coefficients
     int* piv = reinterpret_cast<int*>(base + 4 * NN + 3 * N);
+    // with diagnostics: D0 keeps the layer matrix for the eigen residual, D1 is solve scratch
+    const bool diag = in.flags & PLAN_DIAGNOSTICS;
+    cd* D0 = base + slot_words(N);
+    cd* D1 = D0 + NN;
+    double berr = 0.0;
     int fail = 0;
     if (in.pol == 0) {
         fourier_coefficients(lv, period, 2 * M, false, coef);
@@ -84,6 +90,13 @@ __global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ in
         wg_diag(N, kx, S2, N);
         fail |= wg_lu(N, S1, N, piv);
         wg_lu_solve(N, S1, N, piv, S2, N, N);
+        if (diag) {
+            // [[eps]] X = diag(kx): rebuild [[eps]] (S1 holds its factors) and the right-hand side
+            fourier_coefficients(lv, period, 2 * M, false, coef);
+            toeplitz(coef, N, D0);
+            wg_diag(N, kx, D1, N);
+            berr = max_keep_nan(berr, wg_backward_error(N, D0, S2, D1, N, S0));
+        }
         // S2 = Kx S2 - I
         wg_map((int)NN,
                [&](int e) {
@@ -94,14 +107,37 @@ __global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ in
                },
                [&](int e, cd v) { stc(S2 + e, v); });
         wg_copy(N, N, S3, N, S0, N);
+        if (diag) wg_copy(N, N, S2, N, D1, N);
         fail |= wg_lu(N, S0, N, piv);
         wg_lu_solve(N, S0, N, piv, S2, N, N);
+        if (diag) berr = max_keep_nan(berr, wg_backward_error(N, S3, S2, D1, N, D0));
         wg_copy(N, N, S2, N, S0, N);
     }
+    if (diag) wg_copy(N, N, S0, N, D0, N);
     int bad = 0;
     for (int64_t e = threadIdx.x; e < NN; e += NT) bad |= !finite(S0[e]);
     bad = block_max_int(bad);
-    if (threadIdx.x == 0) task_info[blockIdx.x] = (fail || bad) ? -1 : 0;
+    if (threadIdx.x == 0) {
+        task_info[blockIdx.x] = (fail || bad) ? -1 : 0;
+        // rcwa1d.hpp:432-468: TM assembly makes two dense solves
+        task_diag[blockIdx.x] = TaskDiag{0.0, berr, in.pol == 0 ? 0 : 2, 1};
+    }
+}
+
+// Eigen-pair residual of a grating layer's eigensolve from the layer matrix kept in D0 (diagnostics
+// only); S1 is scratch (the eigensolver's Z, free until the modes kernel writes V there).
+__global__ __launch_bounds__(NT) void k_eigres1d(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
+                                                 const int64_t* __restrict__ layouts, cd* __restrict__ ws,
+                                                 const int* __restrict__ task_info, TaskDiag* __restrict__ task_diag) {
+    const LayerTask tk = tasks[blockIdx.x];
+    const Inst1D in = insts[tk.inst];
+    if (!(in.flags & PLAN_DIAGNOSTICS)) return;
+    const int N = in.N;
+    const int64_t NN = (int64_t)N * N;
+    cd* base = ws + layouts[in.layout_off + layout_layer(in.L, tk.layer)];
+    double res = __builtin_nan("");            // no eigenvectors: not converged, or no layer matrix
+    if (task_info[blockIdx.x] == 0) res = wg_eig_residual(N, base + slot_words(N), base, base + 4 * NN, base + NN);
+    if (threadIdx.x == 0) task_diag[blockIdx.x].eig_res = res;
 }
 #endif
 
@@ -212,20 +248,31 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix1d(const Inst1D* __restrict__ insts,
 
     // failures of assembly or eigensolve (first nonzero eig info in layer order)
     int eig_info = 0, failed = 0;
+    double max_res = 0.0, max_berr = 0.0;
+    int n_solve = 0, n_eig = 0;
+    const bool diag = in.flags & PLAN_DIAGNOSTICS;
     for (int t = 0; t < in.n_tasks; ++t) {
         const int v = task_info[in.first_task + t];
         if (v < 0) failed = 1;
         else if (v > 0 && eig_info == 0) eig_info = v;
+        const TaskDiag d = out.task_diag[in.first_task + t];
+        n_solve += d.n_solve;
+        n_eig += d.n_eig;
+        if (diag) {
+            max_berr = max_keep_nan(max_berr, d.berr);
+            max_res = max_keep_nan(max_res, d.eig_res);
+        }
     }
     int status = (failed || eig_info) ? ST_NUMERICAL : ST_OK;
+    SmatCtx c;
+    c.N = N;
+    c.fail = 0;
+    c.diag = diag;
     if (status == ST_OK) {
         __shared__ double s_thick[SMAT_MAX_LAYERS];
         LayerCursor cur(st);
         bool pure = true;
-        SmatCtx c;
-        c.N = N;
         c.bump = Bump(ws, in.ws_lo, in.ws_hi);
-        c.fail = 0;
         // regions: 0 = incidence, j+1 = layer j, L+1 = exit
         auto region = [&](int r) -> Region {
             Region g;
@@ -302,6 +349,12 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix1d(const Inst1D* __restrict__ insts,
     if (threadIdx.x == 0) {
         out.status[blockIdx.x] = status;
         out.eig_info[blockIdx.x] = eig_info;
+        double* dg = out.diag + (int64_t)DIAG_WORDS * blockIdx.x;
+        dg[0] = max_res;
+        dg[1] = max_keep_nan(max_berr, c.max_berr);
+        dg[2] = n_eig;
+        dg[3] = n_solve + c.n_solve;
+        dg[4] = 0.0;
     }
 }
 void smatrix1d_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out) {
@@ -317,13 +370,14 @@ void smatrix1d_launch(int grid, hipStream_t s, const Inst1D* insts, const double
 // host launchers
 void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                   int part, hipEvent_t* ev) {
+                   int part, int flags, hipEvent_t* ev) {
+    const bool diag = flags & PLAN_DIAGNOSTICS;
     // part 1: setup, assembly, eigensolves (events ev[0..2]); part 2: modes, S-matrix, efficiencies
     // (events ev[3], ev[4]); the host may run the parts of different chunks on two streams
     if (part & 1) {
         if (ev) hipEventRecord(ev[0], s);
         k_setup1d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws);
-        if (n_tasks > 0) k_assemble1d<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
+        if (n_tasks > 0) k_assemble1d<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info, out.task_diag);
         if (ev) hipEventRecord(ev[1], s);
         // each matrix goes to the wave or the workgroup eigensolver by its own order, so a result does not
         // depend on what else is in the batch
@@ -346,6 +400,7 @@ void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, 
             a.grid = n_tasks;
             planar_qc(eig_columns_per_thread(max_n), a);
         }
+        if (n_tasks > 0 && diag) k_eigres1d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info, out.task_diag);
         if (ev) hipEventRecord(ev[2], s);
     }
     if (part & 2) {

@@ -428,6 +428,7 @@ __global__ __launch_bounds__(NT) void k_ops2d(const Inst2D* __restrict__ insts, 
 struct Slots2D {
     cd *EPS, *CX, *CY, *H, *Z, *X, *lam, *ort, *tmp;
     int* piv;
+    cd* D;          // with diagnostics: a copy of H (2N x 2N) for the eigen residual
 };
 DEV Slots2D slots2d(cd* ws, const int64_t* lay, int L, int j, int N) {
     const int64_t NN = (int64_t)N * N, nn = 4 * NN;
@@ -444,6 +445,7 @@ DEV Slots2D slots2d(cd* ws, const int64_t* lay, int L, int j, int N) {
     s.ort = v + 2 * N;
     s.tmp = v + 4 * N;
     s.piv = reinterpret_cast<int*>(v + 6 * N);
+    s.D = base + ((3 * NN + lay[x2_big(L, j)] + 7 * N + 15) & ~int64_t(15));
     return s;
 }
 
@@ -451,7 +453,9 @@ DEV Slots2D slots2d(cd* ws, const int64_t* lay, int L, int j, int N) {
 // Ei = Eps^-1, P = [pc1 | pc2], Q12 = Cy - Kx^2, Q21 = Ky^2 - Cx, H = P Q (rcwa2d.hpp:412-474)
 __global__ __launch_bounds__(NT) void k_pq2d(const Inst2D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                              const int64_t* __restrict__ layouts, cd* __restrict__ ws,
-                                             int* __restrict__ task_info) {
+                                             int* __restrict__ task_info, TaskDiag* __restrict__ task_diag) {
+    // one eigensolve per patterned layer, one dense solve for Ei = Eps^-1 (rcwa2d.hpp:412-474)
+    if (threadIdx.x == 0) task_diag[blockIdx.x] = TaskDiag{0.0, 0.0, 1, 1};
     if (task_info[4 * blockIdx.x] != 0) return;
     const LayerTask tk = tasks[blockIdx.x];
     const Inst2D in = insts[tk.inst];
@@ -508,7 +512,25 @@ __global__ __launch_bounds__(NT) void k_pq2d(const Inst2D* __restrict__ insts, c
     for (int64_t e = threadIdx.x; e < 4 * NN; e += NT) bad |= !finite(s.H[e]);
     for (int64_t e = threadIdx.x; e < NN; e += NT) bad |= !finite(Ei[e]);
     bad = block_max_int(bad) | fail;
+    if (in.flags & PLAN_DIAGNOSTICS) wg_copy(n, n, s.H, n, s.D, n);
     if (threadIdx.x == 0) task_info[4 * blockIdx.x] = bad ? -1 : 0;
+}
+
+// Eigen-pair residual of a patterned layer's 2N eigensolve from the copy of H (diagnostics only); Z is
+// scratch (free until the modes kernel writes V there).
+__global__ __launch_bounds__(NT) void k_eigres2d(const Inst2D* __restrict__ insts, const LayerTask* __restrict__ tasks,
+                                                 const int64_t* __restrict__ layouts, cd* __restrict__ ws,
+                                                 const int* __restrict__ task_info, TaskDiag* __restrict__ task_diag) {
+    const LayerTask tk = tasks[blockIdx.x];
+    const Inst2D in = insts[tk.inst];
+    if (!(in.flags & PLAN_DIAGNOSTICS)) return;
+    const int64_t* lay = layouts + in.layout_off;
+    if (ws[lay[0] + 4 * in.N].x != 0.0) return;     // Rayleigh anomaly: no eigensolve
+    const Slots2D s = slots2d(ws, lay, in.L, tk.layer, in.N);
+    double res = __builtin_nan("");
+    if (task_info[4 * blockIdx.x] == 0 && task_info[4 * blockIdx.x + 1] == 0)
+        res = wg_eig_residual(2 * in.N, s.D, s.H, s.lam, s.Z);
+    if (threadIdx.x == 0) task_diag[blockIdx.x].eig_res = res;
 }
 #endif
 
@@ -747,12 +769,18 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix2d(const Inst2D* __restrict__ insts,
     double* DEt = out.DEt + in.out_off;
 
     int status = ST_OK, eig_info = 0;
-    if (geo[4 * N].x != 0.0) status = ST_RAYLEIGH;
+    double max_res = 0.0;
+    int n_solve = 0, n_eig = 0;
+    if (geo[4 * N].x != 0.0) status = ST_RAYLEIGH;      // found before any layer work: nothing counted
     if (status == ST_OK) {
         for (int t = 0; t < in.n_tasks; ++t) {
             const int* ti = task_info + 4 * (in.first_task + t);
             if (ti[0] < 0 || ti[2] < 0) status = ST_NUMERICAL;
             if (ti[1] != 0 && eig_info == 0) eig_info = ti[1];
+            const TaskDiag d = out.task_diag[in.first_task + t];
+            n_solve += d.n_solve;
+            n_eig += d.n_eig;
+            if (in.flags & PLAN_DIAGNOSTICS) max_res = max_keep_nan(max_res, d.eig_res);
         }
         if (eig_info) status = ST_NUMERICAL;
     }
@@ -1194,6 +1222,13 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix2d(const Inst2D* __restrict__ insts,
     if (threadIdx.x == 0) {
         out.status[blockIdx.x] = status;
         out.eig_info[blockIdx.x] = eig_info;
+        // Result2D reports no solve backward error (rcwa2d_types.hpp:112-130)
+        double* dg = out.diag + (int64_t)DIAG_WORDS * blockIdx.x;
+        dg[0] = max_res;
+        dg[1] = 0.0;
+        dg[2] = n_eig;
+        dg[3] = n_solve + c.n_solve;
+        dg[4] = 0.0;
     }
 }
 void smatrix2d_launch(int grid, hipStream_t s, const Inst2D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out) {
@@ -1208,7 +1243,8 @@ void smatrix2d_launch(int grid, hipStream_t s, const Inst2D* insts, const double
 #if SRCWA_MAIN_TU
 void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    int part, hipEvent_t* ev) {
+                    int part, int flags, hipEvent_t* ev) {
+    const bool diag = flags & PLAN_DIAGNOSTICS;
     // part 1: setup, assembly, eigensolves (events ev[0..2]); part 2: modes, S-matrix, efficiencies
     // (events ev[3], ev[4]); the host may run the parts of different chunks on two streams
     if (part & 1) {
@@ -1216,7 +1252,7 @@ void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts,
         k_setup2d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);
         if (n_tasks > 0) {
             k_ops2d<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
-            k_pq2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
+            k_pq2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info, out.task_diag);
         }
         if (ev) hipEventRecord(ev[1], s);
         // each matrix goes to the wave or the workgroup eigensolver by its own order
@@ -1239,6 +1275,7 @@ void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts,
             a.grid = n_tasks;
             crossed_qc(eig_columns_per_thread(2 * max_n), a);
         }
+        if (n_tasks > 0 && diag) k_eigres2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info, out.task_diag);
         if (ev) hipEventRecord(ev[2], s);
     }
     if (part & 2) {

@@ -152,10 +152,75 @@ struct SmatCtx {
     int N;
     Bump bump;
     int fail;       // 1 numerical, 2 workspace
+    int n_solve = 0;        // dense linear solves (Result::n_dense_solve)
+    bool diag = false;      // compute backward errors (Options::diagnostics)
+    double max_berr = 0.0;  // largest backward error so far (Result::max_solve_backward_error)
     DEV void check() {
         if (bump.overflow) fail = 2;
     }
 };
+
+// max that keeps a NaN (a NaN residual or error marks a failed quantity; dt::max_keep_nan)
+DEV double max_keep_nan(double a, double b) { return (isnan(a) || isnan(b)) ? __builtin_nan("") : fmax(a, b); }
+
+// Backward error ||A X - B||_F / (||A||_F ||X||_F + ||B||_F) of a solve (solve_backward_error,
+// lu.hpp:452-459).  A is n x n, X and B n x nrhs; R (n x nrhs) is scratch.
+DEV double wg_backward_error(int n, const cd* A, const cd* X, const cd* B, int nrhs, cd* R) {
+    const int64_t nn = (int64_t)n * n, nr = (int64_t)n * nrhs;
+    wg_copy(n, nrhs, B, n, R, n);
+    wg_gemm<GEMM_SUB, false>(n, nrhs, n, A, n, X, n, R, n);
+    double sa = 0.0, sx = 0.0, sb = 0.0, sr = 0.0;
+    for (int64_t e = threadIdx.x; e < nn; e += NT) sa += norm2(A[e]);
+    for (int64_t e = threadIdx.x; e < nr; e += NT) {
+        sx += norm2(X[e]);
+        sb += norm2(B[e]);
+        sr += norm2(R[e]);
+    }
+    sa = block_sum(sa);
+    sx = block_sum(sx);
+    sb = block_sum(sb);
+    sr = block_sum(sr);
+    const double denom = sqrt(sa) * sqrt(sx) + sqrt(sb), resid = sqrt(sr);
+    if (denom == 0.0) return resid == 0.0 ? 0.0 : __builtin_inf();
+    return resid / denom;
+}
+
+// Eigen-pair residual max_j ||A v_j - l_j v_j||_2 / (||A||_F ||v_j||_2) (eig_residual, eig.hpp:38-69):
+// A and V n x n, w the n eigenvalues, AV (n x n) scratch.  A zero vector counts as +inf, a NaN as NaN.
+DEV double wg_eig_residual(int n, const cd* A, const cd* V, const cd* w, cd* AV) {
+    const int64_t nn = (int64_t)n * n;
+    wg_gemm<GEMM_SET, false>(n, n, n, A, n, V, n, AV, n);
+    double sa = 0.0;
+    for (int64_t e = threadIdx.x; e < nn; e += NT) sa += norm2(A[e]);
+    const double anorm = sqrt(block_sum(sa));
+    double worst = 0.0;
+    for (int j = wave_id(); j < n; j += NWAVE) {
+        const cd* v = V + (int64_t)j * n;
+        const cd* av = AV + (int64_t)j * n;
+        const cd lam = w[j];
+        double vs = 0.0, ds = 0.0;
+        for (int i = lane_id(); i < n; i += 64) {
+            vs += norm2(v[i]);
+            ds += norm2(av[i] - lam * v[i]);
+        }
+        vs = sqrt(wave_sum(vs));
+        ds = sqrt(wave_sum(ds));
+        double value;
+        if (vs == 0.0) value = __builtin_inf();
+        else if (anorm == 0.0) value = ds == 0.0 ? 0.0 : __builtin_inf();
+        else value = (ds / anorm) / vs;
+        worst = max_keep_nan(worst, value);
+    }
+    // block max that keeps a NaN: NaN -> +inf marker through block_max, then back
+    __shared__ int s_nan;
+    if (threadIdx.x == 0) s_nan = 0;
+    __syncthreads();
+    if (isnan(worst)) s_nan = 1;
+    const double m = block_max(isnan(worst) ? 0.0 : worst);
+    const int any_nan = s_nan;
+    __syncthreads();
+    return any_nan ? __builtin_nan("") : m;
+}
 
 // zero / diagonal / dense operator of order N (rcwa1d.hpp:549-629)
 enum { OP_ZERO = 0, OP_DIAG = 1, OP_DENSE = 2 };
@@ -275,31 +340,67 @@ DEV cd* op_to_dense(SmatCtx& c, DOp a) {
     return m;
 }
 
-// B <- A^{-1} B (A is preserved).  Singular -> numerical failure.
+// B <- A^{-1} B (A is preserved).  Singular -> numerical failure.  Every call is one dense solve of
+// the reference (linear_solve, rcwa1d.hpp:248-281); with diagnostics its backward error is taken from
+// copies of A and B released right after.
 DEV void dense_solve(SmatCtx& c, const cd* A, cd* B, int nrhs) {
     const int N = c.N;
+    ++c.n_solve;
     cd* lu = c.bump.tmp((int64_t)N * N);
     int* piv = reinterpret_cast<int*>(c.bump.tmp(N));
     c.check();
     if (c.fail) return;
+    const int64_t mark = c.bump.lo;
+    cd* b0 = nullptr;
+    if (c.diag) {
+        b0 = c.bump.tmp((int64_t)N * nrhs);
+        c.check();
+        if (c.fail) return;
+        wg_copy(N, nrhs, B, N, b0, N);
+    }
     wg_copy(N, N, A, N, lu, N);
     if (wg_lu(N, lu, N, piv)) {
         c.fail = 1;
         return;
     }
     wg_lu_solve(N, lu, N, piv, B, N, nrhs);
+    if (c.diag) {
+        cd* r = c.bump.tmp((int64_t)N * nrhs);
+        c.check();
+        if (c.fail) return;
+        c.max_berr = max_keep_nan(c.max_berr, wg_backward_error(N, A, B, b0, nrhs, r));
+        c.bump.lo = mark;
+    }
 }
 // B <- A^{-1} B, destroying A.
 DEV void dense_solve_inplace(SmatCtx& c, cd* A, cd* B, int nrhs) {
     const int N = c.N;
+    ++c.n_solve;
     int* piv = reinterpret_cast<int*>(c.bump.tmp(N));
     c.check();
     if (c.fail) return;
+    const int64_t mark = c.bump.lo;
+    cd *a0 = nullptr, *b0 = nullptr;
+    if (c.diag) {
+        a0 = c.bump.tmp((int64_t)N * N);
+        b0 = c.bump.tmp((int64_t)N * nrhs);
+        c.check();
+        if (c.fail) return;
+        wg_copy(N, N, A, N, a0, N);
+        wg_copy(N, nrhs, B, N, b0, N);
+    }
     if (wg_lu(N, A, N, piv)) {
         c.fail = 1;
         return;
     }
     wg_lu_solve(N, A, N, piv, B, N, nrhs);
+    if (c.diag) {
+        cd* r = c.bump.tmp((int64_t)N * nrhs);
+        c.check();
+        if (c.fail) return;
+        c.max_berr = max_keep_nan(c.max_berr, wg_backward_error(N, a0, B, b0, nrhs, r));
+        c.bump.lo = mark;
+    }
 }
 
 // Both sides homogeneous: per-order Fresnel blocks (rcwa1d.hpp:649-669).

@@ -23,13 +23,13 @@
 // kernel launchers (rcwa1d.hip, dense.hip)
 void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                   int part, hipEvent_t* ev);
+                   int part, int flags, hipEvent_t* ev);
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    int part, hipEvent_t* ev);
+                    int part, int flags, hipEvent_t* ev);
 void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    int part, hipEvent_t* ev);
+                    int part, int flags, hipEvent_t* ev);
 void launch_zgemm_batched(hipStream_t s, int count, const DenseItem* items, cd* arena);
 void launch_zgesv_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info);
 void launch_zgeev_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info, int max_iter,
@@ -73,6 +73,8 @@ void invalid(const std::string& m) { throw Failure{SRCWA_EINVAL, m}; }
 }  // namespace
 
 void srcwa_set_last_error(const std::string& msg) { g_error = msg; }
+
+static_assert(int(SRCWA_PLAN_DIAGNOSTICS) == int(PLAN_DIAGNOSTICS), "plan flag of the C-ABI and the kernels");
 
 // A device arena carved first-fit into 256-byte aligned blocks.  Plans created on a context that has
 // a pool take every device buffer from it, so many plans can come and go (a request server) without
@@ -159,6 +161,13 @@ struct DevBuf {
 };
 
 int64_t align16(int64_t x) { return (x + 15) & ~int64_t(15); }
+
+// pool bytes of a plan's diagnostics buffers (per-instance records, per-task records, residuals)
+int64_t diag_bytes(int n_inst, int n_tasks) {
+    const auto b = [](size_t x) { return static_cast<int64_t>((x + 255) & ~size_t(255)); };
+    const size_t t = static_cast<size_t>(std::max(1, n_tasks));
+    return b(size_t(DIAG_WORDS) * n_inst * sizeof(double)) + b(t * sizeof(TaskDiag)) + b(2 * t * sizeof(double));
+}
 
 // ---- validation (rcwa1d.hpp:188-229) ------------------------------------------------------------
 bool finite_c(double re, double im) { return std::isfinite(re) && std::isfinite(im); }
@@ -376,6 +385,10 @@ struct srcwa_plan {
     DevBuf<double> R, T, DEr, DEt;
     DevBuf<double> geometry;      // conical / crossed: GEO_WORDS per order
     DevBuf<int> status, eig_info;
+    DevBuf<double> diag;            // DIAG_WORDS per instance
+    DevBuf<TaskDiag> task_diag;     // per layer task
+    DevBuf<double> item_res;        // conical: two eigen residuals per layer task
+    int flags = 0;                  // SRCWA_PLAN_* flags (from the context at creation)
     // 5 timing events per chunk: front start, after assembly, after eigensolves (front stream); back
     // start, back end (back stream); then the final event on the front stream
     std::vector<hipEvent_t> ev;
@@ -416,6 +429,14 @@ struct srcwa_plan {
         geometry.pool = p;
         status.pool = p;
         eig_info.pool = p;
+        diag.pool = p;
+        task_diag.pool = p;
+        item_res.pool = p;
+    }
+    void alloc_diag() {
+        diag.alloc(size_t(DIAG_WORDS) * n_inst);
+        task_diag.alloc(std::max(1, n_tasks));
+        item_res.alloc(2 * size_t(std::max(1, n_tasks)));
     }
     void make_events() {
         const size_t c = chunks.size();
@@ -431,7 +452,8 @@ struct srcwa_plan {
     }
     int64_t bytes() const {
         return insts.bytes() + insts2d.bytes() + tasks.bytes() + stacks.bytes() + layouts.bytes() + ws.bytes() + task_info.bytes() +
-               R.bytes() + T.bytes() + DEr.bytes() + DEt.bytes() + status.bytes() + eig_info.bytes() + geometry.bytes();
+               R.bytes() + T.bytes() + DEr.bytes() + DEt.bytes() + status.bytes() + eig_info.bytes() + geometry.bytes() +
+               diag.bytes() + task_diag.bytes() + item_res.bytes();
     }
 };
 
@@ -468,21 +490,22 @@ void enqueue(srcwa_plan* plan, bool timing) {
     auto launch = [&](size_t c, hipStream_t st, int part) {
         const PlanChunk& ch = plan->chunks[c];
         Out1D out{plan->R.p, plan->T.p, plan->DEr.p, plan->DEt.p, plan->status.p + ch.i0, plan->eig_info.p + ch.i0,
-                  plan->kind == 1 ? nullptr : plan->geometry.p};
+                  plan->kind == 1 ? nullptr : plan->geometry.p, plan->diag.p + int64_t(DIAG_WORDS) * ch.i0,
+                  plan->task_diag.p + ch.t0, plan->item_res.p + 2 * int64_t(ch.t0)};
         cd* ws = plan->ws.p + (nc > 1 ? static_cast<int64_t>(c % 2) * plan->region_words : 0);
         hipEvent_t* ev = timing ? plan->ev.data() + 5 * c : nullptr;
         const int ni = ch.i1 - ch.i0, nt = ch.t1 - ch.t0;
         if (plan->kind == 1)
             launch_planar(st, ni, nt, plan->insts.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p, plan->layouts.p,
-                          ws, plan->task_info.p + ch.t0, out, plan->max_iter, plan->min_n, plan->max_n, part, ev);
+                          ws, plan->task_info.p + ch.t0, out, plan->max_iter, plan->min_n, plan->max_n, part, plan->flags, ev);
         else if (plan->kind == 2)
             launch_conical(st, ni, nt, plan->insts.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p, plan->layouts.p,
                            ws, plan->task_info.p + 4 * ch.t0, out, plan->max_iter, plan->min_n, plan->max_n, part,
-                           ev);
+                           plan->flags, ev);
         else
             launch_crossed(st, ni, nt, plan->insts2d.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p,
                            plan->layouts.p, ws, plan->task_info.p + 4 * ch.t0, out, plan->max_iter, plan->min_n,
-                           plan->max_n, part, ev);
+                           plan->max_n, part, plan->flags, ev);
         hip_check(hipGetLastError(), "kernel launch");
     };
     for (size_t c = 0; c < nc; ++c) {
@@ -565,6 +588,14 @@ int srcwa_pool_destroy(srcwa_pool* pool) {
     });
 }
 
+int srcwa_context_set_plan_flags(srcwa_context* ctx, int flags) {
+    return guarded([&] {
+        if (!ctx) invalid("srcwa_context_set_plan_flags: null context");
+        if (flags & ~int(SRCWA_PLAN_DIAGNOSTICS)) invalid("srcwa_context_set_plan_flags: unknown flag");
+        ctx->plan_flags = flags;
+    });
+}
+
 int64_t srcwa_pool_free_bytes(srcwa_pool* pool) {
     if (!pool) return -1;
     std::lock_guard<std::mutex> g(pool->lock);
@@ -597,6 +628,8 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         plan->ctx = ctx;
         plan->kind = 1;
         plan->n_inst = n_problems;
+        plan->flags = ctx->plan_flags;
+        const bool diag = plan->flags & PLAN_DIAGNOSTICS;
         const int64_t budget = max_workspace_bytes > 0 ? max_workspace_bytes : (int64_t(64) << 30);
         const int64_t budget_words = budget / int64_t(sizeof(cd));
 
@@ -631,13 +664,16 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
                 lay[layout_p(r)] = w;
                 w += align16(N);
             }
+            // with diagnostics each grating layer keeps its matrix and a solve scratch (2 N^2), and the
+            // recursion's solves copies of A, B and the residual (3 N^2)
+            const int64_t NN1 = int64_t(N) * N;
             for (int j = 0; j < L; ++j)
                 if (li[j].grating) {
                     lay[layout_layer(L, j)] = w;
-                    w += align16(slot_words(N));
+                    w += align16(slot_words(N) + (diag ? 2 * NN1 : 0));
                 }
             const int64_t ws_lo = w;
-            w += cost.bump_words;
+            w += cost.bump_words + (diag ? 3 * NN1 + 64 : 0);
             // admission into the current chunk by workspace footprint
             if (i > cur.i0 && chunk_ws + w > budget_words) {
                 cur.i1 = i;
@@ -655,6 +691,7 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
             Inst1D& in = h_insts[i];
             in.M = M[i];
             in.N = N;
+            in.flags = plan->flags;
             plan->max_n = std::max(plan->max_n, N);
             plan->min_n = std::min(plan->min_n, N);
             in.pol = pol[i];
@@ -684,7 +721,8 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
             *need = b(h_insts.size() * sizeof(Inst1D)) + b(h_tasks.size() * sizeof(LayerTask)) +
                     b(h_stacks.size() * sizeof(double)) + b(h_layouts.size() * sizeof(int64_t)) +
                     b((plan->chunks.size() > 1 ? 2 : 1) * max_ws * sizeof(cd)) + b(std::max(1, plan->n_tasks) * sizeof(int)) +
-                    2 * b(2 * orders * sizeof(double)) + 2 * b(orders * sizeof(double)) + 2 * b(n_problems * sizeof(int));
+                    2 * b(2 * orders * sizeof(double)) + 2 * b(orders * sizeof(double)) + 2 * b(n_problems * sizeof(int)) +
+                    diag_bytes(n_problems, plan->n_tasks);
             return;
         }
         hipStream_t st = ctx->stream;
@@ -701,6 +739,7 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         plan->DEt.alloc(orders);
         plan->status.alloc(n_problems);
         plan->eig_info.alloc(n_problems);
+        plan->alloc_diag();
         plan->make_events();
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
         *out = plan.release();
@@ -742,6 +781,8 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
         plan->ctx = ctx;
         plan->kind = 2;
         plan->n_inst = n_problems;
+        plan->flags = ctx->plan_flags;
+        const bool diag = plan->flags & PLAN_DIAGNOSTICS;
         const int64_t budget = max_workspace_bytes > 0 ? max_workspace_bytes : (int64_t(64) << 30);
         const int64_t budget_words = budget / int64_t(sizeof(cd));
         std::vector<Inst1D> h_insts(n_problems);
@@ -789,13 +830,16 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
                     w += align16(2 * N);
                 }
             }
+            // with diagnostics: copies of A and B C plus two solve scratch matrices per layer (4 N^2), and
+            // the recursion's copies of A, B and the residual at order 2N (3 (2N)^2)
+            const int64_t NN1 = int64_t(N) * N;
             for (int j = 0; j < L; ++j)
                 if (general[j + 1]) {
                     lay[conical_slot(L, j)] = w;
-                    w += align16(conical_slot_words(N));
+                    w += align16(conical_slot_words(N) + (diag ? 4 * NN1 : 0));
                 }
             const int64_t ws_lo = w;
-            w += cost.bump_words;
+            w += cost.bump_words + (diag ? 12 * NN1 + 64 : 0);
             if (i > cur.i0 && chunk_ws + w > budget_words) {
                 cur.i1 = i;
                 cur.t1 = static_cast<int>(h_tasks.size());
@@ -820,6 +864,7 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
             plan->max_n = std::max(plan->max_n, N);
             plan->min_n = std::min(plan->min_n, N);
             in.pol = 2;
+            in.flags = plan->flags;
             in.L = L;
             in.first_task = first_task - cur.t0;
             in.n_tasks = static_cast<int>(h_tasks.size()) - first_task;
@@ -846,7 +891,8 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
                     b(ps.words.size() * sizeof(double)) + b(h_layouts.size() * sizeof(int64_t)) +
                     b((plan->chunks.size() > 1 ? 2 : 1) * max_ws * sizeof(cd)) + b(4 * std::max(1, plan->n_tasks) * sizeof(int)) +
                     2 * b(8 * orders * sizeof(double)) + 2 * b(4 * orders * sizeof(double)) +
-                    b(GEO_WORDS * orders * sizeof(double)) + 2 * b(n_problems * sizeof(int));
+                    b(GEO_WORDS * orders * sizeof(double)) + 2 * b(n_problems * sizeof(int)) +
+                    diag_bytes(n_problems, plan->n_tasks);
             return;
         }
         hipStream_t st = ctx->stream;
@@ -864,6 +910,7 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
         plan->DEt.alloc(4 * orders);
         plan->status.alloc(n_problems);
         plan->eig_info.alloc(n_problems);
+        plan->alloc_diag();
         plan->make_events();
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
         *out = plan.release();
@@ -995,6 +1042,15 @@ int srcwa_plan_results(srcwa_plan* plan, double* R, double* T, double* DEr, doub
 }
 
 int64_t srcwa_plan_orders(const srcwa_plan* plan) { return plan ? plan->total_orders : -1; }
+
+int srcwa_plan_diagnostics(srcwa_plan* plan, double* out) {
+    return guarded([&] {
+        if (!plan || !out) invalid("srcwa_plan_diagnostics: null plan or output");
+        hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+        copy_out(out, plan->diag.p, size_t(DIAG_WORDS) * plan->n_inst * sizeof(double), hipMemcpyDeviceToHost,
+                 "D2H", plan);
+    });
+}
 
 int srcwa_plan_geometry(srcwa_plan* plan, double* out) {
     return guarded([&] {
@@ -1416,6 +1472,8 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         plan->ctx = ctx;
         plan->kind = 3;
         plan->n_inst = n_problems;
+        plan->flags = ctx->plan_flags;
+        const bool diag = plan->flags & PLAN_DIAGNOSTICS;
         const int64_t budget = max_workspace_bytes > 0 ? max_workspace_bytes : (int64_t(64) << 30);
         const int64_t budget_words = budget / int64_t(sizeof(cd));
         std::vector<Inst2D> h_insts(n_problems);
@@ -1470,6 +1528,7 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
                     lay[x2_big(L, j)] = big;
                     lay[x2_cuts(L, j)] = cut_off[s][j];
                     w += align16(3 * NN + big + 7 * int64_t(N));
+                    if (diag) w += align16(4 * NN);      // a copy of H for the eigen residual
                 }
             const int64_t ws_lo = w;
             w += cost.bump_words;
@@ -1501,6 +1560,7 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
             in.first_task = first_task - cur.t0;
             in.n_tasks = static_cast<int>(h_tasks.size()) - first_task;
             in.laurent = fz;
+            in.flags = plan->flags;
             in.wavelength = wavelength[i];
             in.theta_deg = theta_deg[i];
             in.phi_deg = phi_deg[i];
@@ -1525,7 +1585,8 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
                     b(h_stacks.size() * sizeof(double)) + b(h_layouts.size() * sizeof(int64_t)) +
                     b((plan->chunks.size() > 1 ? 2 : 1) * max_ws * sizeof(cd)) + b(4 * std::max(1, plan->n_tasks) * sizeof(int)) +
                     2 * b(8 * orders * sizeof(double)) + 2 * b(orders * sizeof(double)) +
-                    b(GEO_WORDS * orders * sizeof(double)) + 2 * b(n_problems * sizeof(int));
+                    b(GEO_WORDS * orders * sizeof(double)) + 2 * b(n_problems * sizeof(int)) +
+                    diag_bytes(n_problems, plan->n_tasks);
             return;
         }
         hipStream_t st = ctx->stream;
@@ -1543,6 +1604,7 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         plan->DEt.alloc(orders);
         plan->status.alloc(n_problems);
         plan->eig_info.alloc(n_problems);
+        plan->alloc_diag();
         plan->make_events();
         hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
         *out = plan.release();

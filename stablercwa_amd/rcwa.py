@@ -100,6 +100,13 @@ class PlanarBatch:
         return Result(orders=np.arange(-M, M + 1), R=res["R"][a:b].copy(), T=res["T"][a:b].copy(),
                       DEr=res["DEr"][a:b].copy(), DEt=res["DEt"][a:b].copy(), eig_info=int(res["eig_info"][i]))
 
+    def diagnostics(self) -> np.ndarray:
+        """Per problem [max_eig_residual, max_solve_backward_error, n_eig, n_dense_solve, n_schur] of the last
+        execute (residuals need a context created with plan_flags=PLAN_DIAGNOSTICS)."""
+        out = np.zeros((self.n, 5))
+        _lib.check(self.lib.srcwa_plan_diagnostics(self.handle, _lib.ptr(out)))
+        return out
+
     def stage_ms(self) -> np.ndarray:
         ms = np.zeros(4)
         _lib.check(self.lib.srcwa_plan_stage_ms(self.handle, _lib.ptr(ms)))

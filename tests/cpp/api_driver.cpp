@@ -4,6 +4,7 @@
 //
 //   planar  <pol> <M> <wavelength> <theta> <n> <flat record (n doubles)>
 //   sweep   <pol> <M> <points> (<wavelength> <theta>)* <n> <flat record>
+//   planar_diag <pol> <M> <wavelength> <theta> <n> <flat record>   (Options::diagnostics = true)
 //   conical <M> <wavelength> <theta> <phi> <n> <flat record>
 //   crossed <M1> <M2> <wavelength> <theta> <phi> <Es.re> <Es.im> <Ep.re> <Ep.im> <laurent> <n> <flat record>
 //
@@ -125,6 +126,18 @@ int main() {
                 in >> pol >> M >> wl >> th;
                 const auto st = planar_stack(read_record(in));
                 print_planar(rcwa::solve(st, wl, th, M, pol ? rcwa::Polarization::TM : rcwa::Polarization::TE));
+            } else if (kind == "planar_diag") {
+                int pol, M;
+                double wl, th;
+                in >> pol >> M >> wl >> th;
+                const auto st = planar_stack(read_record(in));
+                rcwa::Options opt;
+                opt.diagnostics = true;
+                const rcwa::Result r = rcwa::solve(st, wl, th, M, pol ? rcwa::Polarization::TM : rcwa::Polarization::TE, opt);
+                print_planar(r);
+                std::printf("diag %.17g %.17g %d %d %.17g %.17g %.17g %.17g\n", r.max_eig_residual,
+                            r.max_solve_backward_error, r.n_eig, r.n_dense_solve, r.timing.total, r.timing.eig,
+                            r.timing.linear, r.timing.product);
             } else if (kind == "sweep") {
                 int pol, M, P;
                 in >> pol >> M >> P;

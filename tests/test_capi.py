@@ -41,6 +41,11 @@ def test_cpp_header_compiles_standalone(tmp_path):
                    "  rcwa::Stack s; s.nII = 1.46; s.period = 0.574;\n"
                    "  s.layers.push_back(rcwa::Layer::grating(0.38, {0.0, 0.172, 0.574}, {1.46, 1.0}));\n"
                    "  rcwa::Options o; (void)o; (void)rcwa::jones_from_angles_2d(90.0);\n"
+                   "  // the reference's solve2d signature, capture argument included (rcwa2d.hpp:1359-1360)\n"
+                   "  auto f = [](const rcwa::Stack2D& st) { return rcwa::solve2d(st, 1.0, 0.0, 0.0, 1, 1,\n"
+                   "      rcwa::jones_from_angles_2d(90.0), rcwa::Options2D{},\n"
+                   "      static_cast<rcwa::detail2d::FieldCapture2D*>(nullptr)); };\n"
+                   "  (void)f;\n"
                    "  return rcwa::harmonic_index_2d(0, 0, 2, 2) == 12 ? 0 : 1;\n}\n")
     r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
                         "-I", os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)

@@ -136,3 +136,23 @@ def test_cpp_error_conventions():
     assert b[0][0] == "invalid_argument rcwa: grating edges must span exactly one period"
     assert b[1][0] == "invalid_argument rcwa: incidence angle must lie strictly between -90 and 90 degrees"
     assert b[2][0].startswith("unsupported ")
+
+
+def test_cpp_options_diagnostics(ref_lib):
+    """rcwa::solve with Options::diagnostics: Result::max_eig_residual, max_solve_backward_error, n_eig,
+    n_dense_solve as the reference's (counts exact), and the timing split filled (rcwa1d.hpp:141-163)."""
+    from oracle import ref
+    refs = load_golden("planar_ref.json")["cases"][:6]
+    lines = [f"planar_diag {c['pol']} {c['M']} {c['wavelength']!r} {c['theta']!r} {_rec(c['stack'])}" for c in refs]
+    for c, b in zip(refs, _drive(lines)):
+        f = _fields(b)
+        assert f["head"][0] == "ok", (c["name"], b[0])
+        res, berr, n_eig, n_solve, total, t_eig, t_lin, t_prod = f["diag"]
+        o = ref.solve1d_ex(ref_lib, np.asarray(c["stack"]), c["wavelength"], c["theta"], c["M"], c["pol"])
+        assert (int(n_eig), int(n_solve)) == (o["n_eig"], o["n_dense_solve"]), c["name"]
+        if o["n_eig"]:
+            assert 0.0 < res <= 1e-14 and res <= 10 * max(o["max_eig_residual"], 1e-16), (c["name"], res)
+        if o["n_dense_solve"]:
+            assert 0.0 < berr <= 1e-14, (c["name"], berr)
+        assert total > 0.0 and t_eig >= 0.0 and t_lin > 0.0 and t_prod == 0.0
+        assert np.max(np.abs(f["DEr"] - np.asarray(c["DEr"]))) <= 1e-10
