@@ -192,6 +192,11 @@ int srcwa_solve1d_batch(srcwa_context* ctx, int n_stacks, const double* stacks, 
 int srcwa_zgeev_batched(srcwa_context* ctx, int count, const int* n, const double* A, double* w, double* V, int* info);
 int srcwa_zgesv_batched(srcwa_context* ctx, int count, const int* n, const int* nrhs, const double* A, const double* B,
                         double* X, int* info);
+/* srcwa_zgeev_batched with the QR iteration limit max_iter per eigenvalue (EigOptions::max_iter,
+ * eig_types.hpp:29; 30 in srcwa_zgeev_batched): info_i = k > 0 when eigenvalue k (1-based) did not converge,
+ * and V_i is then not meaningful.  The custom_eig adapter of include/stablercwa_amd/hooks.hpp binds it. */
+int srcwa_zgeev_batched_iter(srcwa_context* ctx, int count, const int* n, const double* A, double* w, double* V,
+                             int* info, int max_iter);
 int srcwa_zgemm_batched(srcwa_context* ctx, int count, const int* m, const int* n, const int* k, const double* A,
                         const double* B, double* C);
 

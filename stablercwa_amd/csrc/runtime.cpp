@@ -1209,8 +1209,14 @@ int srcwa_zgesv_batched(srcwa_context* ctx, int count, const int* n, const int* 
 }
 
 int srcwa_zgeev_batched(srcwa_context* ctx, int count, const int* n, const double* A, double* w, double* V, int* info) {
+    return srcwa_zgeev_batched_iter(ctx, count, n, A, w, V, info, 30);
+}
+
+int srcwa_zgeev_batched_iter(srcwa_context* ctx, int count, const int* n, const double* A, double* w, double* V,
+                             int* info, int max_iter) {
     return dense_batch(ctx, [&](hipStream_t s) {
         if (count < 1 || !n || !A || !w || !V || !info) invalid("srcwa_zgeev_batched: empty or null inputs");
+        if (max_iter < 1 || max_iter > 100000) invalid("srcwa_zgeev_batched: max_iter must lie in 1..100000");
         std::vector<DenseItem> items(count);
         int64_t off = 0;
         for (int i = 0; i < count; ++i) {
@@ -1239,7 +1245,7 @@ int srcwa_zgeev_batched(srcwa_context* ctx, int count, const int* n, const doubl
             hip_check(hipMemcpyAsync(arena.p + items[i].a, A + 2 * ao, as * 16, hipMemcpyHostToDevice, s), "H2D");
             ao += as;
         }
-        launch_zgeev_batched(s, count, d_items.p, arena.p, d_info.p, 30, *std::max_element(n, n + count));
+        launch_zgeev_batched(s, count, d_items.p, arena.p, d_info.p, max_iter, *std::max_element(n, n + count));
         hip_check(hipGetLastError(), "kernel launch");
         ao = 0;
         int64_t wo = 0;
