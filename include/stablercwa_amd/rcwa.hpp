@@ -11,10 +11,11 @@
 //
 // Error behaviour: invalid input throws std::invalid_argument with the reference's message; a numerical
 // failure returns NaN arrays with eig_info set (planar, conical) or status NumericalFailure (crossed);
-// options this build does not provide (FFT Fourier layers, custom hooks, LR eigensolver, non-pivoting or
-// QR linear solvers, homogeneous_fast_path = false) throw srcwa_amd::Unsupported; a HIP runtime error
-// throws std::runtime_error.  sweep* run every point in one batched GPU launch; results keep the input
-// order and each result is bitwise the same as its single solve.
+// options this build does not provide (custom hooks -- see stablercwa_amd/hooks.hpp for the other
+// direction --, LR eigensolver, non-pivoting or QR linear solvers, homogeneous_fast_path = false, near-field
+// capture) throw srcwa_amd::Unsupported; a HIP runtime error throws std::runtime_error.  FFT-sampled
+// grating layers (FourierMethod::FFT) are computed on the GPU.  sweep* run every point in one batched GPU
+// launch; results keep the input order and each result is bitwise the same as its single solve.
 #pragma once
 
 #include <array>

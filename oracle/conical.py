@@ -80,10 +80,12 @@ def is_uniform(layer):
 def assemble(layer, period, kx, M):
     edges = [0.0, period] if layer.homogeneous else layer.edges
     vals = [layer.n] if layer.homogeneous else layer.values
-    e = convolution_matrix(edges, [v * v for v in vals], period, M)
+    method = 0 if layer.homogeneous else layer.fourier_method
+    e = convolution_matrix(edges, [v * v for v in vals], period, M, method, layer.fft_samples)
     N = 2 * M + 1
     inv_eps = _solve(e, np.eye(N, dtype=complex))
-    c = _solve(convolution_matrix(edges, [1.0 / (v * v) for v in vals], period, M), np.eye(N, dtype=complex))
+    c = _solve(convolution_matrix(edges, [1.0 / (v * v) for v in vals], period, M, method, layer.fft_samples),
+               np.eye(N, dtype=complex))
     b = (kx[:, None] * inv_eps) * kx[None, :]
     b[np.diag_indices(N)] -= 1.0
     bc = b @ c

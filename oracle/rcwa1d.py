@@ -13,7 +13,8 @@ Follows include/stablercwa/rcwa1d.hpp of the reference:
 The eigensolve is numpy's LAPACK zgeev instead of CORTH+COMQR2: efficiencies do not depend on the
 order or phase of the eigenpairs (the reference's own test_rcwa1d.cpp:885-898 shuffles and rescales
 them), so any backward-stable eigensolver gives the same efficiencies to rounding. FFT coefficient
-layers (fourier.hpp) are not restated; callers use analytic layers.
+layers follow fourier.hpp:26-83 (``resolve_fft_samples``, ``fourier_coefficients_fft``) with numpy's FFT
+in place of the vendored pocketfft (both compute the same DFT).
 """
 from __future__ import annotations
 
@@ -47,9 +48,57 @@ def fourier_coefficients(edges, values, period, hmax):
     return c
 
 
-def convolution_matrix(edges, values, period, M):
-    """[[f]](m, p) = c_{m-p} (rcwa1d.hpp:391-407)."""
-    c = fourier_coefficients(edges, values, period, 2 * M)
+MAX_FFT_SAMPLES = 4194304
+
+
+def resolve_fft_samples(M, samples=0):
+    """The FFT sample count (fourier.hpp:26-44): fft_samples, or the smallest power of two >= max(8192,
+    64 (2M+1)); more than 4M samples are needed."""
+    if M > (MAX_FFT_SAMPLES - 1) // 4:
+        raise ValueError("rcwa: FFT truncation requires more than 4194304 samples")
+    if samples == 0:
+        target = max(8192, 64 * (2 * M + 1))
+        if target > MAX_FFT_SAMPLES:
+            raise ValueError("rcwa: automatic FFT exceeds 4194304 samples")
+        samples = 1
+        while samples < target:
+            samples *= 2
+    if samples <= 4 * M:
+        raise ValueError("rcwa: fft_samples must be at least 4*M+1")
+    return samples
+
+
+def fourier_coefficients_fft(edges, values, period, hmax, samples):
+    """c_h, h = -hmax..hmax, from samples at the cell midpoints (j + 1/2) period / N: a forward DFT / N,
+    then the phase of the origin and of the half cell (fourier.hpp:46-83)."""
+    e = np.asarray(edges, dtype=np.float64)
+    v = np.asarray(values, dtype=np.complex128)
+    j = np.arange(samples, dtype=np.float64)
+    u = (j + 0.5) / samples * period
+    seg = np.zeros(samples, dtype=np.int64)
+    for k in range(1, len(v)):                       # the reference's walk: u >= edges[k] - edges[0]
+        seg += (u >= e[k] - e[0]).astype(np.int64)
+    spectrum = np.fft.fft(v[seg]) / samples
+    origin = math.fmod(e[0], period)
+    if origin < 0:
+        origin += period
+    offset = origin / period + 0.5 / samples
+    c = np.zeros(2 * hmax + 1, dtype=np.complex128)
+    for h in range(-hmax, hmax + 1):
+        b = samples + h if h < 0 else h
+        arg = -2.0 * PI * h * offset
+        c[h + hmax] = spectrum[b] * (math.cos(arg) + 1j * math.sin(arg))
+    return c
+
+
+def convolution_matrix(edges, values, period, M, method=0, samples=0):
+    """[[f]](m, p) = c_{m-p} (rcwa1d.hpp:391-407), by the layer's Fourier method."""
+    if method == 1:
+        if not np.all(np.isfinite(np.asarray(values, dtype=np.complex128))):
+            raise NumericalFailure()
+        c = fourier_coefficients_fft(edges, values, period, 2 * M, resolve_fft_samples(M, samples))
+    else:
+        c = fourier_coefficients(edges, values, period, 2 * M)
     n = 2 * M + 1
     idx = np.arange(n)[:, None] - np.arange(n)[None, :] + 2 * M
     return c[idx]
@@ -75,12 +124,13 @@ def assemble_layer(layer, period, kx, M, pol):
     vals = [layer.n] if layer.homogeneous else layer.values
     eps = [v * v for v in vals]
     inv = [1.0 / (v * v) for v in vals]
-    e = convolution_matrix(edges, eps, period, M)
+    method = 0 if layer.homogeneous else layer.fourier_method
+    e = convolution_matrix(edges, eps, period, M, method, layer.fft_samples)
     if pol == 0:
         a = -e
         a[np.diag_indices_from(a)] += kx * kx
         return a, None
-    inv_eps = convolution_matrix(edges, inv, period, M)
+    inv_eps = convolution_matrix(edges, inv, period, M, method, layer.fft_samples)
     b = _solve(e, np.diag(kx.astype(np.complex128))) * kx[:, None]
     b[np.diag_indices_from(b)] -= 1.0
     return _solve(inv_eps, b), inv_eps

@@ -44,6 +44,16 @@ struct Inst1D {
     int64_t ws_lo, ws_hi;      // bump region of the S-matrix recursion (cd units)
 };
 
+// Samples of an FFT layer (resolve_fft_samples, fourier.hpp:26-44, for inputs the host has validated):
+// fft_samples itself, or the smallest power of two >= max(8192, 64 (2M + 1)).
+__host__ __device__ inline int fft_samples_for(int M, int samples) {
+    if (samples > 0) return samples;
+    const long long target = 64LL * (2LL * M + 1) > 8192LL ? 64LL * (2LL * M + 1) : 8192LL;
+    int n = 1;
+    while (n < target) n *= 2;
+    return n;
+}
+
 // One layer task: a grating layer of an instance that needs an eigensolve.
 struct LayerTask {
     int inst, layer;

@@ -186,9 +186,10 @@ __global__ __launch_bounds__(NT) void k_assemble_conical(const Inst1D* __restric
     const bool diag = in.flags & PLAN_DIAGNOSTICS;
     cd* D0 = ws + lay[conical_slot(L, tk.layer)] + conical_slot_words(N);
     cd *D1 = D0 + NN, *D2 = D0 + 2 * NN, *D3 = D0 + 3 * NN;
+    cd* F = D0 + (diag ? 4 * NN : 0);          // FFT layers: 2 nfft sample scratch
     double berr = 0.0;
     int fail = 0;
-    fourier_coefficients(lv, st[0], 2 * M, false, coef);
+    fail |= layer_coefficients(lv, st[0], M, false, coef, F);
     toeplitz(coef, N, s.A);                    // [[eps]]
     wg_copy(N, N, s.A, N, s.Z1, N);
     fail |= wg_lu(N, s.Z1, N, s.piv);
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(NT) void k_assemble_conical(const Inst1D* __restric
         wg_identity(N, D2, N);
         berr = max_keep_nan(berr, wg_backward_error(N, s.A, s.IE, D2, N, D3));
     }
-    fourier_coefficients(lv, st[0], 2 * M, true, coef);
+    fail |= layer_coefficients(lv, st[0], M, true, coef, F);
     toeplitz(coef, N, s.Z2);                   // [[1/eps]]
     if (diag) wg_copy(N, N, s.Z2, N, D0, N);
     fail |= wg_lu(N, s.Z2, N, s.piv);
@@ -419,7 +420,8 @@ DEV int schur_modes(const Inst1D& in, const double* st, const int64_t* lay, cd* 
     cd* E4 = E3 + nn;
     // A = Kx^2 - [[eps]] again (the eigensolve overwrote it): into the Z1 slot
     const LayerView lv = layer_at(st, (int)lay[conical_orig(in.L, layer)]);
-    fourier_coefficients(lv, st[0], 2 * M, false, s.lam1);
+    cd* F = ws + lay[conical_slot(in.L, layer)] + conical_slot_words(N) + ((in.flags & PLAN_DIAGNOSTICS) ? 4 * NN : 0);
+    if (layer_coefficients(lv, st[0], M, false, s.lam1, F)) return 1;
     toeplitz(s.lam1, N, s.Z1);
     for (int64_t e = threadIdx.x; e < NN; e += NT) {
         const int i = (int)(e % N), j = (int)(e / N);

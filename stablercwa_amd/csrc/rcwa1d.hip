@@ -67,10 +67,11 @@ __global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ in
     const bool diag = in.flags & PLAN_DIAGNOSTICS;
     cd* D0 = base + slot_words(N);
     cd* D1 = D0 + NN;
+    cd* F = D0 + (diag ? 2 * NN : 0);          // FFT layers: 2 nfft sample scratch
     double berr = 0.0;
     int fail = 0;
     if (in.pol == 0) {
-        fourier_coefficients(lv, period, 2 * M, false, coef);
+        fail |= layer_coefficients(lv, period, M, false, coef, F);
         toeplitz(coef, N, S0);
         // A = Kx^2 - [[eps]]
         wg_map((int)NN,
@@ -83,16 +84,16 @@ __global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ in
                [&](int e, cd v) { stc(S0 + e, v); });
     } else {
         // S1 = [[eps]], S3 = [[1/eps]] (kept for V), S2 = diag(kx)
-        fourier_coefficients(lv, period, 2 * M, false, coef);
+        fail |= layer_coefficients(lv, period, M, false, coef, F);
         toeplitz(coef, N, S1);
-        fourier_coefficients(lv, period, 2 * M, true, coef);
+        fail |= layer_coefficients(lv, period, M, true, coef, F);
         toeplitz(coef, N, S3);
         wg_diag(N, kx, S2, N);
         fail |= wg_lu(N, S1, N, piv);
         wg_lu_solve(N, S1, N, piv, S2, N, N);
         if (diag) {
             // [[eps]] X = diag(kx): rebuild [[eps]] (S1 holds its factors) and the right-hand side
-            fourier_coefficients(lv, period, 2 * M, false, coef);
+            fail |= layer_coefficients(lv, period, M, false, coef, F);
             toeplitz(coef, N, D0);
             wg_diag(N, kx, D1, N);
             berr = max_keep_nan(berr, wg_backward_error(N, D0, S2, D1, N, S0));

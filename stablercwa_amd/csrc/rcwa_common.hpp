@@ -19,6 +19,8 @@ struct LayerView {
     int K;
     const double* edges;
     const double* vals;     // K complex, interleaved
+    int method;             // gratings: 0 analytic, 1 FFT (FourierMethod)
+    int samples;            // gratings: fft_samples (0 automatic)
 };
 
 struct LayerCursor {
@@ -36,6 +38,8 @@ struct LayerCursor {
             v.K = 1;
             v.edges = nullptr;
             v.vals = nullptr;
+            v.method = 0;
+            v.samples = 0;
             k += 2;
         } else {
             v.homogeneous = false;
@@ -43,6 +47,8 @@ struct LayerCursor {
             v.K = (int)p[k];
             v.edges = p + k + 1;
             v.vals = p + k + 2 + v.K;
+            v.method = (int)v.vals[2 * v.K];
+            v.samples = (int)v.vals[2 * v.K + 1];
             k += 1 + (v.K + 1) + 2 * v.K + 2;
         }
         return v;
@@ -88,6 +94,102 @@ DEV void fourier_coefficients(const LayerView& lv, double period, int hmax, bool
         out[hh] = sum;
     }
     __syncthreads();
+}
+
+// eps = n^2 (or 1/eps) of segment j of a grating
+DEV cd segment_value(const LayerView& lv, int j, bool inverse) {
+    const cd nj{lv.vals[2 * j], lv.vals[2 * j + 1]};
+    const cd v = nj * nj;
+    return inverse ? cdiv(cd{1.0, 0.0}, v) : v;
+}
+
+// c_h, h = -hmax..hmax, of a grating's eps (or independently sampled 1/eps) from nfft samples at the cell
+// midpoints u_j = (j + 1/2) / nfft * period: a forward DFT scaled by 1/nfft, then the phase of the grating's
+// origin and of the half cell, exp(-i 2 pi h (origin / period + 1 / (2 nfft))) (fourier.hpp:46-83).  A
+// power-of-two nfft takes a radix-2 Stockham FFT over buf[0, 2 nfft); any other nfft a direct DFT of the
+// 2 hmax + 1 bins that are needed.  Returns 1 when a sampled value is not finite (the reference's numerical
+// failure, rcwa1d.hpp:393-398).
+DEV int fft_coefficients(const LayerView& lv, double period, int hmax, bool inverse, int nfft, cd* buf, cd* out) {
+    const int K = lv.K;
+    const double e0 = lv.edges[0];
+    int bad = 0;
+    for (int k = threadIdx.x; k < K; k += NT) bad |= !finite(segment_value(lv, k, inverse));
+    if (block_max_int(bad)) return 1;
+    // the segment of u: the number of interior edges k with u >= edges[k] - edges[0] (a binary search
+    // over the same comparisons as the reference's walk)
+    auto sample = [&](int j) -> cd {
+        const double u = (static_cast<double>(j) + 0.5) / nfft * period;
+        int lo = 0, hi = K - 1;            // answer in [0, K-1]
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (u >= lv.edges[mid] - e0) lo = mid;
+            else hi = mid - 1;
+        }
+        return segment_value(lv, lo, inverse);
+    };
+    double origin = fmod(e0, period);
+    if (origin < 0) origin += period;
+    const double offset = origin / period + 0.5 / nfft;
+    const double scale = 1.0 / nfft;
+    if ((nfft & (nfft - 1)) == 0) {
+        cd* a = buf;
+        cd* b = buf + nfft;
+        for (int j = threadIdx.x; j < nfft; j += NT) a[j] = sample(j);
+        __syncthreads();
+        // radix-2 Stockham: stage Ns combines pairs of length-Ns transforms with twiddles exp(-i pi k / Ns)
+        const int half = nfft >> 1;
+        for (int Ns = 1; Ns < nfft; Ns <<= 1) {
+            for (int j = threadIdx.x; j < half; j += NT) {
+                const int k = j & (Ns - 1);
+                double sw, cw;
+                sincospi(-static_cast<double>(k) / Ns, &sw, &cw);
+                const cd v0 = a[j], v1 = a[j + half] * cd{cw, sw};
+                const int d = (j / Ns) * 2 * Ns + k;
+                b[d] = v0 + v1;
+                b[d + Ns] = v0 - v1;
+            }
+            __syncthreads();
+            cd* t = a;
+            a = b;
+            b = t;
+        }
+        for (int hh = threadIdx.x; hh <= 2 * hmax; hh += NT) {
+            const int h = hh - hmax;
+            const int bin = h < 0 ? nfft + h : h;
+            const double arg = -2.0 * kPi * h * offset;
+            double sa, ca;
+            sincos(arg, &sa, &ca);
+            out[hh] = (scale * a[bin]) * cd{ca, sa};
+        }
+    } else {
+        // X_h = sum_j x_j exp(-i 2 pi j h / nfft), the exponent reduced exactly (j h mod nfft) before sincospi
+        for (int hh = threadIdx.x; hh <= 2 * hmax; hh += NT) {
+            const int h = hh - hmax;
+            const long long hm = ((long long)h % nfft + nfft) % nfft;
+            cd sum{0.0, 0.0};
+            for (int j = 0; j < nfft; ++j) {
+                const long long r = (hm * j) % nfft;
+                double sw, cw;
+                sincospi(-2.0 * static_cast<double>(r) / nfft, &sw, &cw);
+                sum = sum + sample(j) * cd{cw, sw};
+            }
+            const double arg = -2.0 * kPi * h * offset;
+            double sa, ca;
+            sincos(arg, &sa, &ca);
+            out[hh] = (scale * sum) * cd{ca, sa};
+        }
+    }
+    __syncthreads();
+    return 0;
+}
+
+// The layer's Fourier coefficients by its method (convolution_matrix, rcwa1d.hpp:391-407): analytic, or
+// FFT-sampled for a grating with FourierMethod::FFT (fftbuf: 2 nfft scratch).  Returns 1 on failure.
+DEV int layer_coefficients(const LayerView& lv, double period, int M, bool inverse, cd* out, cd* fftbuf) {
+    if (!lv.homogeneous && lv.method == 1)
+        return fft_coefficients(lv, period, 2 * M, inverse, fft_samples_for(M, lv.samples), fftbuf, out);
+    fourier_coefficients(lv, period, 2 * M, inverse, out);
+    return 0;
 }
 
 // [[f]](m, p) = c_{m-p}; c holds 2N-1 coefficients with c_0 at index N-1.

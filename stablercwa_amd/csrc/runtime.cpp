@@ -181,7 +181,29 @@ struct LayerInfo {
     bool grating;
     bool uniform_values;   // a grating whose segment indices are all exactly equal
     double d;
+    int method = 0;        // gratings: 0 analytic, 1 FFT
+    int samples = 0;       // gratings: fft_samples (0 automatic)
 };
+
+constexpr int kMaxFFTSamples = 4194304;
+
+// resolve_fft_samples (fourier.hpp:26-44): the sample count of an FFT grating at truncation order M, with
+// the reference's messages for the settings it rejects.
+int resolve_fft_samples(int M, int samples) {
+    if (M > (kMaxFFTSamples - 1) / 4) invalid("rcwa: FFT truncation requires more than 4194304 samples");
+    if (samples == 0) {
+        const long long target = std::max(8192LL, 64LL * (2LL * M + 1));
+        if (target > kMaxFFTSamples) invalid("rcwa: automatic FFT exceeds 4194304 samples");
+    }
+    const int n = fft_samples_for(M, samples);
+    if (n <= 4 * M) invalid("rcwa: fft_samples must be at least 4*M+1");
+    return n;
+}
+
+// cd words of FFT sample scratch each grating layer of an instance at order M needs (0 when analytic)
+int64_t fft_scratch_words(const LayerInfo& li, int M) {
+    return li.grating && li.method == 1 ? 2 * int64_t(resolve_fft_samples(M, li.samples)) : 0;
+}
 
 // Parses and validates one planar stack record; returns the per-layer kinds.
 std::vector<LayerInfo> parse_stack1d(const double* p, int64_t avail, int M) {
@@ -212,9 +234,12 @@ std::vector<LayerInfo> parse_stack1d(const double* p, int64_t avail, int M) {
             need(1 + (K + 1) + 2 * K + 2);
             const double* e = p + k + 1;
             const double* v = e + K + 1;
-            const int method = static_cast<int>(v[2 * K]);
-            if (method != 0 && method != 1) invalid("rcwa: unknown Fourier method");
-            if (method == 1) throw Failure{SRCWA_EUNSUPPORTED, "rcwa: FFT Fourier coefficients are not provided by this build"};
+            // validate_fourier_settings (fourier.hpp:17-23)
+            const double method_word = v[2 * K], samples_word = v[2 * K + 1];
+            if (method_word != 0.0 && method_word != 1.0) invalid("rcwa: invalid Fourier method");
+            if (!(samples_word >= 0.0 && samples_word <= kMaxFFTSamples) || samples_word != std::floor(samples_word))
+                invalid("rcwa: fft_samples must be 0..4194304 (0 for automatic)");
+            const int method = static_cast<int>(method_word), samples = static_cast<int>(samples_word);
             if (!std::isfinite(d) || d < 0.0) invalid("rcwa: layer thickness must be finite and >= 0");
             for (int i = 0; i <= K; ++i)
                 if (!std::isfinite(e[i]) || (i > 0 && e[i] <= e[i - 1]))
@@ -227,7 +252,7 @@ std::vector<LayerInfo> parse_stack1d(const double* p, int64_t avail, int M) {
                 same = same && v[2 * i] == v[0] && v[2 * i + 1] == v[1];
             }
             k += 1 + (K + 1) + 2 * K + 2;
-            out.push_back({true, same, d});
+            out.push_back({true, same, d, method, samples});
         } else {
             invalid("rcwa: unknown layer kind");
         }
@@ -670,7 +695,7 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
             for (int j = 0; j < L; ++j)
                 if (li[j].grating) {
                     lay[layout_layer(L, j)] = w;
-                    w += align16(slot_words(N) + (diag ? 2 * NN1 : 0));
+                    w += align16(slot_words(N) + (diag ? 2 * NN1 : 0) + fft_scratch_words(li[j], M[i]));
                 }
             const int64_t ws_lo = w;
             w += cost.bump_words + (diag ? 3 * NN1 + 64 : 0);
@@ -836,7 +861,7 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
             for (int j = 0; j < L; ++j)
                 if (general[j + 1]) {
                     lay[conical_slot(L, j)] = w;
-                    w += align16(conical_slot_words(N) + (diag ? 4 * NN1 : 0));
+                    w += align16(conical_slot_words(N) + (diag ? 4 * NN1 : 0) + fft_scratch_words(all[eff[j]], M[i]));
                 }
             const int64_t ws_lo = w;
             w += cost.bump_words + (diag ? 12 * NN1 + 64 : 0);

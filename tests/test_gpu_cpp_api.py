@@ -128,14 +128,18 @@ def test_cpp_error_conventions():
     bad_edges[11] += 0.01                                  # last edge no longer spans one period
     fft = list(good["stack"])
     K = int(fft[8])
-    fft[9 + (K + 1) + 2 * K] = 1.0                         # fourier_method = FFT
+    fft[9 + (K + 1) + 2 * K] = 1.0                         # fourier_method = FFT (automatic samples)
+    few = list(fft)
+    few[10 + (K + 1) + 2 * K] = 80.0                       # fft_samples = 80 <= 4 M at M = 20
     lines = [f"planar 0 20 {good['wavelength']!r} {good['theta']!r} {_rec(bad_edges)}",
              f"planar 0 20 {good['wavelength']!r} 95.0 {_rec(good['stack'])}",
-             f"planar 0 20 {good['wavelength']!r} {good['theta']!r} {_rec(fft)}"]
+             f"planar 0 20 {good['wavelength']!r} {good['theta']!r} {_rec(fft)}",
+             f"planar 0 20 {good['wavelength']!r} {good['theta']!r} {_rec(few)}"]
     b = _drive(lines)
     assert b[0][0] == "invalid_argument rcwa: grating edges must span exactly one period"
     assert b[1][0] == "invalid_argument rcwa: incidence angle must lie strictly between -90 and 90 degrees"
-    assert b[2][0].startswith("unsupported ")
+    assert b[2][0].startswith("ok ")                       # FFT layers are computed
+    assert b[3][0] == "invalid_argument rcwa: fft_samples must be at least 4*M+1"
 
 
 def test_cpp_options_diagnostics(ref_lib):
