@@ -43,20 +43,51 @@ def mirror_stack():
     return Stack(1.0, layers, 1.46, 0.574)
 
 
-def cpu_baseline(stack, M, n_points):
-    """The reference's own solver (oracle/_ref) on this host's cores, or the numpy oracle when absent."""
+def host_cpu():
+    """What this host offers: logical CPUs, the ones this process may run on, the CPU model, and the
+    thread share one GPU job gets on these machines (OMP_NUM_THREADS, 16 on the GPU hosts)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or affinity
+    return dict(nproc=os.cpu_count(), affinity=affinity, model=model, share=min(share, affinity))
+
+
+def cpu_threads():
+    return max(1, host_cpu()["share"])
+
+
+def cpu_baseline(stack, M, n_points, one_thread_points=16):
+    """The reference's own solver (oracle/_ref) on this host's cores (rcwa::sweep over wavelengths), plus a
+    1-thread row; the numpy oracle on one thread when the reference library is absent."""
     lams = np.linspace(0.95, 1.15, n_points)
     thetas = np.full(n_points, 66.36)
     from oracle import ref
+    host = host_cpu()
     if ref.available():
         lib = ref.load()
-        cores = max(1, min(16, os.cpu_count() or 1))
+        cores = cpu_threads()
         t0 = time.perf_counter()
         for pol in (0, 1):
             ref.sweep1d(lib, stack.flat(), lams, thetas, M, pol, cores)
         dt = time.perf_counter() - t0
+        k = min(one_thread_points, n_points)
+        t1 = time.perf_counter()
+        for pol in (0, 1):
+            ref.sweep1d(lib, stack.flat(), lams[:k], thetas[:k], M, pol, 1)
+        dt1 = time.perf_counter() - t1
         return dict(value=2 * n_points / dt, unit="solves/s", cores=cores, kind="reference",
-                    sample=f"{n_points} wavelengths x TE+TM at M={M} (reference rcwa::sweep, {cores} threads)")
+                    sample=f"{n_points} wavelengths x TE+TM at M={M} (reference rcwa::sweep, {cores} threads)",
+                    one_thread=dict(value=2 * k / dt1, sample=f"{k} wavelengths x TE+TM, 1 thread"), host=host)
     from oracle import rcwa1d
     k = max(4, n_points // 16)
     t0 = time.perf_counter()
@@ -65,7 +96,95 @@ def cpu_baseline(stack, M, n_points):
             rcwa1d.solve(stack.flat(), float(l), 66.36, M, pol)
     dt = time.perf_counter() - t0
     return dict(value=2 * k / dt, unit="solves/s", cores=1, kind="port",
-                sample=f"{k} wavelengths x TE+TM at M={M} (numpy oracle, 1 thread)")
+                sample=f"{k} wavelengths x TE+TM at M={M} (numpy oracle, 1 thread)", host=host)
+
+
+def _strided(count, k):
+    """k instance indices spread evenly over a batch of `count` seeded instances (an unbiased sample of
+    the batch's mix of orders and layer counts)."""
+    return [int(i) for i in np.linspace(0, count - 1, k).round()]
+
+
+def cpu_conical(probs, k, k1):
+    """The reference on k instances of the conical batch (its thread pool) and on k1 of them on 1 thread."""
+    from oracle import ref
+    if not ref.available():
+        return None
+    lib = ref.load()
+    cores = cpu_threads()
+    out = dict(unit="solves/s", cores=cores, kind="reference", host=host_cpu())
+
+    def run(idx, threads):
+        t0 = time.perf_counter()
+        ref.solve_conical_many(lib, [probs[i].stack.flat() for i in idx], [probs[i].wavelength for i in idx],
+                               [probs[i].theta for i in idx], [probs[i].phi for i in idx], [probs[i].M for i in idx],
+                               threads)
+        return len(idx) / (time.perf_counter() - t0)
+    idx = _strided(len(probs), k)
+    out["value"] = run(idx, cores)
+    out["sample"] = f"{k} instances spread over the batch (reference solve_conical, {cores} threads)"
+    idx1 = _strided(len(probs), k1)
+    out["one_thread"] = dict(value=run(idx1, 1), sample=f"{k1} instances spread over the batch, 1 thread")
+    return out
+
+
+def cpu_crossed(probs, jones, k, k1):
+    """The reference on k instances of the pillar batch (its thread pool) and on k1 of them on 1 thread."""
+    from oracle import ref
+    if not ref.available():
+        return None
+    lib = ref.load()
+    cores = cpu_threads()
+    out = dict(unit="solves/s", cores=cores, kind="reference", host=host_cpu())
+    jj = [jones[0].real, jones[0].imag, jones[1].real, jones[1].imag]
+
+    def run(idx, threads):
+        t0 = time.perf_counter()
+        ref.solve2d_many(lib, [probs[i].stack.flat() for i in idx], [probs[i].wavelength for i in idx],
+                         [probs[i].theta for i in idx], [probs[i].phi for i in idx], [probs[i].M1 for i in idx],
+                         [probs[i].M2 for i in idx], [jj] * len(idx), threads)
+        return len(idx) / (time.perf_counter() - t0)
+    idx = _strided(len(probs), k)
+    out["value"] = run(idx, cores)
+    out["sample"] = f"{k} instances spread over the batch (reference solve2d, {cores} threads)"
+    idx1 = _strided(len(probs), k1)
+    out["one_thread"] = dict(value=run(idx1, 1), sample=f"{k1} instances spread over the batch, 1 thread")
+    return out
+
+
+def single_solve_latency(reps=40):
+    """Config-1 latency: the README structure (TE, M = 20, lambda 1.053, theta 66.36) solved alone, on
+    the GPU (a one-solve plan, inputs resident, execute + sync) and by the reference on one CPU thread."""
+    from stablercwa_amd import rcwa
+    from stablercwa_amd.types import Layer, Stack
+    st = Stack(1.0, [Layer.grating(0.38, [0.0, 0.172, 0.574], [1.46, 1.0]), Layer.uniform(0.115, 1.96)], 1.46, 0.574)
+    b = rcwa.PlanarBatch([st], [(0, 1.053, 66.36, 20, 0)])
+    b.execute()
+    b.sync()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        b.execute()
+        b.sync()
+        ts.append(time.perf_counter() - t0)
+    res = b.results()
+    b.close()
+    out = dict(workload="README grating + film, TE, M=20 (N=41), lambda 1.053, theta 66.36",
+               gpu_ms_p50=float(np.median(ts) * 1e3), gpu_ms_min=float(np.min(ts) * 1e3), reps=reps)
+    from oracle import ref
+    if ref.available():
+        lib = ref.load()
+        lib.ref_set_threads(1)
+        tc = []
+        for _ in range(reps // 2):
+            t0 = time.perf_counter()
+            o = ref.solve1d(lib, st.flat(), 1.053, 66.36, 20, 0)
+            tc.append(time.perf_counter() - t0)
+        out["cpu_ms_p50"] = float(np.median(tc) * 1e3)
+        out["cpu_threads"] = 1
+        out["max_abs_dDE_vs_reference"] = max(float(np.max(np.abs(res["DEr"] - o["DEr"]))),
+                                              float(np.max(np.abs(res["DEt"] - o["DEt"]))))
+    return out
 
 
 def measured_eig_traffic():
@@ -96,9 +215,15 @@ def _timed(batch, n):
                 total_tflops=float(fl.sum() / dt / 1e12))
 
 
-def conical_leg(count=2048):
-    """Uneven-size conical batch: 2048 seeded instances, M uniform in [20, 120]; oracle parity on two."""
-    from oracle import conical
+def _spread_by(key, count, k):
+    """k instance indices at evenly spaced ranks of key (smallest to largest order, say)."""
+    order = sorted(range(count), key=key)
+    return [order[int(i)] for i in np.linspace(0, count - 1, k).round()]
+
+
+def conical_leg(count=2048, with_cpu=True):
+    """Uneven-size conical batch: 2048 seeded instances, M uniform in [20, 120]; parity against the
+    reference on six instances spread over the M range (smallest to largest)."""
     from stablercwa_amd import rcwa
     from stablercwa_amd.workloads import conical_batch
     probs = conical_batch(count)
@@ -107,19 +232,34 @@ def conical_leg(count=2048):
     res = b.results()
     out["failed"] = int(np.sum(res["status"] != 0))
     d = 0.0
-    for i in sorted(range(count), key=lambda k: probs[k].M)[:2]:
+    sample = _spread_by(lambda k: probs[k].M, count, 6)
+    for i in sample:
         r = b.result(i, res)
-        o = conical.solve(probs[i].stack.flat(), probs[i].wavelength, probs[i].theta, probs[i].phi, probs[i].M)
+        o = _conical_reference(probs[i])
         d = max(d, float(np.max(np.abs(r.DEr - o["DEr"]))), float(np.max(np.abs(r.DEt - o["DEt"]))))
     out["max_abs_dDE_vs_oracle_sample"] = d
+    out["parity_sample_M"] = [int(probs[i].M) for i in sample]
+    out["within_1e-9"] = bool(d <= 1e-9)
     out["workload"] = "2048 seeded conical instances, M in [20,120], 1-4 grating layers, FP64"
     b.close()
+    if with_cpu:
+        out["cpu_baseline"] = cpu_conical(probs, 64, 8)
     return out
 
 
-def crossed_leg(count=512):
-    """2-D pillar batch: 512 seeded instances, M1 = M2 in {5, 7, 9}; oracle parity on one."""
-    from oracle import rcwa2d
+def _conical_reference(p):
+    """DE of one conical instance from the reference library (the numpy oracle where it is absent)."""
+    from oracle import ref
+    if ref.available():
+        o = ref.solve_conical_ex(ref.load(), p.stack.flat(), p.wavelength, p.theta, p.phi, p.M, diagnostics=False)
+        return dict(DEr=o["DEr"].reshape(-1, 4), DEt=o["DEt"].reshape(-1, 4))
+    from oracle import conical
+    return conical.solve(p.stack.flat(), p.wavelength, p.theta, p.phi, p.M)
+
+
+def crossed_leg(count=512, with_cpu=True):
+    """2-D pillar batch: 512 seeded instances, M1 = M2 in {5, 7, 9}; parity against the reference on one
+    instance of each order."""
     from stablercwa_amd import rcwa
     from stablercwa_amd.workloads import pillar_batch
     probs = pillar_batch(count)
@@ -129,15 +269,30 @@ def crossed_leg(count=512):
     out = _timed(b, count)
     res = b.results()
     out["failed"] = int(np.sum(res["status"] != 0))
-    i = min(range(count), key=lambda k: (probs[k].M1, len(probs[k].stack.layers)))
-    r = b.result(i, res)
-    o = rcwa2d.solve(probs[i].stack.flat(), probs[i].wavelength, probs[i].theta, probs[i].phi, probs[i].M1,
-                     probs[i].M2, jones)
-    out["max_abs_dDE_vs_oracle_sample"] = max(float(np.max(np.abs(r.DEr - o["DEr"]))),
-                                              float(np.max(np.abs(r.DEt - o["DEt"]))))
+    d = 0.0
+    sample = [next(k for k in range(count) if probs[k].M1 == m) for m in (5, 7, 9)]
+    for i in sample:
+        r = b.result(i, res)
+        o = _crossed_reference(probs[i], jones)
+        d = max(d, float(np.max(np.abs(r.DEr - o["DEr"]))), float(np.max(np.abs(r.DEt - o["DEt"]))))
+    out["max_abs_dDE_vs_oracle_sample"] = d
+    out["parity_sample_M"] = [int(probs[i].M1) for i in sample]
+    out["within_1e-9"] = bool(d <= 1e-9)
     out["workload"] = "512 seeded circular-pillar crossed gratings, M1=M2 in {5,7,9}, 1-3 layers, FP64"
     b.close()
+    if with_cpu:
+        out["cpu_baseline"] = cpu_crossed(probs, jones, 16, 2)
     return out
+
+
+def _crossed_reference(p, jones):
+    from oracle import ref
+    j = [jones[0].real, jones[0].imag, jones[1].real, jones[1].imag]
+    if ref.available():
+        return ref.solve2d_ex(ref.load(), p.stack.flat(), p.wavelength, p.theta, p.phi, p.M1, p.M2, j,
+                              diagnostics=False)
+    from oracle import rcwa2d
+    return rcwa2d.solve(p.stack.flat(), p.wavelength, p.theta, p.phi, p.M1, p.M2, jones)
 
 
 def served_leg(rate, count):
@@ -177,6 +332,19 @@ def served_leg(rate, count):
                          "15% crossed pillars M in [3,9]; 1-64 wavelengths per request; 4 HIP streams, 64 GiB pool")
 
 
+def _knee(sweep):
+    """The saturation point of an arrival-rate sweep: the first rate at which the served throughput falls
+    below half of the offered load (points that arrived per second of arrivals), and the highest
+    throughput seen.  (Below the knee the ratio stays under 1 only by the drain after the last arrival.)"""
+    best = max(x["solves_per_s"] for x in sweep)
+    knee = None
+    for x in sweep:
+        x["offered_solves_per_s"] = x["points"] / (x["requests"] / x["rate_per_s"])
+        if knee is None and x["solves_per_s"] < 0.5 * x["offered_solves_per_s"]:
+            knee = x["rate_per_s"]
+    return dict(first_saturated_rate_per_s=knee, max_solves_per_s=best)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,8 +356,10 @@ def main():
     ap.add_argument("--cpu-points", type=int, default=128)
     ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--no-extra", action="store_true", help="skip the conical, crossed and served legs")
-    ap.add_argument("--serve-rate", type=float, default=5.0, help="request arrivals per second (Poisson)")
+    ap.add_argument("--serve-rate", type=float, default=5.0, help="request arrivals per second (Poisson), --only served")
     ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--serve-rates", default="2,5,10,20,40",
+                    help="arrival rates (requests/s) of the served sweep; each runs max(20, min(60, 6 rate)) requests")
     ap.add_argument("--only", choices=["conical", "crossed", "served", "copy"],
                     help="run one extra leg alone and print its result (for profiling); 'copy' copies 1 GiB on "
                          "the GPU once, the known-size kernel that calibrates the HBM traffic counters")
@@ -207,7 +377,7 @@ def main():
         print(json.dumps({"copy": {"bytes_read": src.numel() * 8, "bytes_written": dst.numel() * 8}}), flush=True)
         return
     if args.only:
-        leg = {"conical": conical_leg, "crossed": crossed_leg,
+        leg = {"conical": lambda: conical_leg(with_cpu=False), "crossed": lambda: crossed_leg(with_cpu=False),
                "served": lambda: served_leg(args.serve_rate, args.serve_requests)}[args.only]
         print(json.dumps({args.only: leg()}), flush=True)
         return
@@ -238,7 +408,8 @@ def main():
     print(f"# {ms_per_step:.2f} ms/step, stages (assembly, eig, S-matrix, total) = {np.round(st_ms, 3).tolist()} ms",
           file=sys.stderr, flush=True)
 
-    # correctness of the measured run: energy on every solve, oracle on a sample
+    # correctness of the measured run: energy on every solve, oracle on a sample; the headline only
+    # counts if every sampled solve is within 1e-9 of the oracle
     res = batch.results()
     assert np.all(res["status"] == 0), "a benchmark solve failed"
     der = res["DEr"].reshape(-1, N)
@@ -249,6 +420,7 @@ def main():
     for idx in np.linspace(0, solves - 1, 6).astype(int):
         o = rcwa1d.solve(st.flat(), probs[idx][1], 66.36, M, probs[idx][4])
         dmax = max(dmax, float(np.max(np.abs(der[idx] - o["DEr"]))), float(np.max(np.abs(det[idx] - o["DEt"]))))
+    assert dmax <= 1e-9, f"benchmark solves differ from the oracle by {dmax:.3e} > 1e-9"
 
     # roofline of the dominant kernel
     fl = batch.flops()
@@ -258,9 +430,15 @@ def main():
     else:
         name, flops, ms = "k_smatrix1d (+k_modes1d)", fl[1], smat_ms
     achieved = flops / (ms * 1e-3) / 1e12
+    # "bound" names the roofline the kernel is priced against: MI355X's FP64 peak, 78.6 TFLOP/s for MFMA and
+    # vector FP64 alike; "limiter" says what holds the kernel below it (measured, see the profiles)
     roofline = dict(bound="mfma", kernel=name, achieved=achieved, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=achieved / FP64_PEAK_TFLOPS, traffic=None,
-                    flops_per_launch=float(flops), launch_ms=ms)
+                    flops_per_launch=float(flops), launch_ms=ms,
+                    limiter=("FP64 VALU: the QR sweeps' dependent rotation chain and its global-memory latency, with "
+                             "HBM traffic from the Householder passes and the sweeps; MFMA only in the eigenvector "
+                             "back-transform" if name == "k_eig1d" else
+                             "latency of the LU panel and substitution chains"))
     traffic = measured_eig_traffic() if (name == "k_eig1d" and args.points == 4096 and M == 60) else None
     if traffic:
         roofline["traffic"] = traffic["bytes"]
@@ -293,8 +471,13 @@ def main():
             b.close()
 
     cpu = None if args.no_cpu else cpu_baseline(st, M, args.cpu_points)
-    extra = {} if args.no_extra else dict(conical=conical_leg(), crossed=crossed_leg(),
-                                          served=served_leg(args.serve_rate, args.serve_requests))
+    latency = single_solve_latency()
+    extra = {}
+    if not args.no_extra:
+        extra = dict(conical=conical_leg(with_cpu=not args.no_cpu), crossed=crossed_leg(with_cpu=not args.no_cpu))
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        sweep = [served_leg(r, max(20, min(60, int(6 * r)))) for r in rates]
+        extra["served"] = dict(by_rate=sweep, saturation=_knee(sweep))
     line = {
         "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
@@ -309,6 +492,7 @@ def main():
         "parity": {"max_abs_dDE_vs_oracle_sample": dmax, "max_abs_R_plus_T_minus_1": energy},
         "cpu_baseline": cpu,
         "plain_gpu": plain,
+        "single_solve_latency": latency,
         "batched_over_plain": (value / plain["value"]) if plain else None,
     }
     line.update(extra)

@@ -8,8 +8,6 @@ Contents
 * ``rcwa1d.py``   -- numpy restatement of the planar TE/TM solve (rcwa1d.hpp).
 * ``conical.py``  -- numpy restatement of the conical solve (rcwa1d_conical.hpp).
 * ``rcwa2d.py``   -- numpy restatement of the crossed-grating solve (rcwa2d.hpp, rcwa2d_operators.hpp).
-* ``eig_qr.py``   -- CORTH + COMQR2 restated in numpy (eig_qr.hpp), used to pin the GPU eigensolver's
-                     algorithm on small matrices.
 * ``ref.py``      -- ctypes loader for ``oracle/_ref/libsrcwa_ref.so``, the reference's own header-only
                      C++ built from its sources by ``oracle/Makefile`` (absent where the reference is absent).
 * ``flat.py``     -- decoder of the flat stack records that cross the C-ABI.

@@ -111,3 +111,39 @@ def solve2d_ex(lib, stack_flat, wavelength, theta, phi, M1, M2, jones, max_iter=
         raise ValueError(lib.ref_last_error().decode())
     return dict(DEr=DEr, DEt=DEt, status=st.value, eig_info=int(counts[0]), n_eig=int(counts[1]),
                 n_dense_solve=int(counts[2]), max_eig_residual=float(met[0]))
+
+
+def _stack_ptrs(flats):
+    arrs = [np.ascontiguousarray(f, dtype=np.float64) for f in flats]
+    ptrs = (ctypes.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    return arrs, ptrs
+
+
+def solve_conical_many(lib, flats, wavelengths, thetas, phis, Ms, threads):
+    """ref_solve_conical_many on `threads` pool threads: per-instance total efficiency and eig_info."""
+    lib.ref_set_threads(int(threads))
+    n = len(flats)
+    keep, ptrs = _stack_ptrs(flats)
+    w, t, p = (np.ascontiguousarray(x, dtype=np.float64) for x in (wavelengths, thetas, phis))
+    m = np.ascontiguousarray(Ms, dtype=np.int32)
+    energy = np.zeros(n); info = np.zeros(n, np.int32)
+    rc = lib.ref_solve_conical_many(n, ptrs, _p(w), _p(t), _p(p), _p(m), _p(energy), _p(info))
+    if rc != 0:
+        raise ValueError(lib.ref_last_error().decode())
+    return energy, info
+
+
+def solve2d_many(lib, flats, wavelengths, thetas, phis, M1s, M2s, jones, threads):
+    """ref_solve2d_many on `threads` pool threads: per-instance total efficiency and status.
+    jones: n x 4 doubles (Es.re, Es.im, Ep.re, Ep.im)."""
+    lib.ref_set_threads(int(threads))
+    n = len(flats)
+    keep, ptrs = _stack_ptrs(flats)
+    w, t, p = (np.ascontiguousarray(x, dtype=np.float64) for x in (wavelengths, thetas, phis))
+    m1 = np.ascontiguousarray(M1s, dtype=np.int32); m2 = np.ascontiguousarray(M2s, dtype=np.int32)
+    j = np.ascontiguousarray(jones, dtype=np.float64).reshape(n, 4)
+    energy = np.zeros(n); status = np.zeros(n, np.int32)
+    rc = lib.ref_solve2d_many(n, ptrs, _p(w), _p(t), _p(p), _p(m1), _p(m2), _p(j), _p(energy), _p(status))
+    if rc != 0:
+        raise ValueError(lib.ref_last_error().decode())
+    return energy, status

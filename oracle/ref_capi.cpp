@@ -236,6 +236,45 @@ int ref_solve2d_ex(const double* stack, double wavelength, double theta_deg, dou
     });
 }
 
+// Many independent conical solves, each with its own stack, spread over the reference's thread pool with
+// its own parallel_for (as sweep_conical spreads points, rcwa1d_conical.hpp:992-999).  Per instance:
+// energy[i] = the sum of every channel efficiency (R + T of the s-input plus the p-input), eig_info[i].
+// Used to time the reference on a batch of the benchmark's uneven instances.
+int ref_solve_conical_many(int n, const double* const* stacks, const double* wavelengths, const double* thetas,
+                           const double* phis, const int* Ms, double* energy, int* eig_info) {
+    return guarded([&] {
+        stablercwa::parallel_for(0, n, [&](int i) {
+            const rcwa::ConicalResult r = rcwa::solve_conical(decode1d(stacks[i]), wavelengths[i], thetas[i], phis[i],
+                                                              Ms[i]);
+            double e = 0.0;
+            for (std::size_t j = 0; j < r.DEr.size(); ++j)
+                e += r.DEr[j].ss + r.DEr[j].sp + r.DEr[j].ps + r.DEr[j].pp + r.DEt[j].ss + r.DEt[j].sp + r.DEt[j].ps +
+                     r.DEt[j].pp;
+            energy[i] = e;
+            eig_info[i] = r.eig_info;
+        });
+    });
+}
+
+// The same for 2-D solves: energy[i] = sum(DEr) + sum(DEt), status[i] = Status2D.  jones: 4 doubles per instance.
+int ref_solve2d_many(int n, const double* const* stacks, const double* wavelengths, const double* thetas,
+                     const double* phis, const int* M1s, const int* M2s, const double* jones, double* energy,
+                     int* status) {
+    return guarded([&] {
+        stablercwa::parallel_for(0, n, [&](int i) {
+            const double* jj = jones + 4 * i;
+            const rcwa::Jones2D j{cplx{jj[0], jj[1]}, cplx{jj[2], jj[3]}};
+            const rcwa::Result2D r = rcwa::solve2d(decode2d(stacks[i]), wavelengths[i], thetas[i], phis[i], M1s[i],
+                                                   M2s[i], j);
+            double e = 0.0;
+            for (double v : r.DEr) e += v;
+            for (double v : r.DEt) e += v;
+            energy[i] = e;
+            status[i] = static_cast<int>(r.status);
+        });
+    });
+}
+
 // eig_qr on a column-major n x n complex matrix. w: n complex; V: n x n complex; residual: eig_residual.
 int ref_eig(int n, const double* A, double* w, double* V, int* info, double* residual) {
     return guarded([&] {
