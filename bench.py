@@ -215,6 +215,37 @@ def _timed(batch, n):
                 total_tflops=float(fl.sum() / dt / 1e12))
 
 
+def mixed_form(make_batch, fp64_res, n, per_result):
+    """The same batch with the mixed-precision star products (FP32 LU + FP64 refinement, PLAN_MIXED): one
+    timed pass, and each result's max|dDE| against the FP64 GPU result of the same instance (itself within
+    1e-10 of the reference), with the refinement record."""
+    from stablercwa_amd import _lib
+    try:
+        ctx = _lib.Context(0, plan_flags=_lib.PLAN_MIXED)
+    except NotImplementedError as e:
+        return dict(disabled=str(e))
+    b = make_batch(ctx)
+    out = _timed(b, n)
+    res = b.results()
+    d = np.array([per_result(res, fp64_res, i) for i in range(n)])
+    ref = b.refinement()
+    b.close()
+    ctx.close()
+    out.update(failed=int(np.sum(res["status"] != 0)),
+               max_abs_dDE_vs_fp64=float(np.max(d)), p99_abs_dDE_vs_fp64=float(np.percentile(d, 99)),
+               results_above_1e-9=int(np.sum(d > 1e-9)), within_1e-9=bool(np.max(d) <= 1e-9),
+               refine_fallback_solves=int(ref[:, 0].sum()), refine_corrections_per_result=float(ref[:, 1].mean()),
+               refine_max_residual=float(ref[:, 2].max()))
+    return out
+
+
+def _de_diff(a, b, i, order_offsets):
+    """max|dDE| of instance i between two result sets (rows of the order-major DE arrays)."""
+    lo, hi = int(order_offsets[i]), int(order_offsets[i + 1])
+    return max(float(np.max(np.abs(a["DEr"][lo:hi] - b["DEr"][lo:hi]))),
+               float(np.max(np.abs(a["DEt"][lo:hi] - b["DEt"][lo:hi]))))
+
+
 def _spread_by(key, count, k):
     """k instance indices at evenly spaced ranks of key (smallest to largest order, say)."""
     order = sorted(range(count), key=key)
@@ -241,7 +272,12 @@ def conical_leg(count=2048, with_cpu=True):
     out["parity_sample_M"] = [int(probs[i].M) for i in sample]
     out["within_1e-9"] = bool(d <= 1e-9)
     out["workload"] = "2048 seeded conical instances, M in [20,120], 1-4 grating layers, FP64"
+    offs = b.order_offsets
     b.close()
+    out["mixed"] = mixed_form(
+        lambda c: rcwa.ConicalBatch([p.stack for p in probs],
+                                    [(i, p.wavelength, p.theta, p.phi, p.M) for i, p in enumerate(probs)], c),
+        res, count, lambda r, f, i: _de_diff(r, f, i, offs))
     if with_cpu:
         out["cpu_baseline"] = cpu_conical(probs, 64, 8)
     return out
@@ -279,7 +315,13 @@ def crossed_leg(count=512, with_cpu=True):
     out["parity_sample_M"] = [int(probs[i].M1) for i in sample]
     out["within_1e-9"] = bool(d <= 1e-9)
     out["workload"] = "512 seeded circular-pillar crossed gratings, M1=M2 in {5,7,9}, 1-3 layers, FP64"
+    offs = b.order_offsets
     b.close()
+    out["mixed"] = mixed_form(
+        lambda c: rcwa.CrossedBatch([p.stack for p in probs],
+                                    [(i, p.wavelength, p.theta, p.phi, p.M1, p.M2, jones) for i, p in enumerate(probs)],
+                                    c),
+        res, count, lambda r, f, i: _de_diff(r, f, i, offs))
     if with_cpu:
         out["cpu_baseline"] = cpu_crossed(probs, jones, 16, 2)
     return out

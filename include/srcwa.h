@@ -63,8 +63,14 @@ int srcwa_context_set_pool(srcwa_context* ctx, srcwa_pool* pool);
  *   eigensolve's eigen-pair residual max_j ||A v_j - l_j v_j|| / (||A||_F ||v_j||) (eig.hpp:38-69) and every
  *   dense solve's backward error ||A X - B||_F / (||A||_F ||X||_F + ||B||_F) (lu.hpp:452-459), reduced per
  *   solve by srcwa_plan_diagnostics; the plan holds copies of the layer matrices (more device memory)
- *   and each dense solve does one more product. */
-enum { SRCWA_PLAN_DIAGNOSTICS = 1 };
+ *   and each dense solve does one more product.
+ * SRCWA_PLAN_MIXED: the S-matrix recursion's dense solves (the star products' (I - R r)^{-1} T and the
+ *   interface, top-block and downward-sweep solves) are factored in complex FP32, with the trailing update
+ *   on FP32 MFMA, and refined to FP64 accuracy by FP64 residuals (LAPACK zcgesv's per-column stopping
+ *   test, at most 6 corrections); a solve whose corrections stall falls back to the FP64 LU.  Eigensolves,
+ *   operator assembly and the efficiency sums stay FP64.  No counterpart in the reference (its LU is FP64
+ *   only, lu.hpp:267-443); srcwa_plan_refinement reports what each solve needed. */
+enum { SRCWA_PLAN_DIAGNOSTICS = 1, SRCWA_PLAN_MIXED = 2 };
 int srcwa_context_set_plan_flags(srcwa_context* ctx, int flags);
 
 /* ---------------------------------------------------------------------------------------------
@@ -156,6 +162,11 @@ int srcwa_plan_results(srcwa_plan* plan, double* R, double* T, double* DEr, doub
  * counted as the reference counts them (a conical layer in the Schur basis adds one eigensolve), [4] conical
  * layers propagated in the Schur basis (0 for planar and crossed plans). */
 int srcwa_plan_diagnostics(srcwa_plan* plan, double* out);
+/* The mixed-precision record of the last execute, 3 doubles per problem (zeros for plans without
+ * SRCWA_PLAN_MIXED): [0] dense solves that fell back to the FP64 LU, [1] FP32 correction steps over all of
+ * its refined solves, [2] the largest final max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf) of its
+ * refined solves (0 when every solve fell back).  No counterpart in the reference. */
+int srcwa_plan_refinement(srcwa_plan* plan, double* out);
 /* Per-order geometry of the last execute of a conical or crossed plan (8 doubles per order, at 8 x the
  * order offset).  Conical: kx, kzI (re, im), kzII (re, im), flux_r, flux_t, phi_m in degrees
  * (ConicalResult::kx, kzI, kzII, flux_r, flux_t, phi_m_deg, rcwa1d_conical.hpp:105-112).  Crossed: Kx, Ky,
@@ -192,6 +203,13 @@ int srcwa_solve1d_batch(srcwa_context* ctx, int n_stacks, const double* stacks, 
 int srcwa_zgeev_batched(srcwa_context* ctx, int count, const int* n, const double* A, double* w, double* V, int* info);
 int srcwa_zgesv_batched(srcwa_context* ctx, int count, const int* n, const int* nrhs, const double* A, const double* B,
                         double* X, int* info);
+/* srcwa_zgesv_batched in the mixed-precision form of SRCWA_PLAN_MIXED: each system factored in complex FP32
+ * (trailing updates on FP32 MFMA) and refined to FP64 accuracy by FP64 residuals, falling back to the FP64
+ * LU when the corrections stall.  refine: 3 doubles per item -- fell back to FP64 (0 / 1), FP32 correction
+ * steps, final max_j ||b_j - A x_j||_inf / (||A||_inf ||x_j||_inf) (0 after a fallback).  No counterpart in
+ * the reference (its LU is FP64 only, lu.hpp:267-443); a custom_solve hook may bind it. */
+int srcwa_zgesv_batched_mixed(srcwa_context* ctx, int count, const int* n, const int* nrhs, const double* A,
+                              const double* B, double* X, int* info, double* refine);
 /* srcwa_zgeev_batched with the QR iteration limit max_iter per eigenvalue (EigOptions::max_iter,
  * eig_types.hpp:29; 30 in srcwa_zgeev_batched): info_i = k > 0 when eigenvalue k (1-based) did not converge,
  * and V_i is then not meaningful.  The custom_eig adapter of include/stablercwa_amd/hooks.hpp binds it. */

@@ -75,14 +75,23 @@ inline srcwa_context* context(int device, int plan_flags = 0) {
 // Per-solve diagnostics of a plan's last execute (srcwa_plan_diagnostics) and its stage times.
 struct PlanReport {
     std::vector<double> diag;     // 5 per problem
+    std::vector<double> refine;   // 3 per problem (srcwa_plan_refinement)
     double ms[4] = {0, 0, 0, 0};  // setup + assembly, eigensolves, modes + S-matrix, total
 };
 inline PlanReport report(srcwa_plan* p, int problems) {
     PlanReport r;
     r.diag.resize(5 * static_cast<std::size_t>(problems));
+    r.refine.resize(3 * static_cast<std::size_t>(problems));
     check(srcwa_plan_diagnostics(p, r.diag.data()));
+    check(srcwa_plan_refinement(p, r.refine.data()));
     check(srcwa_plan_stage_ms(p, r.ms));
     return r;
+}
+
+// Plan flags of a call's options.
+template <class Opt>
+int plan_flags(const Opt& o) {
+    return (o.diagnostics ? SRCWA_PLAN_DIAGNOSTICS : 0) | (o.mixed_precision ? SRCWA_PLAN_MIXED : 0);
 }
 
 // RAII owner of a launch plan.
@@ -176,9 +185,11 @@ struct Options {
     std::function<EigResult(const Matrix&)> custom_eig;
     std::function<Matrix(const Matrix&, const Matrix&)> custom_multiply;
     std::function<Matrix(const Matrix&, const Matrix&)> custom_solve;
-    // this build only: the GPU and the per-launch device workspace bound (0 = 64 GiB)
+    // this build only: the GPU, the per-launch device workspace bound (0 = 64 GiB), and the mixed-precision
+    // star products (S-matrix solves by FP32 LU + FP64 iterative refinement; SRCWA_PLAN_MIXED)
     int device = 0;
     std::int64_t max_workspace_bytes = 0;
+    bool mixed_precision = false;
 };
 
 struct Point {
@@ -209,6 +220,10 @@ struct Result {
     int eig_info = 0;
     int n_eig = 0, n_dense_solve = 0;
     Timing timing;
+    // this build only (Options::mixed_precision): solves that fell back to FP64, FP32 correction steps,
+    // largest final refined residual ||b - A x|| / (||A|| ||x||)
+    int refine_fallbacks = 0, refine_iterations = 0;
+    double refine_residual = 0;
 };
 
 // ---- conical ---------------------------------------------------------------------------------
@@ -298,6 +313,8 @@ struct ConicalResult {
     int n_eig = 0, n_dense_solve = 0;
     int n_schur = 0;
     Timing timing;
+    int refine_fallbacks = 0, refine_iterations = 0;   // this build only, as Result
+    double refine_residual = 0;
     double min_abs_lambda = std::numeric_limits<double>::infinity();
     bool degenerate_mode_warning = false;
 };
@@ -362,6 +379,7 @@ struct Options2D {
     std::function<Matrix(const Matrix&, const Matrix&)> custom_solve;
     int device = 0;
     std::int64_t max_workspace_bytes = 0;
+    bool mixed_precision = false;
 };
 
 struct Timing2D {
@@ -403,6 +421,8 @@ struct Result2D {
     int n_eig = 0, n_dense_solve = 0;
     double max_eig_residual = 0;
     Timing2D timing;
+    int refine_fallbacks = 0, refine_iterations = 0;   // this build only, as Result
+    double refine_residual = 0;
 };
 
 namespace detail {
@@ -473,6 +493,10 @@ void fill_diagnostics(R& r, const PlanReport& rep, int i, int problems) {
     r.max_solve_backward_error = d[1];
     r.n_eig = static_cast<int>(d[2]);
     r.n_dense_solve = static_cast<int>(d[3]);
+    const double* f = rep.refine.data() + 3 * static_cast<std::size_t>(i);
+    r.refine_fallbacks = static_cast<int>(f[0]);
+    r.refine_iterations = static_cast<int>(f[1]);
+    r.refine_residual = f[2];
     if (rep.ms[1] >= 0) {
         r.timing.eig = rep.ms[1] * 1e-3 / problems;
         r.timing.linear = rep.ms[2] * 1e-3 / problems;
@@ -496,7 +520,7 @@ inline std::vector<Result> planar_batch(const Stack& stack, const std::vector<Po
         th[i] = points[i].theta_deg;
     }
     Plan plan;
-    check(srcwa_plan1d_create(context(options.device, options.diagnostics ? SRCWA_PLAN_DIAGNOSTICS : 0), 1, rec.data(),
+    check(srcwa_plan1d_create(context(options.device, plan_flags(options)), 1, rec.data(),
                               &off, P, idx.data(), wl.data(), th.data(), Ms.data(), pols.data(),
                               options.max_workspace_bytes, &plan.p));
     plan.run();
@@ -542,7 +566,7 @@ inline std::vector<ConicalResult> conical_batch(const Stack& stack, const std::v
         ph[i] = points[i].phi_deg;
     }
     Plan plan;
-    check(srcwa_plan_conical_create(context(options.device, options.diagnostics ? SRCWA_PLAN_DIAGNOSTICS : 0), 1,
+    check(srcwa_plan_conical_create(context(options.device, plan_flags(options)), 1,
                                     rec.data(), &off, P, idx.data(), wl.data(), th.data(), ph.data(), Ms.data(),
                                     options.max_workspace_bytes, &plan.p));
     plan.run();
@@ -637,7 +661,7 @@ inline Result2D solve2d(const Stack2D& stack, double wavelength, double theta_de
     const int idx = 0, fz = options.factorization == Factorization2D::Li ? 0 : 1;
     const double jv[4] = {jones[0].real(), jones[0].imag(), jones[1].real(), jones[1].imag()};
     detail::Plan plan;
-    detail::check(srcwa_plan2d_create(detail::context(options.device, options.diagnostics ? SRCWA_PLAN_DIAGNOSTICS : 0),
+    detail::check(srcwa_plan2d_create(detail::context(options.device, detail::plan_flags(options)),
                                       1, rec.data(), &off, 1, &idx, &wavelength, &theta_deg, &phi_deg, &M1, &M2, jv,
                                       &fz, options.max_workspace_bytes, &plan.p));
     plan.run();
@@ -702,6 +726,9 @@ inline Result2D solve2d(const Stack2D& stack, double wavelength, double theta_de
     r.max_eig_residual = rep.diag[0];
     r.n_eig = static_cast<int>(rep.diag[2]);
     r.n_dense_solve = static_cast<int>(rep.diag[3]);
+    r.refine_fallbacks = static_cast<int>(rep.refine[0]);
+    r.refine_iterations = static_cast<int>(rep.refine[1]);
+    r.refine_residual = rep.refine[2];
     if (rep.ms[0] >= 0) {
         r.timing.operators = rep.ms[0] * 1e-3;
         r.timing.eig = rep.ms[1] * 1e-3;

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libsrcwa_amd.so")
 
 OK, EINVAL, EDEVICE, ENOMEM, EUNSUPPORTED = 0, -1, -2, -3, -4
 PLAN_DIAGNOSTICS = 1
+PLAN_MIXED = 2          # star-product solves by FP32 LU + FP64 iterative refinement
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -51,6 +52,7 @@ PROTOTYPES = {
     "srcwa_context_set_pool": (ctypes.c_int, [_vp, _vp]),
     "srcwa_context_set_plan_flags": (ctypes.c_int, [_vp, ctypes.c_int]),
     "srcwa_plan_diagnostics": (ctypes.c_int, [_vp, _vp]),
+    "srcwa_plan_refinement": (ctypes.c_int, [_vp, _vp]),
     "srcwa_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.c_int,
                                            ctypes.POINTER(_vp)]),
     "srcwa_server_submit": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ctypes.c_int64)]),
@@ -74,6 +76,7 @@ PROTOTYPES = {
     "srcwa_zgeev_batched": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
     "srcwa_zgeev_batched_iter": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_int]),
     "srcwa_zgesv_batched": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "srcwa_zgesv_batched_mixed": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "srcwa_zgemm_batched": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "srcwa_zgeev_batched_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]),
 }

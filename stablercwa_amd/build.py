@@ -25,6 +25,9 @@ INCLUDE = os.path.join(ROOT, "include")
 # C++ host API driver used by the GPU tests (links the library above)
 DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "api_driver.cpp")
 DRIVER = os.path.join(OBJ, "api_driver")
+# command-line front end of the JSON request layer (include/stablercwa_amd/rcwa_json.hpp)
+JSON_SRC = os.path.join(ROOT, "tests", "cpp", "json_driver.cpp")
+JSON_DRIVER = os.path.join(OBJ, "json_driver")
 
 FAMILIES = ["rcwa1d.hip", "conical.hip", "rcwa2d.hip", "dense.hip"]
 HOST_SOURCES = ["runtime.cpp", "server.cpp"]
@@ -107,12 +110,14 @@ def build_variant(out_lib: str, defines=(), obj_dir: str = None) -> str:
 
 def build(verbose: bool = False, incremental: bool = False) -> str:
     _link(_compile_all(OBJ, incremental=incremental), LIB)
-    if os.path.exists(DRIVER_SRC):
-        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, DRIVER_SRC, "-o", DRIVER, "-L", HERE,
+    for src, out in ((DRIVER_SRC, DRIVER), (JSON_SRC, JSON_DRIVER)):
+        if not os.path.exists(src):
+            continue
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", INCLUDE, src, "-o", out, "-L", HERE,
                "-lsrcwa_amd", "-Wl,-rpath,$ORIGIN/.."]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
-            raise RuntimeError(f"C++ API driver failed to build:\n{r.stderr[-4000:]}")
+            raise RuntimeError(f"{os.path.basename(src)} failed to build:\n{r.stderr[-4000:]}")
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -18,6 +18,7 @@ enum : int {
 // Plan flags (srcwa_context_set_plan_flags), carried per instance.
 enum : int {
     PLAN_DIAGNOSTICS = 1,      // eigen-pair residuals and solve backward errors (Options::diagnostics)
+    PLAN_MIXED = 2,            // S-matrix dense solves by FP32 LU + FP64 iterative refinement
 };
 
 // Per-layer-task diagnostics, reduced per instance by the S-matrix kernels.
@@ -28,9 +29,11 @@ struct TaskDiag {
     int n_eig;                 // dense eigensolves of the task
 };
 
-// Per-instance diagnostics record (srcwa_plan_diagnostics): max_eig_residual, max_solve_backward_error,
-// n_eig, n_dense_solve, n_schur.
-constexpr int DIAG_WORDS = 5;
+// Per-instance diagnostics record: max_eig_residual, max_solve_backward_error, n_eig, n_dense_solve,
+// n_schur (srcwa_plan_diagnostics), then the mixed-precision record (srcwa_plan_refinement): refined
+// solves that fell back to FP64, FP32 correction steps, largest final refined residual.
+constexpr int DIAG_WORDS = 8;
+constexpr int DIAG_REPORT_WORDS = 5;
 
 // One planar (TE/TM) or conical instance.
 struct Inst1D {
@@ -52,6 +55,17 @@ __host__ __device__ inline int fft_samples_for(int M, int samples) {
     int n = 1;
     while (n < target) n *= 2;
     return n;
+}
+
+// Scratch of one mixed-precision solve (wg_refine_solve) in complex-double words, as carved up at its
+// start: the FP32 factors, x, r, the FP32 right-hand side, the pivots and the gather list.
+__host__ __device__ inline int64_t refine_words(int64_t n, int64_t nrhs) {
+    return ((n * n + 1) / 2 + 16) + 2 * (n * nrhs + 16) + ((n * nrhs + 1) / 2 + 16) + 2 * (n / 4 + 16);
+}
+// Bump words a mixed-precision dense solve of order n with up to nrhs right-hand sides takes beyond what
+// the FP64 solve takes (the FP64 factors, which the fallback uses, are held by both forms).
+__host__ __device__ inline int64_t mixed_solve_words(int64_t n, int64_t nrhs) {
+    return refine_words(n, nrhs) + 2 * n * nrhs + 64;
 }
 
 // One layer task: a grating layer of an instance that needs an eigensolve.

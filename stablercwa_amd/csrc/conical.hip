@@ -655,6 +655,7 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix_conical(const Inst1D* __restrict__ 
     c.N = n;
     c.fail = 0;
     c.diag = diag;
+    c.mixed = in.flags & PLAN_MIXED;
     for (int t = 0; t < in.n_tasks; ++t) {
         const int* ti = task_info + 4 * (in.first_task + t);
         if (ti[0] < 0 || ti[3] < 0) failed = 1;
@@ -765,6 +766,7 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix_conical(const Inst1D* __restrict__ 
         dg[2] = n_eig;
         dg[3] = n_solve + c.n_solve;
         dg[4] = n_schur;
+        c.refine_record(dg);
     }
 }
 void smatrix_conical_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out) {

@@ -3,6 +3,9 @@
 // Also the no-allocation eigensolve entry point (srcwa_zgeev_batched_noalloc) and its scratch array.
 #include "batch.hpp"
 #include "eig.hpp"
+#if SRCWA_MIXED_PRECISION
+#include "wg_la32.hpp"
+#endif
 
 #if SRCWA_MAIN_TU
 #include <mutex>
@@ -26,6 +29,27 @@ __global__ __launch_bounds__(NT) void k_zgesv_batched(const DenseItem* __restric
     if (!sing) wg_lu_solve(it.n, arena + it.a, it.n, piv, arena + it.b, it.n, it.nrhs);
     if (threadIdx.x == 0) info[blockIdx.x] = sing;
 }
+
+#if SRCWA_MIXED_PRECISION
+// The mixed-precision form: A is preserved, B is overwritten by the solution; scratch holds the FP64
+// fallback's factors (n x n) and pivots, then wg_refine_solve's scratch.  refine: 3 doubles per item
+// (fell back to FP64, FP32 correction steps, final refined residual).
+__global__ __launch_bounds__(NT) void k_zgesv_mixed_batched(const DenseItem* __restrict__ items, cd* arena, int* info,
+                                                             double* refine) {
+    const DenseItem it = items[blockIdx.x];
+    const int64_t nn = (int64_t)it.n * it.n;
+    cd* lu64 = arena + it.scratch;
+    int* piv = reinterpret_cast<int*>(lu64 + nn);
+    cd* scr = lu64 + nn + (it.n + 15) / 16 * 16;
+    const RefineStats st = wg_refine_solve(it.n, arena + it.a, arena + it.b, it.nrhs, scr, lu64, piv);
+    if (threadIdx.x == 0) {
+        info[blockIdx.x] = st.singular;
+        refine[3 * blockIdx.x] = st.fallback;
+        refine[3 * blockIdx.x + 1] = st.iters;
+        refine[3 * blockIdx.x + 2] = st.resid;
+    }
+}
+#endif
 #endif
 
 // Item i: the matrix at abase + a (overwritten by the eigenvectors), eigenvalues at wbase + w, scratch
@@ -56,6 +80,11 @@ void launch_zgemm_batched(hipStream_t s, int count, const DenseItem* items, cd* 
 void launch_zgesv_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info) {
     k_zgesv_batched<<<count, NT, 0, s>>>(items, arena, info);
 }
+#if SRCWA_MIXED_PRECISION
+void launch_zgesv_mixed_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info, double* refine) {
+    k_zgesv_mixed_batched<<<count, NT, 0, s>>>(items, arena, info, refine);
+}
+#endif
 void launch_zgeev_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info, int max_iter,
                           int max_n) {
     QcLaunch a{};

@@ -289,6 +289,48 @@ DEV void block_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_
     __syncthreads();
 }
 
+// The left half of one QR sweep for large orders, in stages of B = 64 LC rotations.  A stage's rotations
+// G_{i0..i1} pivot on columns i0-1 .. i1-1, so wave 0 runs the stage's chain on the B columns starting
+// at i0-1 exactly as wave_left_sweep does (owner lane broadcast by readlane, no barrier per rotation),
+// leaving each column's running value in row i1; then every thread applies the stage's B rotations,
+// read from LDS, to one column right of that block at a time (rows i0-1 .. i1, loads batched eight
+// ahead).  Two barriers per stage instead of one per rotation; each element sees the same rotations in
+// the same order with the same arithmetic as in the other sweep forms.  H is row-major.
+template <int LC, int RD>
+DEV void staged_left_sweep(int n, int l, int en, cd* __restrict__ H, double* rot_c, double* rot_ci, double* rot_s,
+                           int cend) {
+    constexpr int B = 64 * LC;
+    constexpr int LB = 8;
+    for (int i0 = l + 1; i0 <= en; i0 += B) {
+        const int i1 = min(i0 + B - 1, en);
+        const int c0 = i0 - 1;
+        const int cnear = min(c0 + B, cend);
+        if (wave_id() == 0) wave_left_sweep<LC, RD, true>(n, c0, i1, H, rot_c, rot_ci, rot_s, cnear);
+        __syncthreads();
+        for (int c = cnear + threadIdx.x; c < cend; c += NT) {
+            cd* col = H + c;
+            cd carry = col[(long)c0 * n];
+            for (int ib = i0; ib <= i1; ib += LB) {
+                cd cur[LB];
+#pragma unroll
+                for (int u = 0; u < LB; ++u)
+                    if (ib + u <= i1) cur[u] = ldc(col + (long)(ib + u) * n);
+#pragma unroll
+                for (int u = 0; u < LB; ++u) {
+                    const int i = ib + u;
+                    if (i <= i1) {
+                        cd top;
+                        rotate_left(rot_c[i], rot_ci[i], rot_s[i], carry, cur[u], top, carry);
+                        stc(col + (long)(i - 1) * n, top);
+                    }
+                }
+            }
+            stc(col + (long)i1 * n, carry);
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Deferred updates of the QR sweeps.
 //
@@ -713,13 +755,18 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
 
         // (1) left rotations.  Orders up to 256: the rotation chain runs inside wave 0 (QC columns per
         //     lane, rows of the row-major H loaded coalesced EIG_RING rows ahead) while the other waves
-        //     wait at the barrier.  Larger orders: every thread owns columns and each rotation is
-        //     published through LDS with a barrier.
+        //     wait at the barrier.  Larger orders: stages of 64 SRCWA_EIG_STAGE_LC rotations, each chain
+        //     in wave 0 on the block of columns it pivots on, then the stage's rotations applied to the
+        //     columns right of the block by every thread.
         //     Only columns l..en: the columns right of en are queued.
         if constexpr (QC <= 4) {
             if (wave == 0) wave_left_sweep<QC, EIG_RING, true>(n, l, en, HR, rot_c, rot_ci, rot_s, en + 1);
         } else {
+#if SRCWA_EIG_STAGE_LC > 0
+            staged_left_sweep<SRCWA_EIG_STAGE_LC, SRCWA_EIG_STAGE_RING>(n, l, en, HR, rot_c, rot_ci, rot_s, en + 1);
+#else
             block_left_sweep<QC / 4, EIG_RING>(n, l, en, HR, rot_c, rot_ci, rot_s, en + 1);
+#endif
         }
         __syncthreads();
 

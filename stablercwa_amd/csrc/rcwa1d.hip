@@ -269,6 +269,7 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix1d(const Inst1D* __restrict__ insts,
     c.N = N;
     c.fail = 0;
     c.diag = diag;
+    c.mixed = in.flags & PLAN_MIXED;
     if (status == ST_OK) {
         __shared__ double s_thick[SMAT_MAX_LAYERS];
         LayerCursor cur(st);
@@ -356,6 +357,7 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix1d(const Inst1D* __restrict__ insts,
         dg[2] = n_eig;
         dg[3] = n_solve + c.n_solve;
         dg[4] = 0.0;
+        c.refine_record(dg);
     }
 }
 void smatrix1d_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out) {

@@ -788,6 +788,7 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix2d(const Inst2D* __restrict__ insts,
     c.N = n;
     c.bump = Bump(ws, in.ws_lo, in.ws_hi);
     c.fail = 0;
+    c.mixed = in.flags & PLAN_MIXED;
     auto region = [&](int r) -> Region2D {
         Region2D g;
         g.q = ws + lay[x2_q(r)];
@@ -1229,6 +1230,7 @@ __global__ SMAT_KERNEL_BOUNDS void k_smatrix2d(const Inst2D* __restrict__ insts,
         dg[2] = n_eig;
         dg[3] = n_solve + c.n_solve;
         dg[4] = 0.0;
+        c.refine_record(dg);
     }
 }
 void smatrix2d_launch(int grid, hipStream_t s, const Inst2D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out) {

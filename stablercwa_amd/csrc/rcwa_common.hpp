@@ -4,7 +4,11 @@
 #pragma once
 
 #include "batch.hpp"
+#include "tuned.hpp"
 #include "wg_la.hpp"
+#if SRCWA_MIXED_PRECISION
+#include "wg_la32.hpp"
+#endif
 
 constexpr int SMAT_MAX_LAYERS = 128;
 constexpr int SMAT_MAX_BLOCKS = 2 * SMAT_MAX_LAYERS + 1;
@@ -257,8 +261,18 @@ struct SmatCtx {
     int n_solve = 0;        // dense linear solves (Result::n_dense_solve)
     bool diag = false;      // compute backward errors (Options::diagnostics)
     double max_berr = 0.0;  // largest backward error so far (Result::max_solve_backward_error)
+    bool mixed = false;     // dense solves by FP32 LU + FP64 refinement (PLAN_MIXED)
+    int n_refine = 0;       // FP32 correction steps over all refined solves
+    int n_fallback = 0;     // refined solves that fell back to the FP64 LU
+    double max_refine_res = 0.0;   // largest final ||b - A x|| / (||A|| ||x||) of a refined solve
     DEV void check() {
         if (bump.overflow) fail = 2;
+    }
+    // writes the refinement record (DIAG_WORDS slots 5..7) of one instance
+    DEV void refine_record(double* dg) const {
+        dg[5] = n_fallback;
+        dg[6] = n_refine;
+        dg[7] = max_refine_res;
     }
 };
 
@@ -442,12 +456,45 @@ DEV cd* op_to_dense(SmatCtx& c, DOp a) {
     return m;
 }
 
+#if SRCWA_MIXED_PRECISION
+// The mixed-precision form of one dense solve: B <- A^{-1} B by FP32 LU and FP64 iterative refinement
+// (wg_refine_solve), A preserved; all of its scratch is released on return.
+DEV void mixed_dense_solve(SmatCtx& c, const cd* A, cd* B, int nrhs) {
+    const int N = c.N;
+    const int64_t mark = c.bump.lo;
+    cd* lu64 = c.bump.tmp((int64_t)N * N);
+    int* piv = reinterpret_cast<int*>(c.bump.tmp(N));
+    cd* scr = c.bump.tmp(refine_words(N, nrhs));
+    cd* b0 = c.diag ? c.bump.tmp((int64_t)N * nrhs) : nullptr;
+    cd* r = c.diag ? c.bump.tmp((int64_t)N * nrhs) : nullptr;
+    c.check();
+    if (c.fail) return;
+    if (c.diag) wg_copy(N, nrhs, B, N, b0, N);
+    const RefineStats st = wg_refine_solve(N, A, B, nrhs, scr, lu64, piv);
+    if (st.singular) {
+        c.fail = 1;
+        return;
+    }
+    c.n_refine += st.iters;
+    c.n_fallback += st.fallback;
+    c.max_refine_res = max_keep_nan(c.max_refine_res, st.resid);
+    if (c.diag) c.max_berr = max_keep_nan(c.max_berr, wg_backward_error(N, A, B, b0, nrhs, r));
+    c.bump.lo = mark;
+}
+#endif
+
 // B <- A^{-1} B (A is preserved).  Singular -> numerical failure.  Every call is one dense solve of
 // the reference (linear_solve, rcwa1d.hpp:248-281); with diagnostics its backward error is taken from
 // copies of A and B released right after.
 DEV void dense_solve(SmatCtx& c, const cd* A, cd* B, int nrhs) {
     const int N = c.N;
     ++c.n_solve;
+#if SRCWA_MIXED_PRECISION
+    if (c.mixed) {
+        mixed_dense_solve(c, A, B, nrhs);
+        return;
+    }
+#endif
     cd* lu = c.bump.tmp((int64_t)N * N);
     int* piv = reinterpret_cast<int*>(c.bump.tmp(N));
     c.check();
@@ -474,10 +521,16 @@ DEV void dense_solve(SmatCtx& c, const cd* A, cd* B, int nrhs) {
         c.bump.lo = mark;
     }
 }
-// B <- A^{-1} B, destroying A.
+// B <- A^{-1} B, destroying A (the mixed-precision form leaves A as it was).
 DEV void dense_solve_inplace(SmatCtx& c, cd* A, cd* B, int nrhs) {
     const int N = c.N;
     ++c.n_solve;
+#if SRCWA_MIXED_PRECISION
+    if (c.mixed) {
+        mixed_dense_solve(c, A, B, nrhs);
+        return;
+    }
+#endif
     int* piv = reinterpret_cast<int*>(c.bump.tmp(N));
     c.check();
     if (c.fail) return;

@@ -18,6 +18,7 @@
 #include "../../include/srcwa.h"
 #include "batch.hpp"
 #include "context.hpp"
+#include "tuned.hpp"
 #include "cplx.hpp"
 
 // kernel launchers (rcwa1d.hip, dense.hip)
@@ -32,6 +33,12 @@ void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts,
                     int part, int flags, hipEvent_t* ev);
 void launch_zgemm_batched(hipStream_t s, int count, const DenseItem* items, cd* arena);
 void launch_zgesv_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info);
+#if SRCWA_MIXED_PRECISION
+void launch_zgesv_mixed_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info, double* refine);
+#endif
+constexpr const char* kMixedOff =
+    "mixed-precision star products are compiled out of this build (SRCWA_MIXED_PRECISION 0): their first GPU run "
+    "ended in a memory fault whose cause is not found yet";
 void launch_zgeev_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info, int max_iter,
                           int max_n);
 
@@ -75,6 +82,7 @@ void invalid(const std::string& m) { throw Failure{SRCWA_EINVAL, m}; }
 void srcwa_set_last_error(const std::string& msg) { g_error = msg; }
 
 static_assert(int(SRCWA_PLAN_DIAGNOSTICS) == int(PLAN_DIAGNOSTICS), "plan flag of the C-ABI and the kernels");
+static_assert(int(SRCWA_PLAN_MIXED) == int(PLAN_MIXED), "plan flag of the C-ABI and the kernels");
 
 // A device arena carved first-fit into 256-byte aligned blocks.  Plans created on a context that has
 // a pool take every device buffer from it, so many plans can come and go (a request server) without
@@ -616,7 +624,8 @@ int srcwa_pool_destroy(srcwa_pool* pool) {
 int srcwa_context_set_plan_flags(srcwa_context* ctx, int flags) {
     return guarded([&] {
         if (!ctx) invalid("srcwa_context_set_plan_flags: null context");
-        if (flags & ~int(SRCWA_PLAN_DIAGNOSTICS)) invalid("srcwa_context_set_plan_flags: unknown flag");
+        if (flags & ~int(SRCWA_PLAN_DIAGNOSTICS | SRCWA_PLAN_MIXED)) invalid("srcwa_context_set_plan_flags: unknown flag");
+        if ((flags & SRCWA_PLAN_MIXED) && !SRCWA_MIXED_PRECISION) throw Failure{SRCWA_EUNSUPPORTED, kMixedOff};
         ctx->plan_flags = flags;
     });
 }
@@ -698,7 +707,7 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
                     w += align16(slot_words(N) + (diag ? 2 * NN1 : 0) + fft_scratch_words(li[j], M[i]));
                 }
             const int64_t ws_lo = w;
-            w += cost.bump_words + (diag ? 3 * NN1 + 64 : 0);
+            w += cost.bump_words + (diag ? 3 * NN1 + 64 : 0) + ((plan->flags & PLAN_MIXED) ? mixed_solve_words(N, N) : 0);
             // admission into the current chunk by workspace footprint
             if (i > cur.i0 && chunk_ws + w > budget_words) {
                 cur.i1 = i;
@@ -864,7 +873,8 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
                     w += align16(conical_slot_words(N) + (diag ? 4 * NN1 : 0) + fft_scratch_words(all[eff[j]], M[i]));
                 }
             const int64_t ws_lo = w;
-            w += cost.bump_words + (diag ? 12 * NN1 + 64 : 0);
+            w += cost.bump_words + (diag ? 12 * NN1 + 64 : 0) +
+                 ((plan->flags & PLAN_MIXED) ? mixed_solve_words(2 * N, 2 * N) : 0);
             if (i > cur.i0 && chunk_ws + w > budget_words) {
                 cur.i1 = i;
                 cur.t1 = static_cast<int>(h_tasks.size());
@@ -1068,12 +1078,27 @@ int srcwa_plan_results(srcwa_plan* plan, double* R, double* T, double* DEr, doub
 
 int64_t srcwa_plan_orders(const srcwa_plan* plan) { return plan ? plan->total_orders : -1; }
 
+// The per-instance records hold DIAG_WORDS doubles: the diagnostics report first, then the refinement
+// record; each entry point hands out its part.
+static void copy_diag_words(srcwa_plan* plan, double* out, int first, int count) {
+    hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+    std::vector<double> all(size_t(DIAG_WORDS) * plan->n_inst);
+    copy_out(all.data(), plan->diag.p, all.size() * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
+    for (int i = 0; i < plan->n_inst; ++i)
+        for (int k = 0; k < count; ++k) out[size_t(count) * i + k] = all[size_t(DIAG_WORDS) * i + first + k];
+}
+
 int srcwa_plan_diagnostics(srcwa_plan* plan, double* out) {
     return guarded([&] {
         if (!plan || !out) invalid("srcwa_plan_diagnostics: null plan or output");
-        hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
-        copy_out(out, plan->diag.p, size_t(DIAG_WORDS) * plan->n_inst * sizeof(double), hipMemcpyDeviceToHost,
-                 "D2H", plan);
+        copy_diag_words(plan, out, 0, DIAG_REPORT_WORDS);
+    });
+}
+
+int srcwa_plan_refinement(srcwa_plan* plan, double* out) {
+    return guarded([&] {
+        if (!plan || !out) invalid("srcwa_plan_refinement: null plan or output");
+        copy_diag_words(plan, out, DIAG_REPORT_WORDS, DIAG_WORDS - DIAG_REPORT_WORDS);
     });
 }
 
@@ -1187,14 +1212,18 @@ int srcwa_zgemm_batched(srcwa_context* ctx, int count, const int* m, const int* 
     });
 }
 
-int srcwa_zgesv_batched(srcwa_context* ctx, int count, const int* n, const int* nrhs, const double* A, const double* B,
-                        double* X, int* info) {
+namespace {
+// refine == nullptr: the FP64 LU (srcwa_zgesv_batched); else FP32 LU + FP64 refinement, 3 doubles per item
+int zgesv_batched(srcwa_context* ctx, int count, const int* n, const int* nrhs, const double* A, const double* B,
+                  double* X, int* info, double* refine) {
+    const char* name = refine ? "srcwa_zgesv_batched_mixed" : "srcwa_zgesv_batched";
     return dense_batch(ctx, [&](hipStream_t s) {
-        if (count < 1 || !n || !nrhs || !A || !B || !X || !info) invalid("srcwa_zgesv_batched: empty or null inputs");
+        if (count < 1 || !n || !nrhs || !A || !B || !X || !info)
+            invalid(std::string(name) + ": empty or null inputs");
         std::vector<DenseItem> items(count);
         int64_t off = 0;
         for (int i = 0; i < count; ++i) {
-            if (n[i] < 1 || nrhs[i] < 0) invalid("srcwa_zgesv_batched: bad size");
+            if (n[i] < 1 || nrhs[i] < 0) invalid(std::string(name) + ": bad size");
             DenseItem& it = items[i];
             it.n = n[i];
             it.nrhs = nrhs[i];
@@ -1205,6 +1234,7 @@ int srcwa_zgesv_batched(srcwa_context* ctx, int count, const int* n, const int* 
             off += align16(int64_t(n[i]) * nrhs[i]);
             it.scratch = off;
             off += align16(n[i]);
+            if (refine) off += align16(int64_t(n[i]) * n[i]) + refine_words(n[i], nrhs[i]);
         }
         DevBuf<cd> arena;
         arena.alloc(off);
@@ -1220,8 +1250,19 @@ int srcwa_zgesv_batched(srcwa_context* ctx, int count, const int* n, const int* 
             ao += as;
             bo += bs;
         }
-        launch_zgesv_batched(s, count, d_items.p, arena.p, d_info.p);
+        DevBuf<double> d_refine;
+        if (refine) {
+#if SRCWA_MIXED_PRECISION
+            d_refine.alloc(size_t(3) * count);
+            launch_zgesv_mixed_batched(s, count, d_items.p, arena.p, d_info.p, d_refine.p);
+#endif
+        } else {
+            launch_zgesv_batched(s, count, d_items.p, arena.p, d_info.p);
+        }
         hip_check(hipGetLastError(), "kernel launch");
+        if (refine)
+            hip_check(hipMemcpyAsync(refine, d_refine.p, size_t(3) * count * sizeof(double), hipMemcpyDeviceToHost, s),
+                      "D2H");
         bo = 0;
         for (int i = 0; i < count; ++i) {
             const int64_t bs = int64_t(n[i]) * nrhs[i];
@@ -1231,6 +1272,19 @@ int srcwa_zgesv_batched(srcwa_context* ctx, int count, const int* n, const int* 
         hip_check(hipMemcpyAsync(info, d_info.p, count * sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
         hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
     });
+}
+}  // namespace
+
+int srcwa_zgesv_batched(srcwa_context* ctx, int count, const int* n, const int* nrhs, const double* A, const double* B,
+                        double* X, int* info) {
+    return zgesv_batched(ctx, count, n, nrhs, A, B, X, info, nullptr);
+}
+
+int srcwa_zgesv_batched_mixed(srcwa_context* ctx, int count, const int* n, const int* nrhs, const double* A,
+                              const double* B, double* X, int* info, double* refine) {
+    if (!SRCWA_MIXED_PRECISION) return guarded([&] { throw Failure{SRCWA_EUNSUPPORTED, kMixedOff}; });
+    if (!refine) return guarded([&] { invalid("srcwa_zgesv_batched_mixed: null refinement output"); });
+    return zgesv_batched(ctx, count, n, nrhs, A, B, X, info, refine);
 }
 
 int srcwa_zgeev_batched(srcwa_context* ctx, int count, const int* n, const double* A, double* w, double* V, int* info) {
@@ -1562,7 +1616,7 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
                     if (diag) w += align16(4 * NN);      // a copy of H for the eigen residual
                 }
             const int64_t ws_lo = w;
-            w += cost.bump_words;
+            w += cost.bump_words + ((plan->flags & PLAN_MIXED) ? mixed_solve_words(2 * N, 2 * N) : 0);
             if (i > cur.i0 && chunk_ws + w > budget_words) {
                 cur.i1 = i;
                 cur.t1 = static_cast<int>(h_tasks.size());

@@ -38,6 +38,23 @@
 #ifndef SRCWA_EIG_QUEUE_LARGE
 #define SRCWA_EIG_QUEUE_LARGE 4
 #endif
+// orders above 256: columns per lane of the wave that runs each stage of the left rotation chain (a stage
+// is 64 SRCWA_EIG_STAGE_LC rotations; 0 publishes every rotation through LDS with a workgroup barrier
+// instead), and the rows that wave prefetches ahead
+#ifndef SRCWA_EIG_STAGE_LC
+#define SRCWA_EIG_STAGE_LC 2
+#endif
+#ifndef SRCWA_EIG_STAGE_RING
+#define SRCWA_EIG_STAGE_RING 4
+#endif
+// 1: the mixed-precision star products (FP32 LU + FP64 refinement, wg_la32.hpp) are compiled into the
+// S-matrix kernels and the batched mixed solve.  0 (default): compiled out and SRCWA_PLAN_MIXED refused.
+// Their first GPU run ended in a memory fault whose cause is not found: the same code runs every shape of
+// that run cleanly in a CPU emulation under AddressSanitizer (FP32 path and FP64 fallback), and it is the
+// only code here that makes a real device function call.  Off until the cause is shown.
+#ifndef SRCWA_MIXED_PRECISION
+#define SRCWA_MIXED_PRECISION 0
+#endif
 // minimum waves per SIMD the S-matrix kernels are built for
 #ifndef SRCWA_SMAT_OCC
 #define SRCWA_SMAT_OCC 2

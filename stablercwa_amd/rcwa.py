@@ -107,6 +107,13 @@ class PlanarBatch:
         _lib.check(self.lib.srcwa_plan_diagnostics(self.handle, _lib.ptr(out)))
         return out
 
+    def refinement(self) -> np.ndarray:
+        """Per problem [FP64 fallbacks, FP32 correction steps, largest final refined residual] of the last
+        execute's mixed-precision solves (a context created with plan_flags=PLAN_MIXED; zeros otherwise)."""
+        out = np.zeros((self.n, 3))
+        _lib.check(self.lib.srcwa_plan_refinement(self.handle, _lib.ptr(out)))
+        return out
+
     def stage_ms(self) -> np.ndarray:
         ms = np.zeros(4)
         _lib.check(self.lib.srcwa_plan_stage_ms(self.handle, _lib.ptr(ms)))
@@ -340,20 +347,30 @@ def zgeev_batched(mats, ctx: _lib.Context | None = None):
     return out
 
 
-def zgesv_batched(As, Bs, ctx: _lib.Context | None = None):
-    """X_i = A_i^{-1} B_i (custom_solve shape): returns [(X, info)]."""
+def zgesv_batched(As, Bs, ctx: _lib.Context | None = None, mixed: bool = False):
+    """X_i = A_i^{-1} B_i (custom_solve shape): returns [(X, info)].  mixed=True: FP32 LU + FP64 iterative
+    refinement (srcwa_zgesv_batched_mixed), returning [(X, info, (fell_back, corrections, residual))]."""
     ctx = ctx or _lib.default_context()
     n = np.array([a.shape[0] for a in As], dtype=np.int32)
     nrhs = np.array([b.shape[1] for b in Bs], dtype=np.int32)
     A, B = _pack(As), _pack(Bs)
     X = np.zeros_like(B); info = np.zeros(len(As), dtype=np.int32)
-    _lib.check(_lib.load().srcwa_zgesv_batched(ctx.handle, len(As), _lib.ptr(n), _lib.ptr(nrhs), _lib.ptr(A),
-                                               _lib.ptr(B), _lib.ptr(X), _lib.ptr(info)))
+    lib = _lib.load()
+    if mixed:
+        ref = np.zeros(3 * len(As))
+        _lib.check(lib.srcwa_zgesv_batched_mixed(ctx.handle, len(As), _lib.ptr(n), _lib.ptr(nrhs), _lib.ptr(A),
+                                                 _lib.ptr(B), _lib.ptr(X), _lib.ptr(info), _lib.ptr(ref)))
+    else:
+        _lib.check(lib.srcwa_zgesv_batched(ctx.handle, len(As), _lib.ptr(n), _lib.ptr(nrhs), _lib.ptr(A),
+                                           _lib.ptr(B), _lib.ptr(X), _lib.ptr(info)))
     Xc = X.view(np.complex128)
     out, a = [], 0
     for i in range(len(As)):
         k, r = int(n[i]), int(nrhs[i])
-        out.append((Xc[a:a + k * r].reshape((k, r), order="F").copy(), int(info[i])))
+        item = (Xc[a:a + k * r].reshape((k, r), order="F").copy(), int(info[i]))
+        if mixed:
+            item = item + ((int(ref[3 * i]), int(ref[3 * i + 1]), float(ref[3 * i + 2])),)
+        out.append(item)
         a += k * r
     return out
 
