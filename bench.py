@@ -231,11 +231,12 @@ def mixed_form(make_batch, fp64_res, n, per_result):
     ref = b.refinement()
     b.close()
     ctx.close()
-    out.update(failed=int(np.sum(res["status"] != 0)),
-               max_abs_dDE_vs_fp64=float(np.max(d)), p99_abs_dDE_vs_fp64=float(np.percentile(d, 99)),
-               results_above_1e-9=int(np.sum(d > 1e-9)), within_1e-9=bool(np.max(d) <= 1e-9),
-               refine_fallback_solves=int(ref[:, 0].sum()), refine_corrections_per_result=float(ref[:, 1].mean()),
-               refine_max_residual=float(ref[:, 2].max()))
+    out.update({"failed": int(np.sum(res["status"] != 0)),
+                "max_abs_dDE_vs_fp64": float(np.max(d)), "p99_abs_dDE_vs_fp64": float(np.percentile(d, 99)),
+                "results_above_1e-9": int(np.sum(d > 1e-9)), "within_1e-9": bool(np.max(d) <= 1e-9),
+                "refine_fallback_solves": int(ref[:, 0].sum()),
+                "refine_corrections_per_result": float(ref[:, 1].mean()),
+                "refine_max_residual": float(ref[:, 2].max())})
     return out
 
 

@@ -5,7 +5,7 @@
 
 Each family source (rcwa1d.hip, conical.hip, rcwa2d.hip, dense.hip) is compiled as several units: one
 "main" unit (its light kernels and host launcher), one for its S-matrix kernel (-DSRCWA_PART_ONLY=1)
-and one per eigen-kernel instantiation QC in {1, 2, 4, 8, 16} (-DSRCWA_QC_ONLY=QC), so the slow units
+and one per eigen-kernel instantiation QC in {1, 2, 4, 8, 12, 16} (-DSRCWA_QC_ONLY=QC), so the slow units
 build in parallel.
 """
 from __future__ import annotations
@@ -31,7 +31,7 @@ JSON_DRIVER = os.path.join(OBJ, "json_driver")
 
 FAMILIES = ["rcwa1d.hip", "conical.hip", "rcwa2d.hip", "dense.hip"]
 HOST_SOURCES = ["runtime.cpp", "server.cpp"]
-QCS = (1, 2, 4, 8, 16)
+QCS = (1, 2, 4, 8, 12, 16)
 # (source, unit): 0 = main, "p1" = the S-matrix part, QC = that eigen instantiation
 UNITS = ([(f, 0) for f in FAMILIES] + [(f, "p1") for f in FAMILIES if f != "dense.hip"] +
          [(f, q) for f in FAMILIES for q in QCS] + [(f, 0) for f in HOST_SOURCES])

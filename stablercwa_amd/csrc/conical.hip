@@ -274,8 +274,10 @@ __global__ __launch_bounds__(NT) void k_eigres_conical(const Inst1D* __restrict_
 template <int QC>
 __global__ EIG_KERNEL_BOUNDS_OF(SRCWA_EIG_OCC_CONICAL) void k_eig_conical(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                                     const int64_t* __restrict__ layouts, cd* __restrict__ ws,
-                                                    int* __restrict__ task_info, int max_iter) {
-    const int t = blockIdx.x >> 1, which = blockIdx.x & 1;
+                                                    int* __restrict__ task_info, int max_iter, const int* __restrict__ order) {
+    // the two eigensolves of a layer task are neighbouring blocks; tasks largest first
+    const int b = blockIdx.x >> 1, which = blockIdx.x & 1;
+    const int t = order ? order[b] : b;
     if (task_info[4 * t] != 0) return;
     const LayerTask tk = tasks[t];
     const Inst1D in = insts[tk.inst];
@@ -619,7 +621,7 @@ void SRCWA_QC_FN(conical_qc)(const QcLaunch& a) {
     constexpr int QC = SRCWA_QC_ONLY;
     const Inst1D* insts = static_cast<const Inst1D*>(a.insts);
     if (a.what == 0) {
-        k_eig_conical<QC><<<a.grid, NT, 0, a.stream>>>(insts, a.tasks, a.layouts, a.ws, a.task_info, a.max_iter);
+        k_eig_conical<QC><<<a.grid, NT, 0, a.stream>>>(insts, a.tasks, a.layouts, a.ws, a.task_info, a.max_iter, a.order);
     } else if (a.what == 1) {
         if constexpr (QC <= 4)
             k_eig_conical_wave<QC><<<a.grid, NT, 0, a.stream>>>(a.n_items, insts, a.tasks, a.layouts, a.ws,
@@ -781,7 +783,7 @@ void smatrix_conical_launch(int grid, hipStream_t s, const Inst1D* insts, const 
 #if SRCWA_MAIN_TU
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    int part, int flags, hipEvent_t* ev) {
+                    int part, int flags, hipEvent_t* ev, const int* eig_order) {
     const bool diag = flags & PLAN_DIAGNOSTICS;
     // part 1: setup, assembly, eigensolves (events ev[0..2]); part 2: modes, S-matrix, efficiencies
     // (events ev[3], ev[4]); the host may run the parts of different chunks on two streams
@@ -794,6 +796,7 @@ void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts,
     a.ws = ws;
     a.task_info = task_info;
     a.max_iter = max_iter;
+    a.order = eig_order;
     if (part & 1) {
         if (ev) hipEventRecord(ev[0], s);
         k_setup_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);

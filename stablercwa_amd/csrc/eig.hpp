@@ -34,12 +34,15 @@ constexpr int EIG_MAX_N = EIG_MAX_N_HOST;   // largest order the LDS rotation ta
 constexpr int EIG_RING = SRCWA_EIG_RING;    // rows each column prefetches ahead in the QR sweep
 
 // Columns per lane of one wave for matrices of order n (QC): the eigen kernels are instantiated
-// for 1, 2, 4, 8 and 16 so small matrices do not pay the registers of the largest.
-inline int eig_columns_per_thread(int n) { return n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 512 ? 8 : 16; }
+// for 1, 2, 4, 8, 12 and 16 so small matrices do not pay the registers and LDS of the largest (12: orders
+// up to 768, whose rotation table leaves room for two workgroups per CU; the crossed gratings' 722).
+inline int eig_columns_per_thread(int n) {
+    return n <= 64 ? 1 : n <= 128 ? 2 : n <= 256 ? 4 : n <= 512 ? 8 : n <= 768 ? 12 : 16;
+}
 
 // Translation-unit split.  The QC-templated kernels of a family dominate its compile time, so each
 // family source is compiled once without SRCWA_QC_ONLY (its other kernels and its host launcher, the
-// "main" unit) and once per QC in {1, 2, 4, 8, 16} with SRCWA_QC_ONLY = QC (that QC's template kernels
+// "main" unit) and once per QC in {1, 2, 4, 8, 12, 16} with SRCWA_QC_ONLY = QC (that QC's template kernels
 // only); the units build in parallel.  The main unit reaches the per-QC units through plain host
 // functions NAME_q1 .. NAME_q16 that take one argument struct, declared by SRCWA_QC_DISPATCH together
 // with the dispatcher NAME(qc, args).
@@ -62,6 +65,7 @@ inline int eig_columns_per_thread(int n) { return n <= 64 ? 1 : n <= 128 ? 2 : n
     void NAME##_q2(const ARGS&);                                                          \
     void NAME##_q4(const ARGS&);                                                          \
     void NAME##_q8(const ARGS&);                                                          \
+    void NAME##_q12(const ARGS&);                                                         \
     void NAME##_q16(const ARGS&);                                                         \
     inline void NAME(int qc, const ARGS& a) {                                             \
         switch (qc) {                                                                     \
@@ -69,6 +73,7 @@ inline int eig_columns_per_thread(int n) { return n <= 64 ? 1 : n <= 128 ? 2 : n
             case 2: NAME##_q2(a); break;                                                  \
             case 4: NAME##_q4(a); break;                                                  \
             case 8: NAME##_q8(a); break;                                                  \
+            case 12: NAME##_q12(a); break;                                                \
             default: NAME##_q16(a); break;                                                \
         }                                                                                 \
     }
@@ -91,6 +96,7 @@ struct QcLaunch {
     int* info;
     cd* wbase;                // dense eigensolves: base of the eigenvalue offsets (DenseItem::w)
     cd* sbase;                // dense eigensolves: base of the scratch offsets (DenseItem::scratch)
+    const int* order;         // layer eigensolves: the launch's tasks, longest first (block b solves task order[b])
 };
 
 struct EigScratch {
@@ -537,7 +543,12 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
     cd* H = A;
     cd* ort = s.ort;
     auto h = [&](int i, int j) -> cd& { return H[i + (long)j * n]; };
-    __shared__ cd s_u[64 * QC];     // the current Householder vector
+    // One LDS block holds the current Householder vector in steps 1-2 and the queued sweeps' rotations in
+    // step 4 (never live at once), so the largest instantiations fit two workgroups per CU.
+    constexpr int KQ = eig_queue_sweeps(QC);
+    constexpr int QROT_WORDS = 3 * KQ * (64 * QC + 1), U_WORDS = 2 * 64 * QC;
+    __shared__ __align__(16) double s_lds[QROT_WORDS > U_WORDS ? QROT_WORDS : U_WORDS];
+    cd* const s_u = reinterpret_cast<cd*>(s_lds);
 
     // ---- 1. Householder reduction to Hessenberg form (CORTH)
     for (int m = 1; m < n - 1; ++m) {
@@ -675,8 +686,7 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
     // the threads; the right rotations of H walk each row from one thread.  Z stays column-major.
     for (int e = tid; e < n * n; e += NT) X[e] = H[(e / n) + (e % n) * n];
     __syncthreads();
-    constexpr int KQ = eig_queue_sweeps(QC);
-    __shared__ double s_qrot[3 * KQ * (64 * QC + 1)];
+    double* const s_qrot = s_lds;
     __shared__ int s_qlo[KQ], s_qhi[KQ];
     __shared__ double s_qphase[2 * KQ];
     __shared__ int s_info;

@@ -538,16 +538,17 @@ __global__ __launch_bounds__(NT) void k_eigres2d(const Inst2D* __restrict__ inst
 template <int QC>
 __global__ EIG_KERNEL_BOUNDS_OF(SRCWA_EIG_OCC_CROSSED) void k_eig2d(const Inst2D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                               const int64_t* __restrict__ layouts, cd* __restrict__ ws,
-                                              int* __restrict__ task_info, int max_iter) {
-    if (task_info[4 * blockIdx.x] != 0) return;
-    const LayerTask tk = tasks[blockIdx.x];
+                                              int* __restrict__ task_info, int max_iter, const int* __restrict__ order) {
+    const int t = order ? order[blockIdx.x] : static_cast<int>(blockIdx.x);   // largest matrices first
+    if (task_info[4 * t] != 0) return;
+    const LayerTask tk = tasks[t];
     const Inst2D in = insts[tk.inst];
     const int64_t* lay = layouts + in.layout_off;
     if (ws[lay[0] + 4 * in.N].x != 0.0) return;
     if (2 * in.N <= WAVE_EIG_MAX_N) return;        // the wave eigensolver's matrix
     const Slots2D s = slots2d(ws, lay, in.L, tk.layer, in.N);
     const int info = wg_eig<QC>(2 * in.N, s.H, s.lam, s.Z, s.X, EigScratch{s.ort, s.tmp}, max_iter, kEps);
-    if (threadIdx.x == 0) task_info[4 * blockIdx.x + 1] = info;
+    if (threadIdx.x == 0) task_info[4 * t + 1] = info;
 }
 
 // One wave per layer eigenproblem of order 2N <= WAVE_EIG_MAX_N, four per workgroup.
@@ -577,7 +578,7 @@ void SRCWA_QC_FN(crossed_qc)(const QcLaunch& a) {
     constexpr int QC = SRCWA_QC_ONLY;
     const Inst2D* insts = static_cast<const Inst2D*>(a.insts);
     if (a.what == 0)
-        k_eig2d<QC><<<a.grid, NT, 0, a.stream>>>(insts, a.tasks, a.layouts, a.ws, a.task_info, a.max_iter);
+        k_eig2d<QC><<<a.grid, NT, 0, a.stream>>>(insts, a.tasks, a.layouts, a.ws, a.task_info, a.max_iter, a.order);
     else if constexpr (QC <= 4)
         k_eig2d_wave<QC><<<a.grid, NT, 0, a.stream>>>(a.n_items, insts, a.tasks, a.layouts, a.ws, a.task_info,
                                                       a.max_iter);
@@ -1245,7 +1246,7 @@ void smatrix2d_launch(int grid, hipStream_t s, const Inst2D* insts, const double
 #if SRCWA_MAIN_TU
 void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    int part, int flags, hipEvent_t* ev) {
+                    int part, int flags, hipEvent_t* ev, const int* eig_order) {
     const bool diag = flags & PLAN_DIAGNOSTICS;
     // part 1: setup, assembly, eigensolves (events ev[0..2]); part 2: modes, S-matrix, efficiencies
     // (events ev[3], ev[4]); the host may run the parts of different chunks on two streams
@@ -1267,6 +1268,7 @@ void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts,
         a.ws = ws;
         a.task_info = task_info;
         a.max_iter = max_iter;
+        a.order = eig_order;
         if (n_tasks > 0 && 2 * min_n <= WAVE_EIG_MAX_N) {
             a.what = 1;
             a.grid = (n_tasks + NWAVE - 1) / NWAVE;

@@ -27,10 +27,10 @@ void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, 
                    int part, int flags, hipEvent_t* ev);
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    int part, int flags, hipEvent_t* ev);
+                    int part, int flags, hipEvent_t* ev, const int* eig_order);
 void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    int part, int flags, hipEvent_t* ev);
+                    int part, int flags, hipEvent_t* ev, const int* eig_order);
 void launch_zgemm_batched(hipStream_t s, int count, const DenseItem* items, cd* arena);
 void launch_zgesv_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info);
 #if SRCWA_MIXED_PRECISION
@@ -169,6 +169,22 @@ struct DevBuf {
 };
 
 int64_t align16(int64_t x) { return (x + 15) & ~int64_t(15); }
+
+// Per chunk, the chunk-local indices of its layer tasks ordered by decreasing eigen order (stable), so the
+// eigen kernels start the longest eigensolves first instead of leaving them for the end of the launch.
+template <class Chunk, class Inst>
+std::vector<int> longest_first(const std::vector<Chunk>& chunks, const std::vector<LayerTask>& tasks,
+                               const std::vector<Inst>& insts) {
+    std::vector<int> order(std::max<size_t>(1, tasks.size()), 0);
+    for (const Chunk& ch : chunks) {
+        auto first = order.begin() + ch.t0, last = order.begin() + ch.t1;
+        for (int t = ch.t0; t < ch.t1; ++t) order[t] = t - ch.t0;
+        std::stable_sort(first, last, [&](int a, int b) {
+            return insts[ch.i0 + tasks[ch.t0 + a].inst].N > insts[ch.i0 + tasks[ch.t0 + b].inst].N;
+        });
+    }
+    return order;
+}
 
 // pool bytes of a plan's diagnostics buffers (per-instance records, per-task records, residuals)
 int64_t diag_bytes(int n_inst, int n_tasks) {
@@ -415,6 +431,7 @@ struct srcwa_plan {
     DevBuf<int64_t> layouts;
     DevBuf<cd> ws;
     DevBuf<int> task_info;
+    DevBuf<int> eig_order;          // conical / crossed: each chunk's tasks, largest eigensolve first
     DevBuf<double> R, T, DEr, DEt;
     DevBuf<double> geometry;      // conical / crossed: GEO_WORDS per order
     DevBuf<int> status, eig_info;
@@ -455,6 +472,7 @@ struct srcwa_plan {
         layouts.pool = p;
         ws.pool = p;
         task_info.pool = p;
+        eig_order.pool = p;
         R.pool = p;
         T.pool = p;
         DEr.pool = p;
@@ -534,11 +552,11 @@ void enqueue(srcwa_plan* plan, bool timing) {
         else if (plan->kind == 2)
             launch_conical(st, ni, nt, plan->insts.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p, plan->layouts.p,
                            ws, plan->task_info.p + 4 * ch.t0, out, plan->max_iter, plan->min_n, plan->max_n, part,
-                           plan->flags, ev);
+                           plan->flags, ev, plan->eig_order.p + ch.t0);
         else
             launch_crossed(st, ni, nt, plan->insts2d.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p,
                            plan->layouts.p, ws, plan->task_info.p + 4 * ch.t0, out, plan->max_iter, plan->min_n,
-                           plan->max_n, part, plan->flags, ev);
+                           plan->max_n, part, plan->flags, ev, plan->eig_order.p + ch.t0);
         hip_check(hipGetLastError(), "kernel launch");
     };
     for (size_t c = 0; c < nc; ++c) {
@@ -923,7 +941,7 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
         if (need) {
             const auto b = [](size_t x) { return static_cast<int64_t>(srcwa_pool::round(x)); };
             *need = b(h_insts.size() * sizeof(Inst1D)) + b(h_tasks.size() * sizeof(LayerTask)) +
-                    b(ps.words.size() * sizeof(double)) + b(h_layouts.size() * sizeof(int64_t)) +
+                    b(std::max<size_t>(1, h_tasks.size()) * sizeof(int)) + b(ps.words.size() * sizeof(double)) + b(h_layouts.size() * sizeof(int64_t)) +
                     b((plan->chunks.size() > 1 ? 2 : 1) * max_ws * sizeof(cd)) + b(4 * std::max(1, plan->n_tasks) * sizeof(int)) +
                     2 * b(8 * orders * sizeof(double)) + 2 * b(4 * orders * sizeof(double)) +
                     b(GEO_WORDS * orders * sizeof(double)) + 2 * b(n_problems * sizeof(int)) +
@@ -933,6 +951,7 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
         hipStream_t st = ctx->stream;
         plan->insts.upload(h_insts, st);
         plan->tasks.upload(h_tasks, st);
+        plan->eig_order.upload(longest_first(plan->chunks, h_tasks, h_insts), st);
         plan->stacks.upload(ps.words, st);
         plan->layouts.upload(h_layouts, st);
         plan->region_words = max_ws;
@@ -1667,6 +1686,7 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         if (need) {
             const auto b = [](size_t x) { return static_cast<int64_t>(srcwa_pool::round(x)); };
             *need = b(h_insts.size() * sizeof(Inst2D)) + b(h_tasks.size() * sizeof(LayerTask)) +
+                    b(std::max<size_t>(1, h_tasks.size()) * sizeof(int)) +
                     b(h_stacks.size() * sizeof(double)) + b(h_layouts.size() * sizeof(int64_t)) +
                     b((plan->chunks.size() > 1 ? 2 : 1) * max_ws * sizeof(cd)) + b(4 * std::max(1, plan->n_tasks) * sizeof(int)) +
                     2 * b(8 * orders * sizeof(double)) + 2 * b(orders * sizeof(double)) +
@@ -1677,6 +1697,7 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         hipStream_t st = ctx->stream;
         plan->insts2d.upload(h_insts, st);
         plan->tasks.upload(h_tasks, st);
+        plan->eig_order.upload(longest_first(plan->chunks, h_tasks, h_insts), st);
         plan->stacks.upload(h_stacks, st);
         plan->layouts.upload(h_layouts, st);
         plan->region_words = max_ws;

@@ -39,10 +39,12 @@
 #define SRCWA_EIG_QUEUE_LARGE 4
 #endif
 // orders above 256: columns per lane of the wave that runs each stage of the left rotation chain (a stage
-// is 64 SRCWA_EIG_STAGE_LC rotations; 0 publishes every rotation through LDS with a workgroup barrier
-// instead), and the rows that wave prefetches ahead
+// is 64 SRCWA_EIG_STAGE_LC rotations), and the rows that wave prefetches ahead; 0 (default) publishes every
+// rotation through LDS with a workgroup barrier instead.  Measured on the 512-pillar batch (orders 242 to
+// 722): staged chain with 2 columns per lane 8.19 s of eigensolves against 7.36 s for the barrier form --
+// the chain's dependent rotation latency, not the barrier, sets the time
 #ifndef SRCWA_EIG_STAGE_LC
-#define SRCWA_EIG_STAGE_LC 2
+#define SRCWA_EIG_STAGE_LC 0
 #endif
 #ifndef SRCWA_EIG_STAGE_RING
 #define SRCWA_EIG_STAGE_RING 4
