@@ -527,6 +527,229 @@ DEV void reflect_columns(cd* M, long ld, int r0, int len, int j0, int n, const c
     __syncthreads();
 }
 
+// Reflector blocks of the blocked Householder steps.
+constexpr int HB = 16;
+// Orders from which the blocked forms run (their scratch, 3 n HB words, must fit in one n x n matrix).
+constexpr int HB_MIN_N = 3 * HB;
+
+// Z = P_1 ... P_{n-2} (the CORTH reflectors, step 2) in blocks of HB reflectors, last block first: the block
+// P_{i0} ... P_{i1} = I + V T V^H (compact WY form, T upper triangular from the recurrence
+// T(0:k, k) = tau_k T(0:k, 0:k) V(:, 0:k)^H v_k, T(k, k) = tau_k) is applied to Z(i0:n, i0:n) as two MFMA
+// products, W = V^H Z, Z += V (T W).  Reflector i is v = 2^-e (ort[i], h(i+1, i-1), ..., h(n-1, i-1)) with
+// tau = 1 / Re(conj(2^-e h(i, i-1)) v_0), the same reflector as the one-at-a-time form builds (a skipped
+// reflector is the identity, tau = 0).  scratch: 3 n HB words (V n x HB, V^H and W HB x n); t: HB x HB in LDS.
+DEV void accumulate_reflectors_blocked(int n, const cd* __restrict__ H, const cd* __restrict__ ort, cd* __restrict__ Z,
+                                       cd* __restrict__ scratch, cd* __restrict__ t) {
+    const int tid = threadIdx.x;
+    cd* V = scratch;                        // n x HB, column-major
+    cd* VH = scratch + (long)n * HB;        // HB x n, V^H
+    cd* W = scratch + 2 * (long)n * HB;     // HB x n
+    __shared__ double s_tau[HB];
+    __shared__ int s_exp[HB];
+    for (int i1 = n - 2; i1 >= 1; i1 -= HB) {
+        const int i0 = max(1, i1 - HB + 1), b = i1 - i0 + 1;
+        const int rows = n - i0;
+        // reflector scalings, one thread per reflector
+        if (tid < b) {
+            const int i = i0 + tid;
+            const cd oi = ort[i], sub0 = H[i + (long)(i - 1) * n];
+            double tau = 0.0;
+            int e = 0;
+            if (!is_zero(oi) && !is_zero(sub0)) {
+                e = binary_exponent(abs1(oi));
+                const cd sub = scale2(sub0, -e), v0 = scale2(oi, -e);
+                tau = 1.0 / (sub.x * v0.x + sub.y * v0.y);
+            }
+            s_tau[tid] = tau;
+            s_exp[tid] = e;
+        }
+        __syncthreads();
+        // V (rows i0..n-1) and V^H
+        for (int e = tid; e < rows * b; e += NT) {
+            const int r = i0 + e % rows, k = e / rows, i = i0 + k;
+            cd v{0.0, 0.0};
+            if (s_tau[k] != 0.0 && r >= i) v = scale2(r == i ? ort[i] : H[r + (long)(i - 1) * n], -s_exp[k]);
+            V[r + (long)k * n] = v;
+            VH[k + (long)r * HB] = conj(v);
+        }
+        __syncthreads();
+        // G = V^H V into W (HB x HB), then T by one thread
+        wg_gemm<GEMM_SET, false>(b, b, rows, VH + (long)i0 * HB, HB, V + i0, n, W, HB);
+        if (tid == 0) {
+            for (int k = 0; k < b; ++k) {
+                const double tau = s_tau[k];
+                for (int r = 0; r < k; ++r) {
+                    cd s{0.0, 0.0};
+                    for (int l = r; l < k; ++l) s = s + t[r + l * HB] * W[l + k * HB];
+                    t[r + k * HB] = tau * s;
+                }
+                t[k + k * HB] = cd{tau, 0.0};
+                for (int r = k + 1; r < b; ++r) t[r + k * HB] = cd{0.0, 0.0};
+            }
+        }
+        __syncthreads();
+        // W = V^H Z(i0:n, i0:n);  W = T W (one column per thread);  Z(i0:n, i0:n) += V W
+        wg_gemm<GEMM_SET, false>(b, rows, rows, VH + (long)i0 * HB, HB, Z + i0 + (long)i0 * n, n, W, HB);
+        for (int c = tid; c < rows; c += NT) {
+            cd w[HB];
+#pragma unroll
+            for (int k = 0; k < HB; ++k) w[k] = k < b ? W[k + (long)c * HB] : cd{0.0, 0.0};
+#pragma unroll
+            for (int r = 0; r < HB; ++r) {
+                cd s{0.0, 0.0};
+#pragma unroll
+                for (int k = 0; k < HB; ++k)
+                    if (k >= r && k < b) s = s + t[r + k * HB] * w[k];
+                if (r < b) W[r + (long)c * HB] = s;
+            }
+        }
+        __syncthreads();
+        wg_gemm<GEMM_ADD, false>(rows, rows, b, V + i0, n, W, HB, Z + i0 + (long)i0 * n, n);
+    }
+}
+
+// Orders from which the blocked reduction runs (its scratch, 4 n HB + 2 HB words, fits in one n x n matrix).
+constexpr int HB_MIN_N_REDUCE = 72;
+
+// Step 1 (CORTH) in panels of HB reflectors.  Within a panel, step m = m0 + k needs column c = m - 1 of
+// Q_k^H A Q_k (Q_k = P_{m0} ... P_{m-1} = I - V T V^H, compact WY form; A the matrix at the panel's start):
+// its right part is A(:, c) - Y conj(V(c, :))^T with Y = A V T, its left part then subtracts
+// V T^H V^H (rows >= m0).  From that column the step forms the reflector exactly as the one-at-a-time
+// reduction (scale, u, hh, the modified first entry, the new sub-diagonal, ort[m], and the column's entries
+// below the sub-diagonal left as they were), writes the now final column c, and extends
+// Y by y_k = tau_k (A(:, m:n) u_k - Y V^H u_k) and T by T(0:k, k) = -tau_k T V^H u_k.  After the panel the
+// trailing columns J >= m0 + b - 1 get A Q (A -= Y V^H) and then Q^H A (A -= V T^H (V^H A)), three MFMA
+// products.  Every column of the matrix is read once per reflector (for y_k) instead of the two read-write
+// passes per side of the one-at-a-time form.  wcol: n words; scratch: 4 n HB + 2 HB words; t: HB x HB in LDS.
+DEV void reduce_hessenberg_blocked(int n, cd* __restrict__ A, cd* __restrict__ ort, cd* __restrict__ wcol,
+                                   cd* __restrict__ scratch, cd* __restrict__ t) {
+    const int tid = threadIdx.x;
+    cd* V = scratch;                        // n x HB
+    cd* VH = V + (long)n * HB;              // HB x n
+    cd* Y = VH + (long)n * HB;              // n x HB
+    cd* W = Y + (long)n * HB;               // HB x n
+    cd* sv = W + (long)n * HB;              // HB
+    cd* pv = sv + HB;                       // HB
+    __shared__ cd s_sp[HB];
+    auto a = [&](int i, int j) -> cd& { return A[i + (long)j * n]; };
+    for (int m0 = 1; m0 <= n - 2; m0 += HB) {
+        const int b = min(HB, n - 1 - m0);
+        for (int e = tid; e < n * HB; e += NT) {
+            V[e] = cd{0.0, 0.0};
+            VH[e] = cd{0.0, 0.0};
+        }
+        __syncthreads();
+        for (int k = 0; k < b; ++k) {
+            const int m = m0 + k, c = m - 1;
+            // (a) the column with the panel's earlier right reflectors applied
+            for (int i = tid; i < n; i += NT) {
+                cd v = a(i, c);
+                for (int j = 0; j < k; ++j) v = v - Y[i + (long)j * n] * VH[j + (long)c * HB];
+                wcol[i] = v;
+            }
+            __syncthreads();
+            // (b) and the earlier left reflectors (rows >= m0): wcol -= V T^H (V^H wcol)
+            if (k > 0) {
+                wg_gemm<GEMM_SET, false>(k, 1, n - m0, VH + (long)m0 * HB, HB, wcol + m0, n, sv, HB);
+                if (tid < k) {
+                    cd s{0.0, 0.0};
+                    for (int j = 0; j <= tid; ++j) s = s + conj(t[j + tid * HB]) * sv[j];
+                    s_sp[tid] = s;
+                }
+                __syncthreads();
+                for (int i = m0 + tid; i < n; i += NT) {
+                    cd v = wcol[i];
+                    for (int j = 0; j < k; ++j) v = v - V[i + (long)j * n] * s_sp[j];
+                    wcol[i] = v;
+                }
+                __syncthreads();
+            }
+            // (c) the reflector from wcol(m:n), as the one-at-a-time reduction forms it
+            double part = 0.0;
+            for (int i = m + tid; i < n; i += NT) part += abs1(wcol[i]);
+            const double scale = block_sum(part);
+            double tau = 0.0;
+            if (scale == 0.0) {
+                if (tid == 0) ort[m] = cd{0.0, 0.0};
+                for (int i = tid; i < n; i += NT) a(i, c) = wcol[i];
+            } else {
+                double hp = 0.0;
+                for (int i = m + tid; i < n; i += NT) {
+                    const cd u = cdiv(wcol[i], scale);
+                    V[i + (long)k * n] = u;
+                    hp += norm2(u);
+                }
+                double hh = block_sum(hp);          // also the barrier: V(:, k) is complete
+                double g = sqrt(hh);
+                const cd u0 = V[m + (long)k * n];
+                const double f = cabs(u0);
+                cd sub, first;
+                if (f == 0.0) {
+                    first = cd{g, 0.0};
+                    sub = cd{-g * scale, 0.0};
+                } else {
+                    hh += f * g;
+                    g /= f;
+                    first = u0 * (1.0 + g);
+                    sub = wcol[m] * (-g);
+                }
+                tau = 1.0 / hh;
+                __syncthreads();                     // every thread has read V(m, k)
+                if (tid == 0) {
+                    V[m + (long)k * n] = first;
+                    ort[m] = first * scale;
+                }
+                // the final column c: rows above m as updated, the new sub-diagonal, the entries below left
+                for (int i = tid; i < n; i += NT) a(i, c) = i == m ? sub : wcol[i];
+                __syncthreads();
+                for (int i = m + tid; i < n; i += NT) VH[k + (long)i * HB] = conj(V[i + (long)k * n]);
+            }
+            __syncthreads();
+            // (d) p = V^H u_k,  y_k = tau (A(:, m:n) u_k - Y p)
+            if (k > 0) wg_gemm<GEMM_SET, false>(k, 1, n - m, VH + (long)m * HB, HB, V + m + (long)k * n, n, pv, HB);
+            for (int i = tid; i < n; i += NT) {
+                cd y{0.0, 0.0};
+                if (tau != 0.0) {
+                    for (int j = m; j < n; ++j) y = y + a(i, j) * V[j + (long)k * n];
+                    for (int l = 0; l < k; ++l) y = y - Y[i + (long)l * n] * pv[l];
+                    y = tau * y;
+                }
+                Y[i + (long)k * n] = y;
+            }
+            // (e) T(0:k, k) = -tau T(0:k, 0:k) p,  T(k, k) = tau
+            if (tid == 0) {
+                for (int r = 0; r < k; ++r) {
+                    cd s{0.0, 0.0};
+                    for (int l = r; l < k; ++l) s = s + t[r + l * HB] * pv[l];
+                    t[r + k * HB] = -tau * s;
+                }
+                t[k + k * HB] = cd{tau, 0.0};
+                for (int r = k + 1; r < HB; ++r) t[r + k * HB] = cd{0.0, 0.0};
+            }
+            __syncthreads();
+        }
+        // trailing columns J0..n-1:  A -= Y V^H;  then rows m0..n-1:  A -= V T^H (V^H A)
+        const int J0 = m0 + b - 1, nj = n - J0;
+        wg_gemm<GEMM_SUB, false>(n, nj, b, Y, n, VH + (long)J0 * HB, HB, A + (long)J0 * n, n);
+        wg_gemm<GEMM_SET, false>(b, nj, n - m0, VH + (long)m0 * HB, HB, A + m0 + (long)J0 * n, n, W, HB);
+        for (int col = tid; col < nj; col += NT) {
+            cd w[HB];
+#pragma unroll
+            for (int j = 0; j < HB; ++j) w[j] = j < b ? W[j + (long)col * HB] : cd{0.0, 0.0};
+#pragma unroll
+            for (int r = 0; r < HB; ++r) {
+                cd s{0.0, 0.0};
+#pragma unroll
+                for (int j = 0; j < HB; ++j)
+                    if (j <= r) s = s + conj(t[j + r * HB]) * w[j];
+                if (r < b) W[r + (long)col * HB] = s;
+            }
+        }
+        __syncthreads();
+        wg_gemm<GEMM_SUB, false>(n - m0, nj, b, V + m0, n, W, HB, A + m0 + (long)J0 * n, n);
+    }
+}
+
 // Eigen-decompose the n x n matrix A (column-major, leading dimension n) in place.
 //   in : A = the matrix (overwritten)
 //   out: A = right eigenvectors (unit 2-norm columns) when info == 0; w = eigenvalues
@@ -543,14 +766,18 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
     cd* H = A;
     cd* ort = s.ort;
     auto h = [&](int i, int j) -> cd& { return H[i + (long)j * n]; };
-    // One LDS block holds the current Householder vector in steps 1-2 and the queued sweeps' rotations in
-    // step 4 (never live at once), so the largest instantiations fit two workgroups per CU.
+    // One LDS block holds the current Householder vector and the blocked reflectors' T factor in steps 1-2
+    // and the queued sweeps' rotations in step 4 (never live at once), so the largest instantiations fit
+    // two workgroups per CU.
     constexpr int KQ = eig_queue_sweeps(QC);
-    constexpr int QROT_WORDS = 3 * KQ * (64 * QC + 1), U_WORDS = 2 * 64 * QC;
+    constexpr int QROT_WORDS = 3 * KQ * (64 * QC + 1), U_WORDS = 2 * 64 * QC + 2 * HB * HB;
     __shared__ __align__(16) double s_lds[QROT_WORDS > U_WORDS ? QROT_WORDS : U_WORDS];
     cd* const s_u = reinterpret_cast<cd*>(s_lds);
 
     // ---- 1. Householder reduction to Hessenberg form (CORTH)
+    if (SRCWA_EIG_BLOCKED_REDUCE && n >= HB_MIN_N_REDUCE)
+        reduce_hessenberg_blocked(n, H, ort, s.tmp, X, reinterpret_cast<cd*>(s_lds) + 64 * QC);
+    else
     for (int m = 1; m < n - 1; ++m) {
         cd* x = H + (long)(m - 1) * n;
         double part = 0.0;
@@ -624,6 +851,9 @@ DEV int wg_eig(int n, cd* __restrict__ A, cd* __restrict__ w, cd* __restrict__ Z
 
     // ---- 2. Z = P_1 ... P_{n-2}
     wg_identity(n, Z, n);
+    if (SRCWA_EIG_BLOCKED_Z && n >= HB_MIN_N) {
+        accumulate_reflectors_blocked(n, H, ort, Z, X, reinterpret_cast<cd*>(s_lds) + 64 * QC);
+    } else
     for (int i = n - 2; i >= 1; --i) {
         const cd oi = ort[i];
         const cd sub0 = h(i, i - 1);

@@ -57,6 +57,20 @@
 #ifndef SRCWA_MIXED_PRECISION
 #define SRCWA_MIXED_PRECISION 0
 #endif
+// 1: the eigensolver accumulates its Householder reflectors into Z in blocks of 16 (two MFMA products per
+// block) and runs the Hessenberg reduction in panels of 16 reflectors (three MFMA products per panel on the
+// trailing matrix, orders from 72) instead of one reflector at a time.  0 (default): one at a time.  The
+// blocked forms agree with the one-at-a-time forms in a CPU emulation of the workgroup (eigenvalues to
+// 1e-15, eigen-pair residuals 4e-16 to 8e-16, clean under AddressSanitizer, including the matrix below),
+// but their first GPU run ended in a memory fault on a planar layer matrix of order 101 (the reference's
+// kRidgesSilver50TE table case), and with them the eigen kernels spill 1.2k-2k VGPRs (2.2-2.5 KB of
+// scratch per lane against 0-44 B without); off until the cause is shown.
+#ifndef SRCWA_EIG_BLOCKED_Z
+#define SRCWA_EIG_BLOCKED_Z 0
+#endif
+#ifndef SRCWA_EIG_BLOCKED_REDUCE
+#define SRCWA_EIG_BLOCKED_REDUCE 0
+#endif
 // minimum waves per SIMD the S-matrix kernels are built for
 #ifndef SRCWA_SMAT_OCC
 #define SRCWA_SMAT_OCC 2

@@ -1,0 +1,118 @@
+"""The workgroup device routines run on the CPU: tests/emu/hip/hip_runtime.h emulates one 256-thread
+workgroup (host threads, a barrier for __syncthreads, per-wave exchange buffers for shuffles, readlane and the
+16x16x4 MFMAs), and tests/emu/emu_main.cpp drives the real headers of stablercwa_amd/csrc under
+AddressSanitizer.  What this pins:
+
+* wg_eig (the eigensolver every plan runs) on seeded random matrices: info 0, eigen-pair residual below
+  2e-15, eigenvalues equal to numpy's to 1e-12 of ||A||
+* the blocked Householder reduction and blocked Z accumulation (compiled out of the GPU build, see
+  stablercwa_amd/csrc/tuned.hpp) against the one-at-a-time forms: the same eigenvalues to 1e-13 and the same
+  residual level, also through a zero column and a zero leading entry, and on the reference's layer matrix of
+  the kRidgesSilver50TE table case (order 101) on which their GPU run faulted
+* the mixed-precision solve (wg_la32.hpp, also compiled out): FP64 backward error after a few FP32
+  corrections, no fallback on well-conditioned systems
+No GPU is used; an out-of-bounds access in the emulated code aborts the run under AddressSanitizer.
+"""
+import ctypes
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+CLANG = "/opt/rocm/llvm/bin/clang++"
+EMU = os.path.join(ROOT, "tests", "emu")
+CSRC = os.path.join(ROOT, "stablercwa_amd", "csrc")
+VARIANTS = {"plain": [], "blocked": ["-DSRCWA_EIG_BLOCKED_Z=1", "-DSRCWA_EIG_BLOCKED_REDUCE=1"],
+            "mixed": ["-DSRCWA_MIXED_PRECISION=1"]}
+
+
+@pytest.fixture(scope="module")
+def emu(tmp_path_factory):
+    if not os.path.exists(CLANG):
+        pytest.skip("no host clang++ for the emulator")
+    out = tmp_path_factory.mktemp("emu")
+    exe = {}
+    procs = []
+    for name, defs in VARIANTS.items():
+        exe[name] = str(out / f"emu_{name}")
+        cmd = [CLANG, "-std=c++20", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-Wno-psabi",
+               "-I", EMU, "-I", CSRC] + defs + [os.path.join(EMU, "emu_main.cpp"), "-o", exe[name], "-pthread"]
+        procs.append(subprocess.Popen(cmd, stderr=subprocess.PIPE, text=True))
+    for p in procs:
+        _, err = p.communicate(timeout=600)
+        assert p.returncode == 0, err[-3000:]
+    return exe
+
+
+def run(exe, *args):
+    r = subprocess.run([exe, *map(str, args)], capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, (args, r.stderr[-3000:])
+    out = {"w": []}
+    for line in r.stdout.splitlines():
+        k, *v = line.split()
+        if k == "w":
+            out["w"].append(complex(float(v[0]), float(v[1])))
+        else:
+            out[k] = float(v[0])
+    out["w"] = np.array(out["w"])
+    return out
+
+
+def sorted_eigs(A):
+    w = np.linalg.eigvals(A)
+    return w[np.lexsort((w.imag, w.real))]
+
+
+def test_eigensolver_random_matrices(emu, tmp_path):
+    rng = np.random.default_rng(5)
+    for qc, n in ((1, 40), (2, 90)):
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        f = tmp_path / f"a{n}.bin"
+        np.asfortranarray(A).ravel(order="F").astype(np.complex128).tofile(f)
+        o = run(emu["plain"], "eigfile", qc, n, f)
+        assert o["info"] == 0 and o["residual"] <= 2e-15, o["residual"]
+        assert np.max(np.abs(o["w"] - sorted_eigs(A))) <= 1e-12 * np.linalg.norm(A)
+
+
+@pytest.mark.parametrize("n, seed, special", [(72, 6, False), (80, 12, True)])
+def test_blocked_householder_matches_one_at_a_time(emu, n, seed, special):
+    extra = ["special"] if special else []
+    a = run(emu["plain"], "eig", 2, n, seed, *extra)
+    b = run(emu["blocked"], "eig", 2, n, seed, *extra)
+    assert a["info"] == 0 and b["info"] == 0
+    assert a["residual"] <= 2e-15 and b["residual"] <= 2e-15, (a["residual"], b["residual"])
+    scale = np.max(np.abs(a["w"]))
+    assert np.max(np.abs(a["w"] - b["w"])) <= 1e-13 * scale
+
+
+def test_blocked_householder_on_the_matrix_of_its_gpu_fault(emu, tmp_path):
+    from oracle import ref
+    if not ref.available():
+        pytest.skip("oracle/_ref not built")
+    lib = ref.load()
+    c = next(x for x in json.load(open(os.path.join(GOLDEN, "planar_tables.json")))["cases"]
+             if x["name"] == "kRidgesSilver50TE")
+    f = np.array(c["stack"], dtype=np.float64)
+    N = 2 * c["M"] + 1
+    A = np.zeros(2 * N * N)
+    assert lib.ref_layer_matrix(ctypes.c_void_p(f.ctypes.data), 0, ctypes.c_double(c["wavelength"]),
+                                ctypes.c_double(c["theta"]), c["M"], c["pol"], ctypes.c_void_p(A.ctypes.data)) == 0
+    path = tmp_path / "silver50.bin"
+    A.tofile(path)
+    a = run(emu["plain"], "eigfile", 2, N, path)
+    b = run(emu["blocked"], "eigfile", 2, N, path)
+    assert a["info"] == 0 and b["info"] == 0 and a["residual"] <= 2e-15 and b["residual"] <= 2e-15
+    Am = A.view(np.complex128).reshape((N, N), order="F")
+    assert np.max(np.abs(b["w"] - sorted_eigs(Am))) <= 1e-12 * np.linalg.norm(Am)
+
+
+@pytest.mark.parametrize("n, nrhs", [(1, 1), (33, 8), (64, 7), (100, 100)])
+def test_mixed_precision_solve(emu, n, nrhs):
+    o = run(emu["mixed"], "refine", n, nrhs, 31)
+    assert o["singular"] == 0 and o["fallback"] == 0 and 1 <= o["corrections"] <= 6
+    assert o["berr"] <= 1e-15, o["berr"]
