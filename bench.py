@@ -478,9 +478,10 @@ def main():
     roofline = dict(bound="mfma", kernel=name, achieved=achieved, peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
                     frac=achieved / FP64_PEAK_TFLOPS, traffic=None,
                     flops_per_launch=float(flops), launch_ms=ms,
-                    limiter=("FP64 VALU: the QR sweeps' dependent rotation chain and its global-memory latency, with "
-                             "HBM traffic from the Householder passes and the sweeps; MFMA only in the eigenvector "
-                             "back-transform" if name == "k_eig1d" else
+                    limiter=("latency: 94% of the issued FP64 flops are VALU (MFMA only in the eigenvector "
+                             "back-transform), the VALU is busy at most 22% of the time and HBM runs at about half its "
+                             "peak; the rest is global-memory latency of the dependent rotation chain and workgroup "
+                             "barriers (profiles/eig_planar_instruction_mix.txt)" if name == "k_eig1d" else
                              "latency of the LU panel and substitution chains"))
     traffic = measured_eig_traffic() if (name == "k_eig1d" and args.points == 4096 and M == 60) else None
     if traffic:
