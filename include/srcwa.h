@@ -69,8 +69,15 @@ int srcwa_context_set_pool(srcwa_context* ctx, srcwa_pool* pool);
  *   on FP32 MFMA, and refined to FP64 accuracy by FP64 residuals (LAPACK zcgesv's per-column stopping
  *   test, at most 6 corrections); a solve whose corrections stall falls back to the FP64 LU.  Eigensolves,
  *   operator assembly and the efficiency sums stay FP64.  No counterpart in the reference (its LU is FP64
- *   only, lu.hpp:267-443); srcwa_plan_refinement reports what each solve needed. */
-enum { SRCWA_PLAN_DIAGNOSTICS = 1, SRCWA_PLAN_MIXED = 2 };
+ *   only, lu.hpp:267-443); srcwa_plan_refinement reports what each solve needed.
+ * SRCWA_PLAN_POINT_OPS: every grating layer of every point computes its own wavelength-invariant
+ *   operators, as the reference's solve does (rcwa1d.hpp:432-468, rcwa1d_conical.hpp:302-336,
+ *   rcwa2d_operators.hpp:349-429).  Without it a plan computes them once per execution for all of its
+ *   points that share them (the same layer record apart from the thickness, period or cell, truncation and,
+ *   planar, polarization): a sweep over wavelength or angle, or over the thickness of a layer, factors
+ *   each layer's operators once.  Results are the same bits either way; diagnostics plans always use the
+ *   per-point form. */
+enum { SRCWA_PLAN_DIAGNOSTICS = 1, SRCWA_PLAN_MIXED = 2, SRCWA_PLAN_POINT_OPS = 4 };
 int srcwa_context_set_plan_flags(srcwa_context* ctx, int flags);
 
 /* ---------------------------------------------------------------------------------------------
@@ -176,6 +183,9 @@ int srcwa_plan_geometry(srcwa_plan* plan, double* out);
 int64_t srcwa_plan_orders(const srcwa_plan* plan);
 int64_t srcwa_plan_device_bytes(const srcwa_plan* plan);
 int srcwa_plan_chunks(const srcwa_plan* plan);
+/* Shared operator sets of the plan (0 with SRCWA_PLAN_POINT_OPS, with diagnostics, or when no two points
+ * share a layer's operators).  No counterpart in the reference. */
+int srcwa_plan_operator_sets(const srcwa_plan* plan);
 /* Per-stage GPU milliseconds of the last execute, from HIP events on the plan's stream:
  * [0] setup + assembly, [1] eigensolves, [2] modes + S-matrix + efficiencies (each summed over
  * chunks), [3] wall time from the first chunk's start to the last chunk's end. */

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libsrcwa_amd.so")
 OK, EINVAL, EDEVICE, ENOMEM, EUNSUPPORTED = 0, -1, -2, -3, -4
 PLAN_DIAGNOSTICS = 1
 PLAN_MIXED = 2          # star-product solves by FP32 LU + FP64 iterative refinement
+PLAN_POINT_OPS = 4      # every point computes its own wavelength-invariant layer operators
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -65,6 +66,7 @@ PROTOTYPES = {
     "srcwa_plan_sync": (ctypes.c_int, [_vp]),
     "srcwa_plan_results": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "srcwa_plan_orders": (ctypes.c_int64, [_vp]),
+    "srcwa_plan_operator_sets": (ctypes.c_int, [_vp]),
     "srcwa_plan_geometry": (ctypes.c_int, [_vp, _vp]),
     "srcwa_plan_device_bytes": (ctypes.c_int64, [_vp]),
     "srcwa_plan_chunks": (ctypes.c_int, [_vp]),

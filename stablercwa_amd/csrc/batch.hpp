@@ -19,6 +19,7 @@ enum : int {
 enum : int {
     PLAN_DIAGNOSTICS = 1,      // eigen-pair residuals and solve backward errors (Options::diagnostics)
     PLAN_MIXED = 2,            // S-matrix dense solves by FP32 LU + FP64 iterative refinement
+    PLAN_POINT_OPS = 4,        // every layer task computes its own wavelength-invariant operators
 };
 
 // Per-layer-task diagnostics, reduced per instance by the S-matrix kernels.
@@ -68,10 +69,43 @@ __host__ __device__ inline int64_t mixed_solve_words(int64_t n, int64_t nrhs) {
     return refine_words(n, nrhs) + 2 * n * nrhs + 64;
 }
 
-// One layer task: a grating layer of an instance that needs an eigensolve.
+// One layer task: a grating layer of an instance that needs an eigensolve.  op: the task's shared
+// operator set (below), or -1 when the task computes its layer's operators itself.
 struct LayerTask {
     int inst, layer;
+    int op = -1;
 };
+
+// A shared operator set: the wavelength-invariant operators of one grating layer (the Toeplitz matrices
+// of eps and 1/eps and their LU factors), computed once per plan execution for every task whose layer
+// record, period, truncation order and polarization are the same.  inst (a plan-wide instance index) and
+// layer name one such task; off is the set's cd offset in the plan's operator buffer.
+struct LayerOps {
+    int inst, layer;
+    int64_t off;
+};
+
+// A plan's shared operator sets as the launchers take them: the descriptors, the operator buffer, each
+// set's failure word (nonzero: singular or invalid), the plan-wide instance array the descriptors index,
+// and whether this launch computes the sets (the first chunk of an execution) before its assembly.
+struct cd;
+struct OpsArgs {
+    const LayerOps* descs = nullptr;
+    cd* ops = nullptr;
+    int* info = nullptr;
+    int n = 0;
+    const void* insts = nullptr;
+    bool compute = false;
+};
+
+// Planar operator set of order n (cd words, each part 16-aligned): TE: [[eps]]; TM: the LU factors of
+// [[eps]], [[1/eps]] and the LU factors of [[1/eps]], then the two pivot vectors.  Both: then the 4M+1
+// Fourier coefficients (2n - 1) and, for an FFT layer, its sample scratch.
+__host__ __device__ inline int64_t ops_align16(int64_t x) { return (x + 15) & ~int64_t(15); }
+__host__ __device__ inline int64_t planar_ops_matrices(int pol) { return pol == 0 ? 1 : 3; }
+__host__ __device__ inline int64_t planar_ops_words(int64_t n, int pol) {
+    return planar_ops_matrices(pol) * ops_align16(n * n) + (pol == 0 ? 0 : 2 * ops_align16(n)) + ops_align16(2 * n);
+}
 
 // Layout table of a planar instance (int64 cd offsets into the workspace):
 //   [0]                      kx (N)

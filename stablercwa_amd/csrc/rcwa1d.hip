@@ -2,6 +2,9 @@
 //
 //   k_setup1d      one workgroup per instance: kx_m and the analytic modes of the incidence medium,
 //                  exit medium and uniform films           (rcwa1d.hpp:410-415, 522-544)
+//   k_ops1d        one workgroup per shared operator set: [[eps]] (TE); the LU factors of [[eps]],
+//                  [[1/eps]] and its LU factors (TM) -- the layer's wavelength-invariant operators, once
+//                  for every task of the plan that has the same layer, period, M and polarization
 //   k_assemble1d   one workgroup per grating layer: analytic Fourier coefficients, Toeplitz matrix,
 //                  layer matrix A (TE: Kx^2 - [[eps]]; TM: [[1/eps]]^-1 (Kx [[eps]]^-1 Kx - I))
 //                                                          (rcwa1d.hpp:366-407, 432-468)
@@ -44,15 +47,70 @@ __global__ __launch_bounds__(NT) void k_setup1d(const Inst1D* __restrict__ insts
 
 // ------------------------------------------------------------------------------------------------
 #if SRCWA_MAIN_TU
+DEV bool shares_ops(const LayerTask& tk, const Inst1D& in) { return tk.op >= 0 && !(in.flags & PLAN_DIAGNOSTICS); }
+
+// A planar operator set (layout: planar_ops_words).  E: [[eps]] (TE) or its LU factors (TM).
+struct PlanarOps {
+    cd *E, *Ti, *LUi, *coef;
+    int *pive, *pivi;
+    cd* F;                                     // FFT layers: sample scratch
+};
+DEV PlanarOps planar_ops_at(cd* base, int N, int pol) {
+    const int64_t m = ops_align16((int64_t)N * N);
+    PlanarOps o{};
+    o.E = base;
+    cd* p = base + m;
+    if (pol != 0) {
+        o.Ti = p;
+        o.LUi = p + m;
+        o.pive = reinterpret_cast<int*>(p + 2 * m);
+        o.pivi = reinterpret_cast<int*>(p + 2 * m + ops_align16(N));
+        p += 2 * m + 2 * ops_align16(N);
+    }
+    o.coef = p;
+    o.F = p + ops_align16(2 * (int64_t)N);
+    return o;
+}
+
+// The wavelength-invariant operators of one grating layer, with the arithmetic of the per-task form in
+// k_assemble1d (rcwa1d.hpp:432-468): the same coefficients, Toeplitz matrices and LU factorizations.
+__global__ __launch_bounds__(NT) void k_ops1d(const LayerOps* __restrict__ descs, const Inst1D* __restrict__ insts,
+                                              const double* __restrict__ stacks, cd* __restrict__ ops,
+                                              int* __restrict__ ops_info) {
+    const LayerOps d = descs[blockIdx.x];
+    const Inst1D in = insts[d.inst];
+    const double* st = stacks + in.stack_off;
+    const int N = in.N, M = in.M;
+    LayerCursor cur(st);
+    LayerView lv = cur.next();
+    for (int j = 0; j < d.layer; ++j) lv = cur.next();
+    const PlanarOps o = planar_ops_at(ops + d.off, N, in.pol);
+    int fail = layer_coefficients(lv, st[0], M, false, o.coef, o.F);
+    toeplitz(o.coef, N, o.E);
+    if (in.pol != 0) {
+        fail |= layer_coefficients(lv, st[0], M, true, o.coef, o.F);
+        toeplitz(o.coef, N, o.Ti);
+        fail |= wg_lu(N, o.E, N, o.pive);
+        wg_copy(N, N, o.Ti, N, o.LUi, N);
+        fail |= wg_lu(N, o.LUi, N, o.pivi);
+    }
+    if (threadIdx.x == 0) ops_info[blockIdx.x] = fail;
+}
+
 __global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                                    const double* __restrict__ stacks, const int64_t* __restrict__ layouts,
                                                    cd* __restrict__ ws, int* __restrict__ task_info,
-                                                   TaskDiag* __restrict__ task_diag) {
+                                                   TaskDiag* __restrict__ task_diag, const LayerOps* __restrict__ descs,
+                                                   cd* __restrict__ ops, const int* __restrict__ ops_info) {
     const LayerTask tk = tasks[blockIdx.x];
     const Inst1D in = insts[tk.inst];
     const double* st = stacks + in.stack_off;
     const int64_t* lay = layouts + in.layout_off;
     const int N = in.N, M = in.M;
+    // a task with a shared operator set reads its layer's operators from it (the host gives none with
+    // diagnostics, whose backward errors need the per-task matrices)
+    const bool shared = shares_ops(tk, in);
+    const PlanarOps o = shared ? planar_ops_at(ops + descs[tk.op].off, N, in.pol) : PlanarOps{};
     const double period = st[0];
     LayerCursor cur(st);
     LayerView lv = cur.next();
@@ -69,28 +127,43 @@ __global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ in
     cd* D1 = D0 + NN;
     cd* F = D0 + (diag ? 2 * NN : 0);          // FFT layers: 2 nfft sample scratch
     double berr = 0.0;
-    int fail = 0;
+    int fail = shared ? ops_info[tk.op] : 0;
     if (in.pol == 0) {
-        fail |= layer_coefficients(lv, period, M, false, coef, F);
-        toeplitz(coef, N, S0);
+        const cd* E = S0;
+        if (shared) {
+            E = o.E;
+        } else {
+            fail |= layer_coefficients(lv, period, M, false, coef, F);
+            toeplitz(coef, N, S0);
+        }
         // A = Kx^2 - [[eps]]
         wg_map((int)NN,
                [&](int e) {
                    const int i = e % N;
-                   cd v = -ldc(S0 + e);
+                   cd v = -ldc(E + e);
                    if (i == e / N) v = v + cd{kx[i].x * kx[i].x, 0.0};
                    return v;
                },
                [&](int e, cd v) { stc(S0 + e, v); });
     } else {
-        // S1 = [[eps]], S3 = [[1/eps]] (kept for V), S2 = diag(kx)
-        fail |= layer_coefficients(lv, period, M, false, coef, F);
-        toeplitz(coef, N, S1);
-        fail |= layer_coefficients(lv, period, M, true, coef, F);
-        toeplitz(coef, N, S3);
+        // S1 = [[eps]], S3 = [[1/eps]] (kept for V), S2 = diag(kx); with a shared set its factors and
+        // [[1/eps]] are read from there
+        const cd *LUe = S1, *LUi = S0;
+        const int *pive = piv, *pivi = piv;
+        if (shared) {
+            LUe = o.E;
+            LUi = o.LUi;
+            pive = o.pive;
+            pivi = o.pivi;
+        } else {
+            fail |= layer_coefficients(lv, period, M, false, coef, F);
+            toeplitz(coef, N, S1);
+            fail |= layer_coefficients(lv, period, M, true, coef, F);
+            toeplitz(coef, N, S3);
+        }
         wg_diag(N, kx, S2, N);
-        fail |= wg_lu(N, S1, N, piv);
-        wg_lu_solve(N, S1, N, piv, S2, N, N);
+        if (!shared) fail |= wg_lu(N, S1, N, piv);
+        wg_lu_solve(N, LUe, N, pive, S2, N, N);
         if (diag) {
             // [[eps]] X = diag(kx): rebuild [[eps]] (S1 holds its factors) and the right-hand side
             fail |= layer_coefficients(lv, period, M, false, coef, F);
@@ -107,10 +180,12 @@ __global__ __launch_bounds__(NT) void k_assemble1d(const Inst1D* __restrict__ in
                    return v;
                },
                [&](int e, cd v) { stc(S2 + e, v); });
-        wg_copy(N, N, S3, N, S0, N);
-        if (diag) wg_copy(N, N, S2, N, D1, N);
-        fail |= wg_lu(N, S0, N, piv);
-        wg_lu_solve(N, S0, N, piv, S2, N, N);
+        if (!shared) {
+            wg_copy(N, N, S3, N, S0, N);
+            if (diag) wg_copy(N, N, S2, N, D1, N);
+            fail |= wg_lu(N, S0, N, piv);
+        }
+        wg_lu_solve(N, LUi, N, pivi, S2, N, N);
         if (diag) berr = max_keep_nan(berr, wg_backward_error(N, S3, S2, D1, N, D0));
         wg_copy(N, N, S2, N, S0, N);
     }
@@ -204,7 +279,8 @@ void SRCWA_QC_FN(planar_qc)(const QcLaunch& a) {
 #if SRCWA_MAIN_TU
 __global__ __launch_bounds__(NT) void k_modes1d(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                                 const int64_t* __restrict__ layouts, cd* __restrict__ ws,
-                                                const int* __restrict__ task_info) {
+                                                const int* __restrict__ task_info, const LayerOps* __restrict__ descs,
+                                                cd* __restrict__ ops) {
     if (task_info[blockIdx.x] != 0) return;
     const LayerTask tk = tasks[blockIdx.x];
     const Inst1D in = insts[tk.inst];
@@ -212,7 +288,9 @@ __global__ __launch_bounds__(NT) void k_modes1d(const Inst1D* __restrict__ insts
     const int N = in.N;
     const int64_t NN = (int64_t)N * N;
     cd* base = ws + lay[layout_layer(in.L, tk.layer)];
-    cd *W = base, *V = base + NN, *S3 = base + 3 * NN;
+    cd *W = base, *V = base + NN;
+    // TM: [[1/eps]], kept by the assembly in S3 or held by the task's shared operator set
+    const cd* S3 = shares_ops(tk, in) && in.pol != 0 ? planar_ops_at(ops + descs[tk.op].off, N, in.pol).Ti : base + 3 * NN;
     cd* lam = base + 4 * NN;
     cd* q = ws + lay[layout_q(tk.layer + 1)];
     cd* basis = base + 4 * NN + N;
@@ -373,14 +451,19 @@ void smatrix1d_launch(int grid, hipStream_t s, const Inst1D* insts, const double
 // host launchers
 void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                   int part, int flags, hipEvent_t* ev) {
+                   int part, int flags, hipEvent_t* ev, const OpsArgs& ops) {
     const bool diag = flags & PLAN_DIAGNOSTICS;
-    // part 1: setup, assembly, eigensolves (events ev[0..2]); part 2: modes, S-matrix, efficiencies
-    // (events ev[3], ev[4]); the host may run the parts of different chunks on two streams
+    // part 1: shared operators (first chunk), setup, assembly, eigensolves (events ev[0..2]); part 2:
+    // modes, S-matrix, efficiencies (events ev[3], ev[4]); the host may run the parts of different chunks
+    // on two streams
     if (part & 1) {
         if (ev) hipEventRecord(ev[0], s);
+        if (ops.compute && ops.n > 0)
+            k_ops1d<<<ops.n, NT, 0, s>>>(ops.descs, static_cast<const Inst1D*>(ops.insts), stacks, ops.ops, ops.info);
         k_setup1d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws);
-        if (n_tasks > 0) k_assemble1d<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info, out.task_diag);
+        if (n_tasks > 0)
+            k_assemble1d<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info, out.task_diag, ops.descs,
+                                                ops.ops, ops.info);
         if (ev) hipEventRecord(ev[1], s);
         // each matrix goes to the wave or the workgroup eigensolver by its own order, so a result does not
         // depend on what else is in the batch
@@ -408,7 +491,7 @@ void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, 
     }
     if (part & 2) {
         if (ev) hipEventRecord(ev[3], s);
-        if (n_tasks > 0) k_modes1d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info);
+        if (n_tasks > 0) k_modes1d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info, ops.descs, ops.ops);
         smatrix1d_launch(n_inst, s, insts, stacks, layouts, ws, task_info, out);
         if (ev) hipEventRecord(ev[4], s);
     }

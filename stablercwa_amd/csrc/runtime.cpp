@@ -24,7 +24,7 @@
 // kernel launchers (rcwa1d.hip, dense.hip)
 void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                   int part, int flags, hipEvent_t* ev);
+                   int part, int flags, hipEvent_t* ev, const OpsArgs& ops);
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
                     int part, int flags, hipEvent_t* ev, const int* eig_order);
@@ -83,6 +83,7 @@ void srcwa_set_last_error(const std::string& msg) { g_error = msg; }
 
 static_assert(int(SRCWA_PLAN_DIAGNOSTICS) == int(PLAN_DIAGNOSTICS), "plan flag of the C-ABI and the kernels");
 static_assert(int(SRCWA_PLAN_MIXED) == int(PLAN_MIXED), "plan flag of the C-ABI and the kernels");
+static_assert(int(SRCWA_PLAN_POINT_OPS) == int(PLAN_POINT_OPS), "plan flag of the C-ABI and the kernels");
 
 // A device arena carved first-fit into 256-byte aligned blocks.  Plans created on a context that has
 // a pool take every device buffer from it, so many plans can come and go (a request server) without
@@ -207,6 +208,8 @@ struct LayerInfo {
     double d;
     int method = 0;        // gratings: 0 analytic, 1 FFT
     int samples = 0;       // gratings: fft_samples (0 automatic)
+    int64_t rec = 0;       // the layer's words after its thickness (kind onward) in the stack record
+    int64_t rec_words = 0;
 };
 
 constexpr int kMaxFFTSamples = 4194304;
@@ -244,6 +247,7 @@ std::vector<LayerInfo> parse_stack1d(const double* p, int64_t avail, int M) {
         need(2);
         const double d = p[k];
         const int kind = static_cast<int>(p[k + 1]);
+        const int64_t start = k + 1;
         k += 2;
         if (kind == 0) {
             need(2);
@@ -276,7 +280,7 @@ std::vector<LayerInfo> parse_stack1d(const double* p, int64_t avail, int M) {
                 same = same && v[2 * i] == v[0] && v[2 * i + 1] == v[1];
             }
             k += 1 + (K + 1) + 2 * K + 2;
-            out.push_back({true, same, d, method, samples});
+            out.push_back({true, same, d, method, samples, start, k - start});
         } else {
             invalid("rcwa: unknown layer kind");
         }
@@ -294,6 +298,61 @@ void check_setup(double period, double wavelength, double nIr, double nIi, doubl
         invalid("rcwa: incidence angle must lie strictly between -90 and 90 degrees");
     check_index(nIr, nIi);
     if (nIi != 0.0 || nIr <= 0.0) invalid("rcwa: the incident medium must have a real positive index");
+}
+
+// Shared operator sets: layer tasks whose layer operators cannot depend on the point (the same layer
+// words after the thickness, the same period or cell and truncation, the same polarization) are grouped
+// by those values, bit for bit, and every group of two or more tasks gets one set, computed once per
+// execution; single tasks keep computing their own.
+struct OpSharing {
+    std::map<std::vector<uint64_t>, int> index;
+    std::vector<int> task_key;         // per task of the plan, in task order
+    std::vector<int> count;
+    std::vector<LayerOps> rep;         // a plan-wide (instance, layer) of each key
+    std::vector<int64_t> words;
+    bool enabled = true;
+
+    static void push(std::vector<uint64_t>& key, double v) {
+        uint64_t u;
+        std::memcpy(&u, &v, sizeof u);
+        key.push_back(u);
+    }
+    // key: the values the operators depend on; words: the set's size in cd words
+    void add(const std::vector<uint64_t>& key, int inst, int layer, int64_t set_words) {
+        auto it = index.find(key);
+        if (it == index.end()) {
+            it = index.emplace(key, static_cast<int>(count.size())).first;
+            count.push_back(0);
+            rep.push_back(LayerOps{inst, layer, 0});
+            words.push_back(set_words);
+        }
+        ++count[it->second];
+        task_key.push_back(it->second);
+    }
+    // Gives each shared group its set (tasks in the order add() saw them); returns the operator buffer's
+    // size in cd words.
+    int64_t assign(std::vector<LayerTask>& tasks, std::vector<LayerOps>& descs) const {
+        descs.clear();
+        if (!enabled) return 0;
+        std::vector<int> op(count.size(), -1);
+        int64_t off = 0;
+        for (size_t k = 0; k < count.size(); ++k)
+            if (count[k] >= 2) {
+                op[k] = static_cast<int>(descs.size());
+                descs.push_back(LayerOps{rep[k].inst, rep[k].layer, off});
+                off += (words[k] + 15) & ~int64_t(15);
+            }
+        for (size_t t = 0; t < tasks.size(); ++t) tasks[t].op = op[task_key[t]];
+        return off;
+    }
+};
+
+// device bytes of a plan's shared operator sets (as the pool rounds them)
+int64_t ops_bytes(const std::vector<LayerOps>& descs, int64_t words) {
+    if (descs.empty()) return 0;
+    return static_cast<int64_t>(srcwa_pool::round(descs.size() * sizeof(LayerOps)) +
+                                srcwa_pool::round(static_cast<size_t>(words) * sizeof(cd)) +
+                                srcwa_pool::round(descs.size() * sizeof(int)));
 }
 
 struct ParsedStacks {
@@ -432,6 +491,10 @@ struct srcwa_plan {
     DevBuf<cd> ws;
     DevBuf<int> task_info;
     DevBuf<int> eig_order;          // conical / crossed: each chunk's tasks, largest eigensolve first
+    DevBuf<LayerOps> op_descs;      // shared operator sets (LayerOps), their buffer and failure words
+    DevBuf<cd> ops;
+    DevBuf<int> ops_info;
+    int n_ops = 0;
     DevBuf<double> R, T, DEr, DEt;
     DevBuf<double> geometry;      // conical / crossed: GEO_WORDS per order
     DevBuf<int> status, eig_info;
@@ -473,6 +536,9 @@ struct srcwa_plan {
         ws.pool = p;
         task_info.pool = p;
         eig_order.pool = p;
+        op_descs.pool = p;
+        ops.pool = p;
+        ops_info.pool = p;
         R.pool = p;
         T.pool = p;
         DEr.pool = p;
@@ -504,7 +570,16 @@ struct srcwa_plan {
     int64_t bytes() const {
         return insts.bytes() + insts2d.bytes() + tasks.bytes() + stacks.bytes() + layouts.bytes() + ws.bytes() + task_info.bytes() +
                R.bytes() + T.bytes() + DEr.bytes() + DEt.bytes() + status.bytes() + eig_info.bytes() + geometry.bytes() +
-               diag.bytes() + task_diag.bytes() + item_res.bytes();
+               diag.bytes() + task_diag.bytes() + item_res.bytes() + eig_order.bytes() + op_descs.bytes() + ops.bytes() +
+               ops_info.bytes();
+    }
+    // uploads the shared operator sets of a plan (none: nothing allocated)
+    void upload_ops(const std::vector<LayerOps>& descs, int64_t words, hipStream_t st) {
+        n_ops = static_cast<int>(descs.size());
+        if (n_ops == 0) return;
+        op_descs.upload(descs, st);
+        ops.alloc(static_cast<size_t>(words));
+        ops_info.alloc(static_cast<size_t>(n_ops));
     }
 };
 
@@ -546,9 +621,19 @@ void enqueue(srcwa_plan* plan, bool timing) {
         cd* ws = plan->ws.p + (nc > 1 ? static_cast<int64_t>(c % 2) * plan->region_words : 0);
         hipEvent_t* ev = timing ? plan->ev.data() + 5 * c : nullptr;
         const int ni = ch.i1 - ch.i0, nt = ch.t1 - ch.t0;
+        // the shared operator sets are computed on the front stream ahead of the first chunk; every later
+        // stage that reads them is ordered after it (same stream, or the back stream after front_done)
+        OpsArgs ops;
+        ops.descs = plan->op_descs.p;
+        ops.ops = plan->ops.p;
+        ops.info = plan->ops_info.p;
+        ops.n = plan->n_ops;
+        ops.insts = plan->kind == 3 ? static_cast<const void*>(plan->insts2d.p) : static_cast<const void*>(plan->insts.p);
+        ops.compute = c == 0;
         if (plan->kind == 1)
             launch_planar(st, ni, nt, plan->insts.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p, plan->layouts.p,
-                          ws, plan->task_info.p + ch.t0, out, plan->max_iter, plan->min_n, plan->max_n, part, plan->flags, ev);
+                          ws, plan->task_info.p + ch.t0, out, plan->max_iter, plan->min_n, plan->max_n, part, plan->flags, ev,
+                          ops);
         else if (plan->kind == 2)
             launch_conical(st, ni, nt, plan->insts.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p, plan->layouts.p,
                            ws, plan->task_info.p + 4 * ch.t0, out, plan->max_iter, plan->min_n, plan->max_n, part,
@@ -642,7 +727,8 @@ int srcwa_pool_destroy(srcwa_pool* pool) {
 int srcwa_context_set_plan_flags(srcwa_context* ctx, int flags) {
     return guarded([&] {
         if (!ctx) invalid("srcwa_context_set_plan_flags: null context");
-        if (flags & ~int(SRCWA_PLAN_DIAGNOSTICS | SRCWA_PLAN_MIXED)) invalid("srcwa_context_set_plan_flags: unknown flag");
+        if (flags & ~int(SRCWA_PLAN_DIAGNOSTICS | SRCWA_PLAN_MIXED | SRCWA_PLAN_POINT_OPS))
+            invalid("srcwa_context_set_plan_flags: unknown flag");
         if ((flags & SRCWA_PLAN_MIXED) && !SRCWA_MIXED_PRECISION) throw Failure{SRCWA_EUNSUPPORTED, kMixedOff};
         ctx->plan_flags = flags;
     });
@@ -690,6 +776,8 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         std::vector<int64_t> h_layouts;
         int64_t orders = 0, chunk_ws = 0, max_ws = 0;
         PlanChunk cur{0, 0, 0, 0};
+        OpSharing sharing;
+        sharing.enabled = !(plan->flags & (PLAN_DIAGNOSTICS | PLAN_POINT_OPS));
         for (int i = 0; i < n_problems; ++i) {
             const int s = stack_index[i];
             if (s < 0 || s >= n_stacks) invalid("rcwa: stack index out of range");
@@ -737,7 +825,14 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
             }
             const int first_task = static_cast<int>(h_tasks.size());
             for (int j = 0; j < L; ++j)
-                if (li[j].grating) h_tasks.push_back(LayerTask{i - cur.i0, j});
+                if (li[j].grating) {
+                    h_tasks.push_back(LayerTask{i - cur.i0, j});
+                    // TE: [[eps]]; TM: also [[1/eps]] and both LU factorizations (rcwa1d.hpp:432-468)
+                    std::vector<uint64_t> key;
+                    for (double v : {st[0], double(M[i]), double(pol[i])}) OpSharing::push(key, v);
+                    for (int64_t k = 0; k < li[j].rec_words; ++k) OpSharing::push(key, st[li[j].rec + k]);
+                    sharing.add(key, i, j, planar_ops_words(N, pol[i]) + fft_scratch_words(li[j], M[i]));
+                }
             const int64_t layout_off = static_cast<int64_t>(h_layouts.size());
             for (int64_t v : lay) h_layouts.push_back(v >= 0 ? v + chunk_ws : -1);
             Inst1D& in = h_insts[i];
@@ -767,6 +862,8 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         max_ws = std::max(max_ws, chunk_ws);
         plan->n_tasks = static_cast<int>(h_tasks.size());
         plan->total_orders = orders;
+        std::vector<LayerOps> h_ops;
+        const int64_t op_words = sharing.assign(h_tasks, h_ops);
 
         if (need) {
             const auto b = [](size_t x) { return static_cast<int64_t>(srcwa_pool::round(x)); };
@@ -774,12 +871,13 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
                     b(h_stacks.size() * sizeof(double)) + b(h_layouts.size() * sizeof(int64_t)) +
                     b((plan->chunks.size() > 1 ? 2 : 1) * max_ws * sizeof(cd)) + b(std::max(1, plan->n_tasks) * sizeof(int)) +
                     2 * b(2 * orders * sizeof(double)) + 2 * b(orders * sizeof(double)) + 2 * b(n_problems * sizeof(int)) +
-                    diag_bytes(n_problems, plan->n_tasks);
+                    diag_bytes(n_problems, plan->n_tasks) + ops_bytes(h_ops, op_words);
             return;
         }
         hipStream_t st = ctx->stream;
         plan->insts.upload(h_insts, st);
         plan->tasks.upload(h_tasks, st);
+        plan->upload_ops(h_ops, op_words, st);
         plan->stacks.upload(h_stacks, st);
         plan->layouts.upload(h_layouts, st);
         plan->region_words = max_ws;
@@ -1133,6 +1231,7 @@ int srcwa_plan_geometry(srcwa_plan* plan, double* out) {
     });
 }
 int64_t srcwa_plan_device_bytes(const srcwa_plan* plan) { return plan ? plan->bytes() : -1; }
+int srcwa_plan_operator_sets(const srcwa_plan* plan) { return plan ? plan->n_ops : -1; }
 int srcwa_plan_chunks(const srcwa_plan* plan) { return plan ? static_cast<int>(plan->chunks.size()) : -1; }
 
 int srcwa_plan_stage_ms(const srcwa_plan* plan, double* ms4) {
@@ -1413,6 +1512,7 @@ std::vector<Layer2DInfo> parse_stack2d(const double* p, int64_t avail, int64_t& 
         need(2);
         const double d = p[k];
         const int kind = static_cast<int>(p[k + 1]);
+        const int64_t start = k + 1;
         k += 2;
         if (!std::isfinite(d) || d < 0.0) invalid("rcwa2d: layer thickness must be finite and >= 0");
         Layer2DInfo info;

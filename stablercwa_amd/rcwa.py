@@ -127,6 +127,11 @@ class PlanarBatch:
     def device_bytes(self) -> int:
         return int(self.lib.srcwa_plan_device_bytes(self.handle))
 
+    def operator_sets(self) -> int:
+        """Shared operator sets of the plan: layers whose wavelength-invariant operators several points share,
+        computed once per execute (0 with PLAN_POINT_OPS or diagnostics)."""
+        return int(self.lib.srcwa_plan_operator_sets(self.handle))
+
     def chunks(self) -> int:
         return int(self.lib.srcwa_plan_chunks(self.handle))
 
