@@ -107,6 +107,18 @@ __host__ __device__ inline int64_t planar_ops_words(int64_t n, int pol) {
     return planar_ops_matrices(pol) * ops_align16(n * n) + (pol == 0 ? 0 : 2 * ops_align16(n)) + ops_align16(2 * n);
 }
 
+// Conical operator set of order n: [[eps]], [[eps]]^-1, [[1/eps]]^-1, an LU scratch matrix, the pivots,
+// the 4M+1 Fourier coefficients and, for an FFT layer, its sample scratch.
+__host__ __device__ inline int64_t conical_ops_words(int64_t n) {
+    return 4 * ops_align16(n * n) + ops_align16(n) + ops_align16(2 * n);
+}
+
+// Crossed-grating operator set of N harmonics: Eps, Cx, Cy, the LU factors of Eps, Eps^-1, the pivots and
+// the operator scratch (scratch words, as the per-layer slot's).
+__host__ __device__ inline int64_t crossed_ops_words(int64_t N, int64_t scratch) {
+    return 5 * ops_align16(N * N) + ops_align16(N) + ops_align16(scratch);
+}
+
 // Layout table of a planar instance (int64 cd offsets into the workspace):
 //   [0]                      kx (N)
 //   [1 + 2r], [2 + 2r]       q, p of region r = 0 .. L+1 (N each)

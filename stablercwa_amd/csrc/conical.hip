@@ -102,6 +102,36 @@ DEV LayerView layer_at(const double* st, int index) {
     return lv;
 }
 
+// A conical operator set (layout: conical_ops_words): [[eps]], [[eps]]^-1, C = [[1/eps]]^-1.
+struct ConicalOps {
+    cd *E, *IE, *C, *Z, *coef, *F;
+    int* piv;
+};
+DEV ConicalOps conical_ops_at(cd* base, int N) {
+    const int64_t m = ops_align16((int64_t)N * N);
+    ConicalOps o;
+    o.E = base;
+    o.IE = base + m;
+    o.C = base + 2 * m;
+    o.Z = base + 3 * m;
+    o.piv = reinterpret_cast<int*>(base + 4 * m);
+    o.coef = base + 4 * m + ops_align16(N);
+    o.F = o.coef + ops_align16(2 * (int64_t)N);
+    return o;
+}
+DEV bool shares_ops(const LayerTask& tk, const Inst1D& in) { return tk.op >= 0 && !(in.flags & PLAN_DIAGNOSTICS); }
+// a task's slots, with [[eps]]^-1 and C read from its shared operator set when it has one
+DEV ConicalSlots task_slots(const LayerTask& tk, const Inst1D& in, cd* ws, const int64_t* lay,
+                            const LayerOps* descs, cd* ops) {
+    ConicalSlots s = slots_of(ws + lay[conical_slot(in.L, tk.layer)], in.N);
+    if (shares_ops(tk, in)) {
+        const ConicalOps o = conical_ops_at(ops + descs[tk.op].off, in.N);
+        s.IE = o.IE;
+        s.C = o.C;
+    }
+    return s;
+}
+
 // ------------------------------------------------------------------------------------------------
 #if SRCWA_MAIN_TU
 __global__ __launch_bounds__(NT) void k_setup_conical(const Inst1D* __restrict__ insts, const double* __restrict__ stacks,
@@ -167,11 +197,38 @@ __global__ __launch_bounds__(NT) void k_setup_conical(const Inst1D* __restrict__
 
 // ------------------------------------------------------------------------------------------------
 #if SRCWA_MAIN_TU
+// The wavelength-invariant operators of one grating layer, with the arithmetic of the per-task form in
+// k_assemble_conical (rcwa1d_conical.hpp:302-336): [[eps]], its inverse and the inverse of [[1/eps]].
+__global__ __launch_bounds__(NT) void k_ops_conical(const LayerOps* __restrict__ descs, const Inst1D* __restrict__ insts,
+                                                    const double* __restrict__ stacks, cd* __restrict__ ops,
+                                                    int* __restrict__ ops_info) {
+    const LayerOps d = descs[blockIdx.x];
+    const Inst1D in = insts[d.inst];
+    const double* st = stacks + in.stack_off;
+    const int N = in.N, M = in.M;
+    const LayerView lv = layer_at(st, d.layer);
+    const ConicalOps o = conical_ops_at(ops + d.off, N);
+    int fail = layer_coefficients(lv, st[0], M, false, o.coef, o.F);
+    toeplitz(o.coef, N, o.E);                  // [[eps]]
+    wg_copy(N, N, o.E, N, o.Z, N);
+    fail |= wg_lu(N, o.Z, N, o.piv);
+    wg_identity(N, o.IE, N);
+    if (!fail) wg_lu_solve(N, o.Z, N, o.piv, o.IE, N, N);            // [[eps]]^-1
+    fail |= layer_coefficients(lv, st[0], M, true, o.coef, o.F);
+    toeplitz(o.coef, N, o.Z);                  // [[1/eps]]
+    fail |= wg_lu(N, o.Z, N, o.piv);
+    wg_identity(N, o.C, N);
+    if (!fail) wg_lu_solve(N, o.Z, N, o.piv, o.C, N, N);             // C = [[1/eps]]^-1
+    if (threadIdx.x == 0) ops_info[blockIdx.x] = fail;
+}
+
 __global__ __launch_bounds__(NT) void k_assemble_conical(const Inst1D* __restrict__ insts,
                                                          const LayerTask* __restrict__ tasks,
                                                          const double* __restrict__ stacks,
                                                          const int64_t* __restrict__ layouts, cd* __restrict__ ws,
-                                                         int* __restrict__ task_info, TaskDiag* __restrict__ task_diag) {
+                                                         int* __restrict__ task_info, TaskDiag* __restrict__ task_diag,
+                                                         const LayerOps* __restrict__ descs, cd* __restrict__ ops,
+                                                         const int* __restrict__ ops_info) {
     const LayerTask tk = tasks[blockIdx.x];
     const Inst1D in = insts[tk.inst];
     const double* st = stacks + in.stack_off;
@@ -180,7 +237,10 @@ __global__ __launch_bounds__(NT) void k_assemble_conical(const Inst1D* __restric
     const int64_t NN = (int64_t)N * N;
     const LayerView lv = layer_at(st, (int)lay[conical_orig(L, tk.layer)]);
     const ConicalGeo g = geo_of(ws, lay, N);
-    const ConicalSlots s = slots_of(ws + lay[conical_slot(L, tk.layer)], N);
+    // with a shared operator set, [[eps]]^-1 and C are read from it and [[eps]] comes from its copy
+    const bool shared = shares_ops(tk, in);
+    const ConicalSlots s = task_slots(tk, in, ws, lay, descs, ops);
+    const cd* E = shared ? conical_ops_at(ops + descs[tk.op].off, N).E : s.A;
     cd* coef = s.lam1;                         // lambda1 .. tmp2 hold the 4M+1 coefficients
     // with diagnostics: D0, D1 keep A and B C for the eigen residuals, D2, D3 are solve scratch
     const bool diag = in.flags & PLAN_DIAGNOSTICS;
@@ -188,24 +248,26 @@ __global__ __launch_bounds__(NT) void k_assemble_conical(const Inst1D* __restric
     cd *D1 = D0 + NN, *D2 = D0 + 2 * NN, *D3 = D0 + 3 * NN;
     cd* F = D0 + (diag ? 4 * NN : 0);          // FFT layers: 2 nfft sample scratch
     double berr = 0.0;
-    int fail = 0;
-    fail |= layer_coefficients(lv, st[0], M, false, coef, F);
-    toeplitz(coef, N, s.A);                    // [[eps]]
-    wg_copy(N, N, s.A, N, s.Z1, N);
-    fail |= wg_lu(N, s.Z1, N, s.piv);
-    wg_identity(N, s.IE, N);
-    if (!fail) wg_lu_solve(N, s.Z1, N, s.piv, s.IE, N, N);          // [[eps]]^-1
-    if (diag && !fail) {
-        wg_identity(N, D2, N);
-        berr = max_keep_nan(berr, wg_backward_error(N, s.A, s.IE, D2, N, D3));
+    int fail = shared ? ops_info[tk.op] : 0;
+    if (!shared) {
+        fail |= layer_coefficients(lv, st[0], M, false, coef, F);
+        toeplitz(coef, N, s.A);                // [[eps]]
+        wg_copy(N, N, s.A, N, s.Z1, N);
+        fail |= wg_lu(N, s.Z1, N, s.piv);
+        wg_identity(N, s.IE, N);
+        if (!fail) wg_lu_solve(N, s.Z1, N, s.piv, s.IE, N, N);      // [[eps]]^-1
+        if (diag && !fail) {
+            wg_identity(N, D2, N);
+            berr = max_keep_nan(berr, wg_backward_error(N, s.A, s.IE, D2, N, D3));
+        }
+        fail |= layer_coefficients(lv, st[0], M, true, coef, F);
+        toeplitz(coef, N, s.Z2);               // [[1/eps]]
+        if (diag) wg_copy(N, N, s.Z2, N, D0, N);
+        fail |= wg_lu(N, s.Z2, N, s.piv);
+        wg_identity(N, s.C, N);
+        if (!fail) wg_lu_solve(N, s.Z2, N, s.piv, s.C, N, N);       // C = [[1/eps]]^-1
+        if (diag && !fail) berr = max_keep_nan(berr, wg_backward_error(N, D0, s.C, D2, N, D3));
     }
-    fail |= layer_coefficients(lv, st[0], M, true, coef, F);
-    toeplitz(coef, N, s.Z2);                   // [[1/eps]]
-    if (diag) wg_copy(N, N, s.Z2, N, D0, N);
-    fail |= wg_lu(N, s.Z2, N, s.piv);
-    wg_identity(N, s.C, N);
-    if (!fail) wg_lu_solve(N, s.Z2, N, s.piv, s.C, N, N);           // C = [[1/eps]]^-1
-    if (diag && !fail) berr = max_keep_nan(berr, wg_backward_error(N, D0, s.C, D2, N, D3));
     // B = Kx [[eps]]^-1 Kx - I, then B C
     for (int64_t e = threadIdx.x; e < NN; e += NT) {
         const int i = (int)(e % N), j = (int)(e / N);
@@ -218,7 +280,7 @@ __global__ __launch_bounds__(NT) void k_assemble_conical(const Inst1D* __restric
     // A = Kx^2 - [[eps]]
     for (int64_t e = threadIdx.x; e < NN; e += NT) {
         const int i = (int)(e % N), j = (int)(e / N);
-        cd v = -s.A[e];
+        cd v = -E[e];
         if (i == j) v = v + cd{g.kx[i].x * g.kx[i].x, 0.0};
         s.A[e] = v;
     }
@@ -527,16 +589,17 @@ template <int QC>
 __global__ __launch_bounds__(NT) void k_modes_conical(const Inst1D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                                       const double* __restrict__ stacks,
                                                       const int64_t* __restrict__ layouts, cd* __restrict__ ws,
-                                                      int* __restrict__ task_info) {
+                                                      int* __restrict__ task_info, const LayerOps* __restrict__ descs,
+                                                      cd* __restrict__ ops) {
     const int t = blockIdx.x;
     if (task_info[4 * t] != 0 || task_info[4 * t + 1] != 0 || task_info[4 * t + 2] != 0) return;
     const LayerTask tk = tasks[t];
     const Inst1D in = insts[tk.inst];
     const int64_t* lay = layouts + in.layout_off;
-    const int N = in.N, L = in.L, n = 2 * N;
+    const int N = in.N, n = 2 * N;
     const int64_t NN = (int64_t)N * N;
     const ConicalGeo g = geo_of(ws, lay, N);
-    const ConicalSlots s = slots_of(ws + lay[conical_slot(L, tk.layer)], N);
+    const ConicalSlots s = task_slots(tk, in, ws, lay, descs, ops);
     cd* q = ws + lay[conical_q(tk.layer + 1)];
     // ---- degeneracy test (rcwa1d_conical.hpp:826-848)
     double lm = INFINITY;
@@ -627,7 +690,8 @@ void SRCWA_QC_FN(conical_qc)(const QcLaunch& a) {
             k_eig_conical_wave<QC><<<a.grid, NT, 0, a.stream>>>(a.n_items, insts, a.tasks, a.layouts, a.ws,
                                                                 a.task_info, a.max_iter);
     } else {
-        k_modes_conical<QC><<<a.grid, NT, 0, a.stream>>>(insts, a.tasks, a.stacks, a.layouts, a.ws, a.task_info);
+        k_modes_conical<QC><<<a.grid, NT, 0, a.stream>>>(insts, a.tasks, a.stacks, a.layouts, a.ws, a.task_info,
+                                                         a.op_descs, a.ops);
     }
 }
 #endif
@@ -783,10 +847,11 @@ void smatrix_conical_launch(int grid, hipStream_t s, const Inst1D* insts, const 
 #if SRCWA_MAIN_TU
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    int part, int flags, hipEvent_t* ev, const int* eig_order) {
+                    int part, int flags, hipEvent_t* ev, const int* eig_order, const OpsArgs& ops) {
     const bool diag = flags & PLAN_DIAGNOSTICS;
-    // part 1: setup, assembly, eigensolves (events ev[0..2]); part 2: modes, S-matrix, efficiencies
-    // (events ev[3], ev[4]); the host may run the parts of different chunks on two streams
+    // part 1: shared operators (first chunk), setup, assembly, eigensolves (events ev[0..2]); part 2:
+    // modes, S-matrix, efficiencies (events ev[3], ev[4]); the host may run the parts of different chunks
+    // on two streams
     QcLaunch a{};
     a.stream = s;
     a.insts = insts;
@@ -797,11 +862,17 @@ void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts,
     a.task_info = task_info;
     a.max_iter = max_iter;
     a.order = eig_order;
+    a.op_descs = ops.descs;
+    a.ops = ops.ops;
     if (part & 1) {
         if (ev) hipEventRecord(ev[0], s);
+        if (ops.compute && ops.n > 0)
+            k_ops_conical<<<ops.n, NT, 0, s>>>(ops.descs, static_cast<const Inst1D*>(ops.insts), stacks, ops.ops,
+                                               ops.info);
         k_setup_conical<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);
         if (n_tasks > 0)
-            k_assemble_conical<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info, out.task_diag);
+            k_assemble_conical<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info, out.task_diag,
+                                                      ops.descs, ops.ops, ops.info);
         if (ev) hipEventRecord(ev[1], s);
         // each matrix goes to the wave or the workgroup eigensolver by its own order
         if (n_tasks > 0 && min_n <= WAVE_EIG_MAX_N) {

@@ -97,6 +97,8 @@ struct QcLaunch {
     cd* wbase;                // dense eigensolves: base of the eigenvalue offsets (DenseItem::w)
     cd* sbase;                // dense eigensolves: base of the scratch offsets (DenseItem::scratch)
     const int* order;         // layer eigensolves: the launch's tasks, longest first (block b solves task order[b])
+    const struct LayerOps* op_descs;   // conical modes: the plan's shared operator sets and their buffer
+    cd* ops;
 };
 
 struct EigScratch {

@@ -284,30 +284,17 @@ struct Ops2DScratch {
 };
 
 #if SRCWA_MAIN_TU
-__global__ __launch_bounds__(NT) void k_ops2d(const Inst2D* __restrict__ insts, const LayerTask* __restrict__ tasks,
-                                              const double* __restrict__ stacks, const int64_t* __restrict__ layouts,
-                                              cd* __restrict__ ws, int* __restrict__ task_info) {
-    const LayerTask tk = tasks[blockIdx.x];
-    const Inst2D in = insts[tk.inst];
-    const double* st = stacks + in.stack_off;
-    const int64_t* lay = layouts + in.layout_off;
-    const int M1 = in.M1, M2 = in.M2, N = in.N, L = in.L;
+// Eps, Cx, Cy of one patterned layer (Li factorization or the Laurent rule) into EPS, CX, CY, with the
+// scratch region sc (ops2d_scratch_words); returns 1 if a per-segment inverse failed or a value is not finite.
+DEV int build_ops2d(const Inst2D& in, const double* st, const Layer2DView& lv, const double* cuts_rec, cd* EPS,
+                    cd* CX, cd* CY, cd* sc) {
+    const int M1 = in.M1, M2 = in.M2, N = in.N;
     const int n1 = 2 * M1 + 1, n2 = 2 * M2 + 1;
     const int64_t NN = (int64_t)N * N;
     const double Lx = st[0], Ly = st[1];
-    if (threadIdx.x == 0) task_info[4 * blockIdx.x + 1] = task_info[4 * blockIdx.x + 2] = task_info[4 * blockIdx.x + 3] = 0;
-    if (ws[lay[0] + 4 * N].x != 0.0) {           // Rayleigh instance: nothing to assemble
-        if (threadIdx.x == 0) task_info[4 * blockIdx.x] = 0;
-        return;
-    }
-    const Layer2DView lv = layer2d_at(st, tk.layer);
     const int K = lv.K;
-    const double* cuts_rec = stacks + lay[x2_cuts(L, tk.layer)];
     const int Kc = (int)cuts_rec[0];
     const double* xcuts = cuts_rec + 1;
-    cd* base = ws + lay[x2_slot(L, tk.layer)];
-    cd *EPS = base, *CX = base + NN, *CY = base + 2 * NN;
-    cd* sc = base + 3 * NN;                       // BIG region
     const int64_t n1s = (int64_t)n1 * n1, n2s = (int64_t)n2 * n2;
     cd* wseg = sc;
     cd* lau = wseg + (int64_t)K * (4 * M2 + 1);
@@ -419,7 +406,72 @@ __global__ __launch_bounds__(NT) void k_ops2d(const Inst2D* __restrict__ insts, 
     }
     int bad = 0;
     for (int64_t e = threadIdx.x; e < NN; e += NT) bad |= !finite(EPS[e]) || !finite(CX[e]) || !finite(CY[e]);
-    bad = block_max_int(bad) | block_max_int(fail);
+    return block_max_int(bad) | block_max_int(fail);
+}
+
+// A crossed-grating operator set (layout: crossed_ops_words): Eps, Cx, Cy, the LU factors of Eps, Ei = Eps^-1,
+// the pivots, then the operator scratch.  Its failure word: bit 0 from build_ops2d, bit 1 a singular Eps.
+struct CrossedOps {
+    cd *EPS, *CX, *CY, *LU, *Ei, *sc;
+    int* piv;
+};
+DEV CrossedOps crossed_ops_at(cd* base, int N) {
+    const int64_t m = ops_align16((int64_t)N * N);
+    CrossedOps o;
+    o.EPS = base;
+    o.CX = base + m;
+    o.CY = base + 2 * m;
+    o.LU = base + 3 * m;
+    o.Ei = base + 4 * m;
+    o.piv = reinterpret_cast<int*>(base + 5 * m);
+    o.sc = base + 5 * m + ops_align16(N);
+    return o;
+}
+DEV bool shares_ops2d(const LayerTask& tk, const Inst2D& in) { return tk.op >= 0 && !(in.flags & PLAN_DIAGNOSTICS); }
+
+// The wavelength-invariant operators of one patterned layer, with the arithmetic of the per-task form
+// (k_ops2d, and Ei as in k_pq2d; rcwa2d_operators.hpp:349-429, rcwa2d.hpp:412-420).
+__global__ __launch_bounds__(NT) void k_ops2d_shared(const LayerOps* __restrict__ descs, const Inst2D* __restrict__ insts,
+                                                     const double* __restrict__ stacks,
+                                                     const int64_t* __restrict__ layouts, cd* __restrict__ ops,
+                                                     int* __restrict__ ops_info) {
+    const LayerOps d = descs[blockIdx.x];
+    const Inst2D in = insts[d.inst];
+    const double* st = stacks + in.stack_off;
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N;
+    const CrossedOps o = crossed_ops_at(ops + d.off, N);
+    const int bad = build_ops2d(in, st, layer2d_at(st, d.layer), stacks + lay[x2_cuts(in.L, d.layer)], o.EPS, o.CX,
+                                o.CY, o.sc);
+    wg_copy(N, N, o.EPS, N, o.LU, N);
+    const int fail = wg_lu(N, o.LU, N, o.piv);
+    wg_identity(N, o.Ei, N);
+    if (!fail) wg_lu_solve(N, o.LU, N, o.piv, o.Ei, N, N);
+    if (threadIdx.x == 0) ops_info[blockIdx.x] = (bad ? 1 : 0) | (fail ? 2 : 0);
+}
+
+__global__ __launch_bounds__(NT) void k_ops2d(const Inst2D* __restrict__ insts, const LayerTask* __restrict__ tasks,
+                                              const double* __restrict__ stacks, const int64_t* __restrict__ layouts,
+                                              cd* __restrict__ ws, int* __restrict__ task_info,
+                                              const int* __restrict__ ops_info) {
+    const LayerTask tk = tasks[blockIdx.x];
+    const Inst2D in = insts[tk.inst];
+    const double* st = stacks + in.stack_off;
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N, L = in.L;
+    const int64_t NN = (int64_t)N * N;
+    if (threadIdx.x == 0) task_info[4 * blockIdx.x + 1] = task_info[4 * blockIdx.x + 2] = task_info[4 * blockIdx.x + 3] = 0;
+    if (ws[lay[0] + 4 * N].x != 0.0) {           // Rayleigh instance: nothing to assemble
+        if (threadIdx.x == 0) task_info[4 * blockIdx.x] = 0;
+        return;
+    }
+    if (shares_ops2d(tk, in)) {                  // the layer's operators are in its shared set
+        if (threadIdx.x == 0) task_info[4 * blockIdx.x] = (ops_info[tk.op] & 1) ? -1 : 0;
+        return;
+    }
+    cd* base = ws + lay[x2_slot(L, tk.layer)];
+    const int bad = build_ops2d(in, st, layer2d_at(st, tk.layer), stacks + lay[x2_cuts(L, tk.layer)], base,
+                                base + NN, base + 2 * NN, base + 3 * NN);   // scratch: the BIG region
     if (threadIdx.x == 0) task_info[4 * blockIdx.x] = bad ? -1 : 0;
 }
 #endif
@@ -453,7 +505,9 @@ DEV Slots2D slots2d(cd* ws, const int64_t* lay, int L, int j, int N) {
 // Ei = Eps^-1, P = [pc1 | pc2], Q12 = Cy - Kx^2, Q21 = Ky^2 - Cx, H = P Q (rcwa2d.hpp:412-474)
 __global__ __launch_bounds__(NT) void k_pq2d(const Inst2D* __restrict__ insts, const LayerTask* __restrict__ tasks,
                                              const int64_t* __restrict__ layouts, cd* __restrict__ ws,
-                                             int* __restrict__ task_info, TaskDiag* __restrict__ task_diag) {
+                                             int* __restrict__ task_info, TaskDiag* __restrict__ task_diag,
+                                             const LayerOps* __restrict__ descs, cd* __restrict__ ops,
+                                             const int* __restrict__ ops_info) {
     // one eigensolve per patterned layer, one dense solve for Ei = Eps^-1 (rcwa2d.hpp:412-474)
     if (threadIdx.x == 0) task_diag[blockIdx.x] = TaskDiag{0.0, 0.0, 1, 1};
     if (task_info[4 * blockIdx.x] != 0) return;
@@ -465,13 +519,24 @@ __global__ __launch_bounds__(NT) void k_pq2d(const Inst2D* __restrict__ insts, c
     const cd* geo = ws + lay[0];
     if (geo[4 * N].x != 0.0) return;
     const Slots2D s = slots2d(ws, lay, in.L, tk.layer, N);
-    // Ei into Z[NN..2NN) (Z[0..NN) holds the LU of Eps)
-    cd* lu = s.Z;
-    cd* Ei = s.Z + NN;
-    wg_copy(N, N, s.EPS, N, lu, N);
-    int fail = wg_lu(N, lu, N, s.piv);
-    wg_identity(N, Ei, N);
-    if (!fail) wg_lu_solve(N, lu, N, s.piv, Ei, N, N);
+    // Ei into Z[NN..2NN) (Z[0..NN) holds the LU of Eps), or Ei, Cx and Cy from the layer's shared set
+    const cd *Ei, *CXin = s.CX, *CYin = s.CY;
+    int fail;
+    if (shares_ops2d(tk, in)) {
+        const CrossedOps o = crossed_ops_at(ops + descs[tk.op].off, N);
+        Ei = o.Ei;
+        CXin = o.CX;
+        CYin = o.CY;
+        fail = (ops_info[tk.op] >> 1) & 1;
+    } else {
+        cd* lu = s.Z;
+        cd* ei = s.Z + NN;
+        wg_copy(N, N, s.EPS, N, lu, N);
+        fail = wg_lu(N, lu, N, s.piv);
+        wg_identity(N, ei, N);
+        if (!fail) wg_lu_solve(N, lu, N, s.piv, ei, N, N);
+        Ei = ei;
+    }
     // pc1 = [Kx Ei Ky ; Ky Ei Ky - I], pc2 = [I - Kx Ei Kx ; -Ky Ei Kx], 2N x N each (ld 2N) in X
     cd* pc1 = s.X;
     cd* pc2 = s.X + 2 * NN;
@@ -489,7 +554,7 @@ __global__ __launch_bounds__(NT) void k_pq2d(const Inst2D* __restrict__ insts, c
     // Q12 = Cy - Kx^2 (in CY), Q21 = Ky^2 - Cx (in CX)
     for (int64_t e = threadIdx.x; e < NN; e += NT) {
         const int i = (int)(e % N), j = (int)(e / N);
-        cd q12 = s.CY[e], q21 = -s.CX[e];
+        cd q12 = CYin[e], q21 = -CXin[e];
         if (i == j) {
             q12 = q12 - cd{geo[i].x * geo[i].x, 0.0};
             q21 = q21 + cd{geo[N + i].x * geo[N + i].x, 0.0};
@@ -1246,16 +1311,21 @@ void smatrix2d_launch(int grid, hipStream_t s, const Inst2D* insts, const double
 #if SRCWA_MAIN_TU
 void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    int part, int flags, hipEvent_t* ev, const int* eig_order) {
+                    int part, int flags, hipEvent_t* ev, const int* eig_order, const OpsArgs& ops) {
     const bool diag = flags & PLAN_DIAGNOSTICS;
-    // part 1: setup, assembly, eigensolves (events ev[0..2]); part 2: modes, S-matrix, efficiencies
-    // (events ev[3], ev[4]); the host may run the parts of different chunks on two streams
+    // part 1: shared operators (first chunk), setup, assembly, eigensolves (events ev[0..2]); part 2:
+    // modes, S-matrix, efficiencies (events ev[3], ev[4]); the host may run the parts of different chunks
+    // on two streams
     if (part & 1) {
         if (ev) hipEventRecord(ev[0], s);
+        if (ops.compute && ops.n > 0)
+            k_ops2d_shared<<<ops.n, NT, 0, s>>>(ops.descs, static_cast<const Inst2D*>(ops.insts), stacks, layouts,
+                                                ops.ops, ops.info);
         k_setup2d<<<n_inst, NT, 0, s>>>(insts, stacks, layouts, ws, out.geometry);
         if (n_tasks > 0) {
-            k_ops2d<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info);
-            k_pq2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info, out.task_diag);
+            k_ops2d<<<n_tasks, NT, 0, s>>>(insts, tasks, stacks, layouts, ws, task_info, ops.info);
+            k_pq2d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info, out.task_diag, ops.descs, ops.ops,
+                                          ops.info);
         }
         if (ev) hipEventRecord(ev[1], s);
         // each matrix goes to the wave or the workgroup eigensolver by its own order

@@ -27,10 +27,10 @@ void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, 
                    int part, int flags, hipEvent_t* ev, const OpsArgs& ops);
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    int part, int flags, hipEvent_t* ev, const int* eig_order);
+                    int part, int flags, hipEvent_t* ev, const int* eig_order, const OpsArgs& ops);
 void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
-                    int part, int flags, hipEvent_t* ev, const int* eig_order);
+                    int part, int flags, hipEvent_t* ev, const int* eig_order, const OpsArgs& ops);
 void launch_zgemm_batched(hipStream_t s, int count, const DenseItem* items, cd* arena);
 void launch_zgesv_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info);
 #if SRCWA_MIXED_PRECISION
@@ -637,11 +637,11 @@ void enqueue(srcwa_plan* plan, bool timing) {
         else if (plan->kind == 2)
             launch_conical(st, ni, nt, plan->insts.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p, plan->layouts.p,
                            ws, plan->task_info.p + 4 * ch.t0, out, plan->max_iter, plan->min_n, plan->max_n, part,
-                           plan->flags, ev, plan->eig_order.p + ch.t0);
+                           plan->flags, ev, plan->eig_order.p + ch.t0, ops);
         else
             launch_crossed(st, ni, nt, plan->insts2d.p + ch.i0, plan->tasks.p + ch.t0, plan->stacks.p,
                            plan->layouts.p, ws, plan->task_info.p + 4 * ch.t0, out, plan->max_iter, plan->min_n,
-                           plan->max_n, part, plan->flags, ev, plan->eig_order.p + ch.t0);
+                           plan->max_n, part, plan->flags, ev, plan->eig_order.p + ch.t0, ops);
         hip_check(hipGetLastError(), "kernel launch");
     };
     for (size_t c = 0; c < nc; ++c) {
@@ -940,6 +940,8 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
         std::vector<int64_t> h_layouts;
         int64_t orders = 0, chunk_ws = 0, max_ws = 0;
         PlanChunk cur{0, 0, 0, 0};
+        OpSharing sharing;
+        sharing.enabled = !(plan->flags & (PLAN_DIAGNOSTICS | PLAN_POINT_OPS));
         for (int i = 0; i < n_problems; ++i) {
             const int s = stack_index[i];
             if (s < 0 || s >= n_stacks) invalid("rcwa: stack index out of range");
@@ -1001,7 +1003,15 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
             }
             const int first_task = static_cast<int>(h_tasks.size());
             for (int j = 0; j < L; ++j)
-                if (general[j + 1]) h_tasks.push_back(LayerTask{i - cur.i0, j});
+                if (general[j + 1]) {
+                    h_tasks.push_back(LayerTask{i - cur.i0, j});
+                    // [[eps]], [[eps]]^-1 and [[1/eps]]^-1 (rcwa1d_conical.hpp:302-336)
+                    const LayerInfo& lj = all[eff[j]];
+                    std::vector<uint64_t> key;
+                    for (double v : {st[0], double(M[i])}) OpSharing::push(key, v);
+                    for (int64_t k = 0; k < lj.rec_words; ++k) OpSharing::push(key, st[lj.rec + k]);
+                    sharing.add(key, i, eff[j], conical_ops_words(N) + fft_scratch_words(lj, M[i]));
+                }
             const int64_t layout_off = static_cast<int64_t>(h_layouts.size());
             for (int r = 0; r < conical_layout_size(L); ++r) {
                 int64_t v = lay[r];
@@ -1036,6 +1046,8 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
         max_ws = std::max(max_ws, chunk_ws);
         plan->n_tasks = static_cast<int>(h_tasks.size());
         plan->total_orders = orders;
+        std::vector<LayerOps> h_ops;
+        const int64_t op_words = sharing.assign(h_tasks, h_ops);
         if (need) {
             const auto b = [](size_t x) { return static_cast<int64_t>(srcwa_pool::round(x)); };
             *need = b(h_insts.size() * sizeof(Inst1D)) + b(h_tasks.size() * sizeof(LayerTask)) +
@@ -1043,12 +1055,13 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
                     b((plan->chunks.size() > 1 ? 2 : 1) * max_ws * sizeof(cd)) + b(4 * std::max(1, plan->n_tasks) * sizeof(int)) +
                     2 * b(8 * orders * sizeof(double)) + 2 * b(4 * orders * sizeof(double)) +
                     b(GEO_WORDS * orders * sizeof(double)) + 2 * b(n_problems * sizeof(int)) +
-                    diag_bytes(n_problems, plan->n_tasks);
+                    diag_bytes(n_problems, plan->n_tasks) + ops_bytes(h_ops, op_words);
             return;
         }
         hipStream_t st = ctx->stream;
         plan->insts.upload(h_insts, st);
         plan->tasks.upload(h_tasks, st);
+        plan->upload_ops(h_ops, op_words, st);
         plan->eig_order.upload(longest_first(plan->chunks, h_tasks, h_insts), st);
         plan->stacks.upload(ps.words, st);
         plan->layouts.upload(h_layouts, st);
@@ -1491,6 +1504,7 @@ struct Layer2DInfo {
     bool patterned;
     int K = 0, maxkx = 1;
     std::vector<double> cuts;     // merged x cuts (patterned layers)
+    int64_t rec = 0, rec_words = 0;   // the layer's words after its thickness in the stack record
 };
 
 // Parses and validates one crossed-grating stack record; returns per-layer info and the record length.
@@ -1565,6 +1579,8 @@ std::vector<Layer2DInfo> parse_stack2d(const double* p, int64_t avail, int64_t& 
             if (cuts.empty() || cuts.front() > 1e-12) cuts.insert(cuts.begin(), 0.0);
             if (cuts.back() < Lx - 1e-12) cuts.push_back(Lx);
             info.cuts = std::move(cuts);
+            info.rec = start;
+            info.rec_words = k - start;
         } else {
             invalid("rcwa2d: unknown layer kind");
         }
@@ -1685,6 +1701,8 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         std::vector<int64_t> h_layouts;
         int64_t orders = 0, chunk_ws = 0, max_ws = 0;
         PlanChunk cur{0, 0, 0, 0};
+        OpSharing sharing;
+        sharing.enabled = !(plan->flags & (PLAN_DIAGNOSTICS | PLAN_POINT_OPS));
         const int64_t limit = (std::numeric_limits<int>::max() - 1) / 4;
         for (int i = 0; i < n_problems; ++i) {
             const int s = stack_index[i];
@@ -1746,7 +1764,17 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
             }
             const int first_task = static_cast<int>(h_tasks.size());
             for (int j = 0; j < L; ++j)
-                if (li[j].patterned) h_tasks.push_back(LayerTask{i - cur.i0, j});
+                if (li[j].patterned) {
+                    h_tasks.push_back(LayerTask{i - cur.i0, j});
+                    // Eps, Cx, Cy (rcwa2d_operators.hpp:349-429) and Eps^-1 (rcwa2d.hpp:412-420)
+                    const double* st = stacks + stack_offsets[s];
+                    std::vector<uint64_t> key;
+                    for (double v : {st[0], st[1], double(M1[i]), double(M2[i]), double(fz)}) OpSharing::push(key, v);
+                    for (int64_t k = 0; k < li[j].rec_words; ++k) OpSharing::push(key, st[li[j].rec + k]);
+                    const int Kc = static_cast<int>(li[j].cuts.size()) - 1;
+                    sharing.add(key, i, j,
+                                crossed_ops_words(N, ops2d_scratch_words(li[j].K, Kc, li[j].maxkx, M1[i], M2[i])));
+                }
             const int64_t layout_off = static_cast<int64_t>(h_layouts.size());
             for (int r = 0; r < x2_layout_size(L); ++r) {
                 int64_t v = lay[r];
@@ -1783,6 +1811,8 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         max_ws = std::max(max_ws, chunk_ws);
         plan->n_tasks = static_cast<int>(h_tasks.size());
         plan->total_orders = orders;
+        std::vector<LayerOps> h_ops;
+        const int64_t op_words = sharing.assign(h_tasks, h_ops);
         if (need) {
             const auto b = [](size_t x) { return static_cast<int64_t>(srcwa_pool::round(x)); };
             *need = b(h_insts.size() * sizeof(Inst2D)) + b(h_tasks.size() * sizeof(LayerTask)) +
@@ -1791,12 +1821,13 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
                     b((plan->chunks.size() > 1 ? 2 : 1) * max_ws * sizeof(cd)) + b(4 * std::max(1, plan->n_tasks) * sizeof(int)) +
                     2 * b(8 * orders * sizeof(double)) + 2 * b(orders * sizeof(double)) +
                     b(GEO_WORDS * orders * sizeof(double)) + 2 * b(n_problems * sizeof(int)) +
-                    diag_bytes(n_problems, plan->n_tasks);
+                    diag_bytes(n_problems, plan->n_tasks) + ops_bytes(h_ops, op_words);
             return;
         }
         hipStream_t st = ctx->stream;
         plan->insts2d.upload(h_insts, st);
         plan->tasks.upload(h_tasks, st);
+        plan->upload_ops(h_ops, op_words, st);
         plan->eig_order.upload(longest_first(plan->chunks, h_tasks, h_insts), st);
         plan->stacks.upload(h_stacks, st);
         plan->layouts.upload(h_layouts, st);

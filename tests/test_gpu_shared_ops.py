@@ -128,3 +128,108 @@ def test_planar_singular_shared_operator(ctx, pctx):
     assert np.all(np.isnan(ra["DEr"][:2]))
     a.close()
     b.close()
+
+
+def test_conical_shared_equals_per_point(ctx, pctx, ref_lib):
+    from oracle import ref
+    from stablercwa_amd import rcwa
+    from stablercwa_amd.types import FourierMethod, Layer, Stack
+    p = 0.7
+    g1 = Layer.grating(0.3, [0.0, 0.25, p], [complex(1.8, 0.01), 1.0])
+    g2 = Layer.grating(0.2, [0.0, 0.4, p], [2.2, 1.3], method=FourierMethod.FFT, samples=2048)
+    a = Stack(1.0, [g1, Layer.uniform(0.1, 1.6), g2], 1.5, p)
+    b = Stack(1.0, [Layer.grating(0.45, [0.0, 0.25, p], [complex(1.8, 0.01), 1.0]), Layer.uniform(0.1, 1.6), g2], 1.5, p)
+    probs = [(s, wl, th, ph, 8) for s in (0, 1) for wl, th, ph in ((0.9, 20.0, 30.0), (1.1, 45.0, 10.0), (1.25, 5.0, 70.0))]
+    probs += [(0, 1.0, 30.0, 45.0, 14), (1, 1.05, 10.0, 20.0, 14), (0, 1.2, 0.0, 0.0, 5)]
+    x = rcwa.ConicalBatch([a, b], probs, ctx)
+    y = rcwa.ConicalBatch([a, b], probs, pctx)
+    rx, nx = _run(x)
+    ry, ny = _run(y)
+    assert nx == 4 and ny == 0              # two grating layers at M = 8 and at M = 14; M = 5 alone
+    for k in ("Jr", "Jt", "DEr", "DEt", "status", "eig_info", "geometry"):
+        assert rx[k].tobytes() == ry[k].tobytes(), k
+    assert np.all(rx["status"] == 0)
+    stacks = [a, b]
+    for i in (0, 4, 6, 7):
+        s, wl, th, ph, M = probs[i]
+        o = ref.solve_conical_ex(ref_lib, stacks[s].flat(), wl, th, ph, M, diagnostics=False)
+        assert np.max(np.abs(rx["DEr"][x.order_offsets[i]:x.order_offsets[i + 1]].ravel() - o["DEr"])) <= 1e-10, i
+        assert np.max(np.abs(rx["DEt"][x.order_offsets[i]:x.order_offsets[i + 1]].ravel() - o["DEt"])) <= 1e-10, i
+    x.close()
+    y.close()
+
+
+def test_conical_schur_layers_shared(ctx, pctx):
+    """The reference's degenerate (Schur-basis) layers: the Schur path reads [[eps]]^-1 and C from the set."""
+    import json
+    import os
+    from conftest import GOLDEN
+    from stablercwa_amd import rcwa
+    cases = [c for c in json.load(open(os.path.join(GOLDEN, "conical_ref.json")))["cases"] if c["n_schur"] > 0]
+
+    class _Flat:                       # a stack given as its flat record
+        def __init__(self, f):
+            self.f = np.asarray(f, dtype=np.float64)
+
+        def flat(self):
+            return self.f
+
+    stack = _Flat
+    probs = [(i, c["wavelength"], c["theta"], c["phi"], c["M"]) for i, c in enumerate(cases)]
+    x = rcwa.ConicalBatch([stack(c["stack"]) for c in cases], probs, ctx)
+    y = rcwa.ConicalBatch([stack(c["stack"]) for c in cases], probs, pctx)
+    rx, nx = _run(x)
+    ry, _ = _run(y)
+    assert nx >= 1
+    for k in ("Jr", "Jt", "DEr", "DEt", "status", "eig_info"):
+        assert rx[k].tobytes() == ry[k].tobytes(), k
+    for i, c in enumerate(cases):
+        d = rx["DEr"][x.order_offsets[i]:x.order_offsets[i + 1]]
+        assert np.max(np.abs(d - np.reshape(c["DEr"], (-1, 4)))) <= 1e-10, c["name"]
+    x.close()
+    y.close()
+
+
+def _pillar(d, r):
+    from stablercwa_amd.types import Layer2D, XProfile2D
+    L = 0.8
+    ys = [0.0, 0.4 - r, 0.4 + r, L]
+    rows = [XProfile2D([0.0, L], [1.0]), XProfile2D([0.0, 0.4 - r, 0.4 + r, L], [1.0, complex(2.0, 0.01), 1.0]),
+            XProfile2D([0.0, L], [1.0])]
+    return Layer2D.patterned(d, ys, rows)
+
+
+def test_crossed_shared_equals_per_point(ctx, pctx, ref_lib):
+    from oracle import ref
+    from stablercwa_amd import rcwa
+    from stablercwa_amd.types import Layer2D, Stack2D
+    a = Stack2D(nI=1.0, nII=1.46, Lx=0.8, Ly=0.8, layers=[_pillar(0.3, 0.2), Layer2D.uniform(0.1, 1.7), _pillar(0.2, 0.12)])
+    b = Stack2D(nI=1.0, nII=1.46, Lx=0.8, Ly=0.8, layers=[_pillar(0.5, 0.2), Layer2D.uniform(0.1, 1.7), _pillar(0.2, 0.12)])
+    j = (complex(0.6), complex(0.0, 0.8))
+    probs = [(s, wl, th, ph, 3, 3, j) for s in (0, 1) for wl, th, ph in ((1.0, 10.0, 30.0), (1.2, 35.0, -20.0))]
+    probs += [(0, 0.8, 0.0, 0.0, 3, 3, j)]                       # lambda = Lx at normal incidence: Rayleigh
+    probs += [(0, 1.1, 20.0, 10.0, 3, 3, j, "laurent"), (1, 1.3, 5.0, 0.0, 3, 3, j, "laurent")]
+    probs += [(0, 1.05, 15.0, 45.0, 2, 4, j)]                    # alone
+    x = rcwa.CrossedBatch([a, b], probs, ctx)
+    y = rcwa.CrossedBatch([a, b], probs, pctx)
+    rx, nx = _run(x)
+    ry, ny = _run(y)
+    assert nx == 4 and ny == 0              # two patterned layers, Li and Laurent, at M = 3
+    for k in ("R", "T", "DEr", "DEt", "status", "eig_info", "geometry"):
+        assert rx[k].tobytes() == ry[k].tobytes(), k
+    assert rx["status"][4] == 3 and np.all(np.delete(rx["status"], 4) == 0)
+    stacks = [a, b]
+    for i in (0, 3, 7):
+        s, wl, th, ph, M1, M2, jj = probs[i][:7]
+        jv = [jj[0].real, jj[0].imag, jj[1].real, jj[1].imag]
+        o = ref.solve2d_ex(ref_lib, stacks[s].flat(), wl, th, ph, M1, M2, jv, diagnostics=False)
+        sl = slice(x.order_offsets[i], x.order_offsets[i + 1])
+        assert np.max(np.abs(rx["DEr"][sl] - o["DEr"])) <= 1e-10 and np.max(np.abs(rx["DEt"][sl] - o["DEt"])) <= 1e-10, i
+    many = rcwa.CrossedBatch([a, b], probs, ctx, max_workspace_bytes=max(1 << 20, x.device_bytes() // 5))
+    assert many.chunks() > 1
+    rm, _ = _run(many)
+    for k in ("R", "T", "DEr", "DEt", "status", "eig_info"):
+        assert rm[k].tobytes() == rx[k].tobytes(), ("chunked", k)
+    many.close()
+    x.close()
+    y.close()
