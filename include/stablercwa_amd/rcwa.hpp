@@ -91,7 +91,8 @@ inline PlanReport report(srcwa_plan* p, int problems) {
 // Plan flags of a call's options.
 template <class Opt>
 int plan_flags(const Opt& o) {
-    return (o.diagnostics ? SRCWA_PLAN_DIAGNOSTICS : 0) | (o.mixed_precision ? SRCWA_PLAN_MIXED : 0);
+    return (o.diagnostics ? SRCWA_PLAN_DIAGNOSTICS : 0) | (o.mixed_precision ? SRCWA_PLAN_MIXED : 0) |
+           (o.share_layer_operators ? 0 : SRCWA_PLAN_POINT_OPS);
 }
 
 // RAII owner of a launch plan.
@@ -185,11 +186,14 @@ struct Options {
     std::function<EigResult(const Matrix&)> custom_eig;
     std::function<Matrix(const Matrix&, const Matrix&)> custom_multiply;
     std::function<Matrix(const Matrix&, const Matrix&)> custom_solve;
-    // this build only: the GPU, the per-launch device workspace bound (0 = 64 GiB), and the mixed-precision
-    // star products (S-matrix solves by FP32 LU + FP64 iterative refinement; SRCWA_PLAN_MIXED)
+    // this build only: the GPU, the per-launch device workspace bound (0 = 64 GiB), the mixed-precision
+    // star products (S-matrix solves by FP32 LU + FP64 iterative refinement; SRCWA_PLAN_MIXED), and whether
+    // the points of a sweep share each layer's wavelength-invariant operators (false: every point computes
+    // its own, SRCWA_PLAN_POINT_OPS; the results are the same bits)
     int device = 0;
     std::int64_t max_workspace_bytes = 0;
     bool mixed_precision = false;
+    bool share_layer_operators = true;
 };
 
 struct Point {
@@ -380,6 +384,7 @@ struct Options2D {
     int device = 0;
     std::int64_t max_workspace_bytes = 0;
     bool mixed_precision = false;
+    bool share_layer_operators = true;
 };
 
 struct Timing2D {
