@@ -698,7 +698,7 @@ void SRCWA_QC_FN(conical_qc)(const QcLaunch& a) {
 
 // ------------------------------------------------------------------------------------------------
 #if SRCWA_IN_PART(1)
-__global__ SMAT_KERNEL_BOUNDS void k_smatrix_conical(const Inst1D* __restrict__ insts,
+__global__ SMAT_KERNEL_BOUNDS_OF(SRCWA_SMAT_OCC_CONICAL) void k_smatrix_conical(const Inst1D* __restrict__ insts,
                                                         const double* __restrict__ stacks,
                                                         const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                                         const int* __restrict__ task_info, Out1D out) {

@@ -311,7 +311,7 @@ __global__ __launch_bounds__(NT) void k_modes1d(const Inst1D* __restrict__ insts
 
 // ------------------------------------------------------------------------------------------------
 #if SRCWA_IN_PART(1)
-__global__ SMAT_KERNEL_BOUNDS void k_smatrix1d(const Inst1D* __restrict__ insts, const double* __restrict__ stacks,
+__global__ SMAT_KERNEL_BOUNDS_OF(SRCWA_SMAT_OCC_PLANAR) void k_smatrix1d(const Inst1D* __restrict__ insts, const double* __restrict__ stacks,
                                                   const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                                   const int* __restrict__ task_info, Out1D out) {
     const Inst1D in = insts[blockIdx.x];

@@ -807,7 +807,7 @@ struct Region2D {
 };
 
 #if SRCWA_IN_PART(1)
-__global__ SMAT_KERNEL_BOUNDS void k_smatrix2d(const Inst2D* __restrict__ insts, const double* __restrict__ stacks,
+__global__ SMAT_KERNEL_BOUNDS_OF(SRCWA_SMAT_OCC_CROSSED) void k_smatrix2d(const Inst2D* __restrict__ insts, const double* __restrict__ stacks,
                                                   const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                                   const int* __restrict__ task_info, Out1D out) {
     __shared__ int s_kind[SMAT_MAX_BLOCKS];

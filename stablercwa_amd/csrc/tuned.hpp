@@ -71,9 +71,25 @@
 #ifndef SRCWA_EIG_BLOCKED_REDUCE
 #define SRCWA_EIG_BLOCKED_REDUCE 0
 #endif
-// minimum waves per SIMD the S-matrix kernels are built for
-#ifndef SRCWA_SMAT_OCC
-#define SRCWA_SMAT_OCC 2
+// minimum waves per SIMD each family's S-matrix kernel is built for (two: a few registers spill around
+// phase boundaries, but a second instance per CU hides the LU and substitution latency chains)
+#ifndef SRCWA_SMAT_OCC_PLANAR
+#define SRCWA_SMAT_OCC_PLANAR 2
+#endif
+#ifndef SRCWA_SMAT_OCC_CONICAL
+#define SRCWA_SMAT_OCC_CONICAL 2
+#endif
+#ifndef SRCWA_SMAT_OCC_CROSSED
+#define SRCWA_SMAT_OCC_CROSSED 2
+#endif
+// SRCWA_SMAT_OCC (one value for every family) is what the tuner's variant builds set
+#ifdef SRCWA_SMAT_OCC
+#undef SRCWA_SMAT_OCC_PLANAR
+#undef SRCWA_SMAT_OCC_CONICAL
+#undef SRCWA_SMAT_OCC_CROSSED
+#define SRCWA_SMAT_OCC_PLANAR SRCWA_SMAT_OCC
+#define SRCWA_SMAT_OCC_CONICAL SRCWA_SMAT_OCC
+#define SRCWA_SMAT_OCC_CROSSED SRCWA_SMAT_OCC
 #endif
 
 // 1: wg_lu and wg_lu_solve are called, not inlined, at each of their call sites (one copy per kernel
@@ -91,4 +107,4 @@
 
 // (a minimum of one wave per SIMD is the same as no bound)
 #define EIG_KERNEL_BOUNDS_OF(occ) __launch_bounds__(NT, ((occ) > 0 ? (occ) : 1))
-#define SMAT_KERNEL_BOUNDS __launch_bounds__(NT, SRCWA_SMAT_OCC)
+#define SMAT_KERNEL_BOUNDS_OF(occ) __launch_bounds__(NT, ((occ) > 0 ? (occ) : 1))

@@ -1,18 +1,23 @@
-"""Autotuner for the eigen kernels' compile-time constants (stablercwa_amd/csrc/tuned.hpp).
+"""Autotuner for the eigen and S-matrix kernels' compile-time constants (stablercwa_amd/csrc/tuned.hpp).
 
-The eigensolver dominates every workload; its speed depends on how far each column prefetches in the
-QR sweep (SRCWA_EIG_RING), on the occupancy the workgroup kernels are built for (SRCWA_EIG_OCC: fewer
-registers and more resident matrices per CU, against spills) and on the order up to which one wave
-solves a matrix alone (SRCWA_WAVE_EIG_MAX_N).  The best point differs between chips and compilers, so
-it is measured, not guessed:
+Two search spaces (--space):
 
-    python -m stablercwa_amd.tune build                  # CPU: one library per variant in _tune/
-    python -m stablercwa_amd.tune measure -o r.json      # GPU: time the eigen stage of each variant
-    python -m stablercwa_amd.tune write r.json           # rewrite csrc/tuned.hpp with the fastest
+* eig: the eigensolver dominates every workload; its speed depends on how far each column prefetches in
+  the QR sweep (SRCWA_EIG_RING), on the occupancy the workgroup kernels are built for (SRCWA_EIG_OCC:
+  fewer registers and more resident matrices per CU, against spills) and on the order up to which one
+  wave solves a matrix alone (SRCWA_WAVE_EIG_MAX_N); scored by the summed eigen-stage time;
+* smat: the occupancy the S-matrix kernels are built for (SRCWA_SMAT_OCC: one instance per SIMD with no
+  spills, or two with a few spills around phase boundaries); scored per family by its S-matrix stage.
+
+The best point differs between chips and compilers, so it is measured, not guessed:
+
+    python -m stablercwa_amd.tune build [--space S]              # CPU: one library per variant in _tune/
+    python -m stablercwa_amd.tune measure [--space S] -o r.json  # GPU: time the stages of each variant
+    python -m stablercwa_amd.tune write [--space S] r.json       # rewrite csrc/tuned.hpp with the fastest
 
 The measured workloads are the benchmark's: the planar M = 60 sweep (8192 solves), 512 instances of
-the conical batch and 128 of the crossed pillar batch; the score is the summed eigen-stage time.
-Every variant's planar results are also checked against the default build's (max |dDE| <= 1e-10).
+the conical batch and 128 of the crossed pillar batch.  Every variant's planar results are also checked
+against the first variant's (max |dDE| <= 1e-10).
 """
 from __future__ import annotations
 
@@ -36,14 +41,21 @@ OCCS = (0, 3, 4, 5)
 WAVE_MAX = (64,)
 
 
-def variants():
+SMAT_OCCS = (2, 1)
+
+
+def variants(space="eig"):
+    if space == "smat":
+        for occ in SMAT_OCCS:
+            yield f"s{occ}", dict(SRCWA_SMAT_OCC=occ)
+        return
     for ring, occ, wmax in itertools.product(RINGS, OCCS, WAVE_MAX):
         yield f"r{ring}_o{occ}_w{wmax}", dict(SRCWA_EIG_RING=ring, SRCWA_EIG_OCC=occ, SRCWA_WAVE_EIG_MAX_N=wmax)
 
 
 def cmd_build(args):
     from . import build as b
-    for name, defs in variants():
+    for name, defs in variants(args.space):
         out = os.path.join(TUNE_DIR, name, "libsrcwa_amd.so")
         if os.path.exists(out) and not args.force:
             continue
@@ -115,7 +127,7 @@ def _measure_one(path):
         _lib.check(lib.srcwa_plan_sync(h))
         ms = np.zeros(4)
         _lib.check(lib.srcwa_plan_stage_ms(h, _lib.ptr(ms)))
-        res[name] = dict(eig_ms=float(ms[1]), total_ms=float(ms[3]))
+        res[name] = dict(eig_ms=float(ms[1]), smat_ms=float(ms[2]), total_ms=float(ms[3]), stages_ms=ms.tolist())
         if name == "planar":
             o = lib.srcwa_plan_orders(h)
             planar_der = np.empty(o)
@@ -128,7 +140,7 @@ def _measure_one(path):
 def cmd_measure(args):
     rows = {}
     ref = None
-    names = [n for n, _ in variants()]
+    names = [n for n, _ in variants(args.space)]
     for name in names:
         path = os.path.join(TUNE_DIR, name, "libsrcwa_amd.so")
         if not os.path.exists(path):
@@ -146,7 +158,8 @@ def cmd_measure(args):
         if ref is None:
             ref = der
         row["max_dDE_vs_first"] = float(np.max(np.abs(der - ref)))
-        row["score_ms"] = sum(row[k]["eig_ms"] for k in ("planar", "conical", "crossed"))
+        stage = "smat_ms" if args.space == "smat" else "eig_ms"
+        row["score_ms"] = sum(row[k][stage] for k in ("planar", "conical", "crossed"))
         rows[name] = row
         print(name, json.dumps(row), flush=True)
     with open(args.output, "w") as f:
@@ -166,6 +179,14 @@ def cmd_write(args):
     with open(args.results) as f:
         rows = json.load(f)
     ok = {k: v for k, v in rows.items() if v["max_dDE_vs_first"] <= 1e-10}
+    if args.space == "smat":
+        defs = dict(variants("smat"))
+        chosen = {}
+        for fam, key in (("planar", "SRCWA_SMAT_OCC_PLANAR"), ("conical", "SRCWA_SMAT_OCC_CONICAL"),
+                         ("crossed", "SRCWA_SMAT_OCC_CROSSED")):
+            chosen[key] = defs[min(ok, key=lambda k: ok[k][fam]["smat_ms"])]["SRCWA_SMAT_OCC"]
+        _write_tuned(chosen)
+        return
     best = min(ok, key=lambda k: ok[k]["score_ms"])
     defs = dict(variants())
     chosen = {"SRCWA_EIG_RING": defs[best]["SRCWA_EIG_RING"], "SRCWA_WAVE_EIG_MAX_N": defs[best]["SRCWA_WAVE_EIG_MAX_N"]}
@@ -173,6 +194,10 @@ def cmd_write(args):
     for fam, key in (("planar", "SRCWA_EIG_OCC_PLANAR"), ("conical", "SRCWA_EIG_OCC_CONICAL"),
                      ("crossed", "SRCWA_EIG_OCC_CROSSED")):
         chosen[key] = defs[min(same, key=lambda k: ok[k][fam]["eig_ms"])]["SRCWA_EIG_OCC"]
+    _write_tuned(chosen)
+
+
+def _write_tuned(chosen):
     src = open(TUNED_HPP).read()
     for k, v in chosen.items():
         src = re.sub(rf"(#define {k} )\d+", rf"\g<1>{v}", src)
@@ -188,6 +213,8 @@ def main():
     m = sub.add_parser("measure"); m.add_argument("-o", "--output", default="tune_results.json")
     o = sub.add_parser("_one"); o.add_argument("path"); o.add_argument("out")
     w = sub.add_parser("write"); w.add_argument("results")
+    for p in (b, m, w):
+        p.add_argument("--space", choices=("eig", "smat"), default="eig")
     args = ap.parse_args()
     {"build": cmd_build, "measure": cmd_measure, "_one": cmd_one, "write": cmd_write}[args.cmd](args)
 
