@@ -7,8 +7,9 @@
 //                                      below the sub-diagonal and then a zero leading entry); prints
 //                                      info, the largest eigen-pair residual and the sorted eigenvalues
 //   emu_main eigfile QC n path         the same for a column-major complex matrix read from a file
-//   emu_main refine n nrhs seed        the mixed-precision solve (wg_refine_solve) of a seeded diagonally
+//   emu_main refine n nrhs seed [nan]  the mixed-precision solve (wg_refine_solve) of a seeded diagonally
 //                                      weighted system; prints fallback, corrections and the backward error
+//                                      (nan: its scratch starts as NaNs, as uncleared device memory may)
 #include <algorithm>
 #include <complex>
 #include <cstdio>
@@ -131,6 +132,10 @@ int main(int argc, char** argv) {
         for (int j = 0; j < n; ++j)
             for (int i = 0; i < n; ++i) A[i + (long)j * n] = cd{nd(rng) + (i == j ? std::sqrt(double(n)) : 0.0), nd(rng)};
         for (cd& v : B) v = cd{nd(rng), nd(rng)};
+        // "nan": the scratch starts as quiet NaNs instead of zeros (device memory is not cleared), so a
+        // read of scratch that was never written shows in the results
+        if (argc >= 6 && std::string(argv[5]) == "nan")
+            for (cd& v : S) v = cd{std::nan(""), std::nan("")};
         const std::vector<cd> B0 = B;
         cd* lu64 = S.data();
         int* piv = reinterpret_cast<int*>(lu64 + (long)n * n);

@@ -10,7 +10,7 @@ AddressSanitizer.  What this pins:
   residual level, also through a zero column and a zero leading entry, and on the reference's layer matrix of
   the kRidgesSilver50TE table case (order 101) on which their GPU run faulted
 * the mixed-precision solve (wg_la32.hpp, also compiled out): FP64 backward error after a few FP32
-  corrections, no fallback on well-conditioned systems
+  corrections, no fallback on well-conditioned systems, also with NaN-filled scratch
 No GPU is used; an out-of-bounds access in the emulated code aborts the run under AddressSanitizer.
 """
 import ctypes
@@ -111,8 +111,11 @@ def test_blocked_householder_on_the_matrix_of_its_gpu_fault(emu, tmp_path):
     assert np.max(np.abs(b["w"] - sorted_eigs(Am))) <= 1e-12 * np.linalg.norm(Am)
 
 
-@pytest.mark.parametrize("n, nrhs", [(1, 1), (33, 8), (64, 7), (100, 100)])
-def test_mixed_precision_solve(emu, n, nrhs):
-    o = run(emu["mixed"], "refine", n, nrhs, 31)
+@pytest.mark.parametrize("n, nrhs, scratch", [(1, 1, ""), (33, 8, ""), (64, 7, ""), (100, 100, ""), (33, 8, "nan"),
+                                               (100, 100, "nan")])
+def test_mixed_precision_solve(emu, n, nrhs, scratch):
+    """scratch "nan": the solve's scratch starts as NaNs (device memory is not cleared), so any read of
+    scratch before it is written would show in the result."""
+    o = run(emu["mixed"], "refine", n, nrhs, 31, *([scratch] if scratch else []))
     assert o["singular"] == 0 and o["fallback"] == 0 and 1 <= o["corrections"] <= 6
     assert o["berr"] <= 1e-15, o["berr"]
