@@ -151,7 +151,8 @@ DEV int wg_lu32(int n, cf* A, int lda, int* piv) {
                     bi = i;
                 }
             }
-            const int p = block_argmax(best, bi);
+            int p = block_argmax(best, bi);
+            if (p >= n) p = k;                 // no finite modulus in the column: keep the row; the pivot test fails
             if (tid == 0) piv[k] = p;
             if (p != k)
                 for (int j = tid; j < n; j += NT) {
