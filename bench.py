@@ -187,6 +187,41 @@ def single_solve_latency(reps=40):
     return out
 
 
+def near_fields_leg(nz=256, nx=256, reps=5):
+    """Planar near fields (solve_with_fields + field_at + field_map): the grating on the 20-film mirror at M = 60,
+    lambda 1.053, theta 66.36, TE and TM, sampled at nz depths through the whole stack and on an nx x nz grid;
+    GPU time per call (capture kept on the device) and the reference's own field_at / field_map on one thread,
+    with the largest field difference between them."""
+    from stablercwa_amd import _lib, rcwa
+    st = mirror_stack()
+    ctx = _lib.Context(0, plan_flags=_lib.PLAN_FIELDS)
+    zs = np.linspace(-0.2, 2.2, nz)
+    out = dict(workload=f"grating on the 20-film mirror, M=60 (N=121), TE and TM, {nz} depths, {nx} x {nz} grid")
+    for pol in (0, 1):
+        b = rcwa.PlanarBatch([st], [(0, 1.053, 66.36, 60, pol)], ctx)
+        b.execute()
+        b.sync()
+        h = b.field_at(0, zs)
+        ta, tm = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter(); h = b.field_at(0, zs); ta.append(time.perf_counter() - t0)
+            t0 = time.perf_counter(); g = b.field_map(0, zs, nx); tm.append(time.perf_counter() - t0)
+        leg = dict(field_at_ms=float(np.median(ta) * 1e3), field_map_ms=float(np.median(tm) * 1e3))
+        from oracle import ref
+        if ref.available():
+            lib = ref.load()
+            lib.ref_set_threads(1)
+            t0 = time.perf_counter()
+            o = ref.planar_fields(lib, st.flat(), 1.053, 66.36, 60, pol, zs, nx=nx)
+            leg["reference_solve_field_at_field_map_ms_1thread"] = (time.perf_counter() - t0) * 1e3
+            scale = max(1.0, float(np.max(np.abs(o["harm"]))))
+            leg["max_abs_dfield_vs_reference"] = max(float(np.max(np.abs(h - o["harm"]))),
+                                                     float(np.max(np.abs(g - o["grid"])))) / scale
+        b.close()
+        out["TE" if pol == 0 else "TM"] = leg
+    return out
+
+
 def measured_eig_traffic():
     """HBM bytes per launch of the planar eigen kernel on this workload, from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE measurement (scaled by the copy-kernel calibration in the same file)."""
@@ -403,7 +438,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--serve-rates", default="2,5,10,20,40",
                     help="arrival rates (requests/s) of the served sweep; each runs max(20, min(60, 6 rate)) requests")
-    ap.add_argument("--only", choices=["conical", "crossed", "served", "copy"],
+    ap.add_argument("--only", choices=["conical", "crossed", "served", "copy", "fields"],
                     help="run one extra leg alone and print its result (for profiling); 'copy' copies 1 GiB on "
                          "the GPU once, the known-size kernel that calibrates the HBM traffic counters")
     args = ap.parse_args()
@@ -421,7 +456,7 @@ def main():
         return
     if args.only:
         leg = {"conical": lambda: conical_leg(with_cpu=False), "crossed": lambda: crossed_leg(with_cpu=False),
-               "served": lambda: served_leg(args.serve_rate, args.serve_requests)}[args.only]
+               "served": lambda: served_leg(args.serve_rate, args.serve_requests), "fields": near_fields_leg}[args.only]
         print(json.dumps({args.only: leg()}), flush=True)
         return
 
@@ -522,6 +557,7 @@ def main():
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         sweep = [served_leg(r, max(20, min(60, int(6 * r)))) for r in rates]
         extra["served"] = dict(by_rate=sweep, saturation=_knee(sweep))
+        extra["near_fields"] = near_fields_leg()
     line = {
         "metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",

@@ -36,6 +36,8 @@ enum {
     SRCWA_EDEVICE = -2,       /* HIP runtime error */
     SRCWA_ENOMEM = -3,        /* device allocation failed */
     SRCWA_EUNSUPPORTED = -4,  /* option not provided by this build */
+    SRCWA_EUNAVAILABLE = -5,  /* a near-field context is unavailable: the reference throws std::logic_error */
+    SRCWA_ENUMERICAL = -6,    /* a near-field evaluation is not finite: the reference throws std::runtime_error */
 };
 
 typedef struct srcwa_context srcwa_context;
@@ -76,8 +78,13 @@ int srcwa_context_set_pool(srcwa_context* ctx, srcwa_pool* pool);
  *   points that share them (the same layer record apart from the thickness, period or cell, truncation and,
  *   planar, polarization): a sweep over wavelength or angle, or over the thickness of a layer, factors
  *   each layer's operators once.  Results are the same bits either way; diagnostics plans always use the
- *   per-point form. */
-enum { SRCWA_PLAN_DIAGNOSTICS = 1, SRCWA_PLAN_MIXED = 2, SRCWA_PLAN_POINT_OPS = 4 };
+ *   per-point form.
+ * SRCWA_PLAN_FIELDS: planar (TE/TM) plans keep each solve's near-field capture, as rcwa::solve_with_fields
+ *   does (rcwa1d.hpp:480-489, 1006-1023; rcwa1d_fields.hpp:55-104): the layer amplitudes a+ (layer top) and
+ *   a- (layer bottom), beside the modes and R, T the solve leaves on the device; srcwa_plan1d_field_at and
+ *   srcwa_plan1d_field_map evaluate them.  Such a plan must fit one workspace chunk.  Conical and crossed
+ *   plans refuse the flag (SRCWA_EUNSUPPORTED). */
+enum { SRCWA_PLAN_DIAGNOSTICS = 1, SRCWA_PLAN_MIXED = 2, SRCWA_PLAN_POINT_OPS = 4, SRCWA_PLAN_FIELDS = 8 };
 int srcwa_context_set_plan_flags(srcwa_context* ctx, int flags);
 
 /* ---------------------------------------------------------------------------------------------
@@ -181,6 +188,23 @@ int srcwa_plan_refinement(srcwa_plan* plan, double* out);
 int srcwa_plan_geometry(srcwa_plan* plan, double* out);
 /* Total result orders (sum of N over problems), device bytes held, number of launch chunks. */
 int64_t srcwa_plan_orders(const srcwa_plan* plan);
+
+/* Near fields of problem `problem` of an executed planar plan created with SRCWA_PLAN_FIELDS.
+ * srcwa_plan1d_field_available: 1 when the problem's capture is usable (PlanarFieldContext::available(),
+ *   rcwa1d_fields.hpp:33; the solve succeeded and every captured mode and amplitude is finite), else 0.
+ * srcwa_plan1d_field_at: the harmonic fields at nz depths z (field_at, rcwa1d_fields.hpp:193-198): out holds
+ *   6 components (Ex, Ey, Ez, Hx, Hy, Hz; H is Z0 H) of N x nz complex, column-major (order i, depth iz at
+ *   i + iz N), component c from c N nz; depths keep their order and repeats (HarmonicFields,
+ *   rcwa_fields.hpp:108-128).  z < 0 is the incidence side, z past the stack the exit side; an interface
+ *   belongs to the layer above it.
+ * srcwa_plan1d_field_map: the same fields on the x grid x_k = k period / nx, k < nx, at height y
+ *   (field_map, rcwa1d_fields.hpp:200-206; reconstruct_field_grid, rcwa_fields.hpp:326-355): 6 components of
+ *   nx x nz complex, (x_k, z_iz) at k + iz nx.
+ * Errors: SRCWA_EINVAL (non-finite depth, nx <= 0, non-finite y, a plan without fields or not executed),
+ * SRCWA_EUNAVAILABLE (the problem's capture is unusable), SRCWA_ENUMERICAL (a non-finite field value). */
+int srcwa_plan1d_field_available(srcwa_plan* plan, int problem, int* available);
+int srcwa_plan1d_field_at(srcwa_plan* plan, int problem, int nz, const double* zs, double* out);
+int srcwa_plan1d_field_map(srcwa_plan* plan, int problem, int nz, const double* zs, int nx, double y, double* out);
 int64_t srcwa_plan_device_bytes(const srcwa_plan* plan);
 int srcwa_plan_chunks(const srcwa_plan* plan);
 /* Shared operator sets of the plan (0 with SRCWA_PLAN_POINT_OPS, with diagnostics, or when no two points

@@ -12,8 +12,9 @@
 // Error behaviour: invalid input throws std::invalid_argument with the reference's message; a numerical
 // failure returns NaN arrays with eig_info set (planar, conical) or status NumericalFailure (crossed);
 // options this build does not provide (custom hooks -- see stablercwa_amd/hooks.hpp for the other
-// direction --, LR eigensolver, non-pivoting or QR linear solvers, homogeneous_fast_path = false, near-field
-// capture) throw srcwa_amd::Unsupported; a HIP runtime error throws std::runtime_error.  FFT-sampled
+// direction --, LR eigensolver, non-pivoting or QR linear solvers, homogeneous_fast_path = false, conical and
+// crossed-grating near-field capture) throw srcwa_amd::Unsupported; a HIP runtime error throws
+// std::runtime_error.  Planar near fields are in stablercwa_amd/fields.hpp.  FFT-sampled
 // grating layers (FourierMethod::FFT) are computed on the GPU.  sweep* run every point in one batched GPU
 // launch; results keep the input order and each result is bitwise the same as its single solve.
 #pragma once
@@ -52,6 +53,7 @@ inline void check(int code) {
     const std::string msg = srcwa_last_error();
     if (code == SRCWA_EINVAL) throw std::invalid_argument(msg);
     if (code == SRCWA_EUNSUPPORTED) throw Unsupported(msg);
+    if (code == SRCWA_EUNAVAILABLE) throw std::logic_error(msg);
     throw std::runtime_error(msg);
 }
 
@@ -649,8 +651,8 @@ inline std::vector<ConicalResult> sweep_conical(const Stack& stack, const std::v
 }
 
 // The reference's near-field capture (rcwa2d.hpp:939-950).  Declared so that reference callers that pass
-// the capture argument compile; this build computes no near fields and throws Unsupported when one is
-// requested.
+// the capture argument compile; this build computes crossed-grating near fields nowhere and throws
+// Unsupported when one is requested (planar near fields: stablercwa_amd/fields.hpp).
 namespace detail2d {
 struct FieldCapture2D;
 }

@@ -147,3 +147,25 @@ def solve2d_many(lib, flats, wavelengths, thetas, phis, M1s, M2s, jones, threads
     if rc != 0:
         raise ValueError(lib.ref_last_error().decode())
     return energy, status
+
+
+def planar_fields(lib, stack_flat, wavelength, theta, M, pol, zs, nx=0, y=0.0):
+    """rcwa::solve_with_fields + field_at (+ field_map with nx > 0) of the reference (rcwa1d_fields.hpp:55-206).
+    Returns dict(available, harm [6, N, nz], grid [6, nz, nx] or None, DEr, DEt); harm/grid are None when the
+    context is unavailable."""
+    st = np.ascontiguousarray(stack_flat, dtype=np.float64)
+    z = np.ascontiguousarray(np.atleast_1d(np.asarray(zs, dtype=np.float64)))
+    N, nz = 2 * M + 1, z.size
+    harm = np.zeros(12 * N * nz)
+    grid = np.zeros(12 * max(nx, 0) * nz)
+    DEr = np.zeros(N); DEt = np.zeros(N)
+    avail = ctypes.c_int(0)
+    rc = lib.ref_planar_fields(_p(st), ctypes.c_double(wavelength), ctypes.c_double(theta), int(M), int(pol), int(nz),
+                               _p(z), int(nx), ctypes.c_double(y), ctypes.byref(avail), _p(harm), _p(grid), _p(DEr),
+                               _p(DEt))
+    if rc != 0:
+        raise RuntimeError(lib.ref_last_error().decode())
+    ok = bool(avail.value)
+    h = harm.view(np.complex128).reshape(6, nz, N).transpose(0, 2, 1) if ok else None
+    g = grid.view(np.complex128).reshape(6, nz, nx) if ok and nx > 0 else None
+    return dict(available=ok, harm=h, grid=g, DEr=DEr, DEt=DEt)

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "stablercwa/rcwa1d.hpp"
+#include "stablercwa/rcwa1d_fields.hpp"
 #include "stablercwa/rcwa1d_conical.hpp"
 #include "stablercwa/rcwa2d.hpp"
 #include "stablercwa/eig.hpp"
@@ -115,6 +116,41 @@ int ref_sweep1d(const double* stack, int n_points, const double* wavelengths, co
             std::memcpy(DEr + static_cast<std::size_t>(i) * n, res[i].DEr.data(), n * sizeof(double));
             std::memcpy(DEt + static_cast<std::size_t>(i) * n, res[i].DEt.data(), n * sizeof(double));
         }
+    });
+}
+
+// Near fields of one planar solve: rcwa::solve_with_fields, then field_at and field_map (rcwa1d_fields.hpp:55-206).
+// available = 1: harm receives the 6 harmonic components (N x nz complex each, order i and depth iz at
+// i + iz N) and, with nx > 0, grid the 6 grid components (nx x nz complex each, x_k and depth iz at
+// k + iz nx); available = 0: the context is unavailable and nothing is written.  DEr, DEt: the solve's
+// efficiencies (N each).
+int ref_planar_fields(const double* stack, double wavelength, double theta_deg, int M, int pol, int nz,
+                      const double* zs, int nx, double y, int* available, double* harm, double* grid, double* DEr,
+                      double* DEt) {
+    return guarded([&] {
+        const auto res = rcwa::solve_with_fields(decode1d(stack), wavelength, theta_deg, M,
+                                                 pol ? rcwa::Polarization::TM : rcwa::Polarization::TE);
+        const int n = 2 * M + 1;
+        std::memcpy(DEr, res.result.DEr.data(), n * sizeof(double));
+        std::memcpy(DEt, res.result.DEt.data(), n * sizeof(double));
+        *available = res.fields.available() ? 1 : 0;
+        if (!*available) return;
+        const stablercwa::RealVector z(zs, zs + nz);
+        const auto h = rcwa::field_at(res, z);
+        for (int c = 0; c < 6; ++c)
+            for (int k = 0; k < n * nz; ++k) {
+                const cplx v = h.components[c].data()[k];
+                harm[2 * (static_cast<std::size_t>(c) * n * nz + k)] = v.real();
+                harm[2 * (static_cast<std::size_t>(c) * n * nz + k) + 1] = v.imag();
+            }
+        if (nx <= 0) return;
+        const auto g = rcwa::field_map(res, z, nx, y);
+        for (int c = 0; c < 6; ++c)
+            for (int k = 0; k < nx * nz; ++k) {
+                const cplx v = g.components[c].data()[k];
+                grid[2 * (static_cast<std::size_t>(c) * nx * nz + k)] = v.real();
+                grid[2 * (static_cast<std::size_t>(c) * nx * nz + k) + 1] = v.imag();
+            }
     });
 }
 

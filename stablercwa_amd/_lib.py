@@ -15,9 +15,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsrcwa_amd.so")
 
 OK, EINVAL, EDEVICE, ENOMEM, EUNSUPPORTED = 0, -1, -2, -3, -4
+EUNAVAILABLE, ENUMERICAL = -5, -6   # near fields: unavailable context (logic_error), non-finite values (runtime_error)
 PLAN_DIAGNOSTICS = 1
 PLAN_MIXED = 2          # star-product solves by FP32 LU + FP64 iterative refinement
 PLAN_POINT_OPS = 4      # every point computes its own wavelength-invariant layer operators
+PLAN_FIELDS = 8         # planar plans keep each solve's near-field capture (field_at / field_map)
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -66,6 +68,10 @@ PROTOTYPES = {
     "srcwa_plan_sync": (ctypes.c_int, [_vp]),
     "srcwa_plan_results": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "srcwa_plan_orders": (ctypes.c_int64, [_vp]),
+    "srcwa_plan1d_field_available": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
+    "srcwa_plan1d_field_at": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    "srcwa_plan1d_field_map": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double,
+                                              _vp]),
     "srcwa_plan_operator_sets": (ctypes.c_int, [_vp]),
     "srcwa_plan_geometry": (ctypes.c_int, [_vp, _vp]),
     "srcwa_plan_device_bytes": (ctypes.c_int64, [_vp]),
@@ -108,6 +114,10 @@ class SrcwaError(RuntimeError):
         self.code = code
 
 
+class FieldUnavailable(SrcwaError):
+    """A near-field context is unavailable (the reference's std::logic_error)."""
+
+
 def check(code: int) -> None:
     if code != OK:
         msg = load().srcwa_last_error().decode("utf-8", "replace")
@@ -115,6 +125,8 @@ def check(code: int) -> None:
             raise ValueError(msg)
         if code == EUNSUPPORTED:
             raise NotImplementedError(msg)
+        if code == EUNAVAILABLE:
+            raise FieldUnavailable(code, msg)
         raise SrcwaError(code, msg)
 
 

@@ -20,6 +20,7 @@ enum : int {
     PLAN_DIAGNOSTICS = 1,      // eigen-pair residuals and solve backward errors (Options::diagnostics)
     PLAN_MIXED = 2,            // S-matrix dense solves by FP32 LU + FP64 iterative refinement
     PLAN_POINT_OPS = 4,        // every layer task computes its own wavelength-invariant operators
+    PLAN_FIELDS = 8,           // planar: keep the near-field capture (layer amplitudes a+, a-) of every solve
 };
 
 // Per-layer-task diagnostics, reduced per instance by the S-matrix kernels.
@@ -46,6 +47,7 @@ struct Inst1D {
     int64_t layout_off;        // offset of the layout table (int64)
     int64_t out_off;           // offset of this instance's orders in the result arrays
     int64_t ws_lo, ws_hi;      // bump region of the S-matrix recursion (cd units)
+    int64_t field_off;         // PLAN_FIELDS: offset of this instance's a+ (L x N) then a- (L x N), cd units
 };
 
 // Samples of an FFT layer (resolve_fft_samples, fourier.hpp:26-44, for inputs the host has validated):
@@ -183,6 +185,18 @@ struct Out1D {
     double* diag;       // DIAG_WORDS per instance
     TaskDiag* task_diag;   // per layer task of the launch chunk
     double* item_res;      // conical: the eigen residual of each of a task's two eigensolves (2 per task)
+    double* fields;        // planar with PLAN_FIELDS: complex layer amplitudes (Inst1D::field_off), else null
+    int* field_ok;         // planar with PLAN_FIELDS: 1 when the instance's capture is whole and finite
+};
+
+// One depth group of a planar near-field sample (rcwa_fields.hpp:253-279): the depths of one region.
+struct FieldGroup1D {
+    int region;           // -1 incidence side, -2 exit side, j >= 0 layer j
+    int first, count;     // its depths: zs[cols[first .. first + count)]
+    int tm_grating;       // TM in a grating layer: Ez needs a [[eps]] solve
+    double z_top, d;      // layer groups: the layer's top and thickness
+    double z_bottom;      // the stack's bottom (exit-side phase reference)
+    int64_t scratch;      // offset of the group's scratch (cd units)
 };
 
 // Per-order geometry written by the conical and crossed setup kernels:

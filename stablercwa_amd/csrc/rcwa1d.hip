@@ -310,7 +310,10 @@ __global__ __launch_bounds__(NT) void k_modes1d(const Inst1D* __restrict__ insts
 #endif
 
 // ------------------------------------------------------------------------------------------------
-#if SRCWA_IN_PART(1)
+// FIELDS: the plan keeps the near-field capture (PLAN_FIELDS).  The two forms are instantiated in units of their
+// own (part 1 without, part 2 with the capture), so plans without fields run the kernel with no capture code.
+#if SRCWA_IN_PART(1) || SRCWA_IN_PART(2)
+template <bool FIELDS>
 __global__ SMAT_KERNEL_BOUNDS_OF(SRCWA_SMAT_OCC_PLANAR) void k_smatrix1d(const Inst1D* __restrict__ insts, const double* __restrict__ stacks,
                                                   const int64_t* __restrict__ layouts, cd* __restrict__ ws,
                                                   const int* __restrict__ task_info, Out1D out) {
@@ -384,8 +387,22 @@ __global__ SMAT_KERNEL_BOUNDS_OF(SRCWA_SMAT_OCC_PLANAR) void k_smatrix1d(const I
         }
         __syncthreads();
         const int cols[1] = {M};
-        run_recursion(c, L, k0, s_thick, region, 1, cols, R, T);
+        cd* fields = FIELDS ? reinterpret_cast<cd*>(out.fields) + in.field_off : nullptr;
+        run_recursion(c, L, k0, s_thick, region, 1, cols, R, T, fields, pure, M);
         status = c.fail ? (c.fail == 2 ? ST_WORKSPACE : ST_NUMERICAL) : ST_OK;
+        if (FIELDS && status == ST_OK) {
+            // the capture is usable when every mode set and amplitude is finite (rcwa1d.hpp:496-508)
+            int bad = 0;
+            for (int64_t e = threadIdx.x; e < 2 * (int64_t)L * N; e += NT) bad |= !finite(fields[e]);
+            for (int r = 0; r < L + 2; ++r) {
+                const Region g = region(r);
+                for (int i = threadIdx.x; i < N; i += NT) bad |= !finite(g.q[i]) || !finite(g.p[i]);
+                if (!g.analytic)
+                    for (int64_t e = threadIdx.x; e < (int64_t)N * N; e += NT) bad |= !finite(g.W[e]) || !finite(g.V[e]);
+            }
+            bad = block_max_int(bad);
+            if (threadIdx.x == 0) out.field_ok[blockIdx.x] = !bad;
+        }
         if (status == ST_OK) {
             __syncthreads();
             if (pure)
@@ -427,6 +444,7 @@ __global__ SMAT_KERNEL_BOUNDS_OF(SRCWA_SMAT_OCC_PLANAR) void k_smatrix1d(const I
         }
     }
     if (threadIdx.x == 0) {
+        if (FIELDS && status != ST_OK) out.field_ok[blockIdx.x] = 0;
         out.status[blockIdx.x] = status;
         out.eig_info[blockIdx.x] = eig_info;
         double* dg = out.diag + (int64_t)DIAG_WORDS * blockIdx.x;
@@ -438,16 +456,209 @@ __global__ SMAT_KERNEL_BOUNDS_OF(SRCWA_SMAT_OCC_PLANAR) void k_smatrix1d(const I
         c.refine_record(dg);
     }
 }
+#endif
+#if SRCWA_IN_PART(1)
 void smatrix1d_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out) {
-    k_smatrix1d<<<grid, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
+    k_smatrix1d<false><<<grid, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
+}
+#endif
+#if SRCWA_IN_PART(2)
+void smatrix1d_fields_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out) {
+    k_smatrix1d<true><<<grid, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
 }
 #endif
 
 // ------------------------------------------------------------------------------------------------
-// the S-matrix kernel is compiled in its own unit
+// the S-matrix kernel is compiled in units of its own
 void smatrix1d_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out);
+void smatrix1d_fields_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out);
 
 #if SRCWA_MAIN_TU
+// ------------------------------------------------------------------------------------------------
+// Near fields of one planar solve (PlanarFieldSampler::sample, rcwa1d_fields.hpp:107-189): one workgroup per
+// depth group.  From the capture the S-matrix kernel kept (a+, a- per layer, R, T) and the modes still in
+// the workspace (q, p per region; W, V per grating layer): the harmonic amplitudes S, U of every depth,
+// then the six components (Ex, Ey, Ez, Hx, Hy, Hz; H is Z0 H), out[c][i + iz N].  TM needs Ez = [[eps]]^-1
+// Kx S, with [[eps]] of a grating layer rebuilt and factored here by the assembly's own routines.
+__global__ __launch_bounds__(NT) void k_fields1d(const Inst1D* __restrict__ inst, const double* __restrict__ stacks,
+                                                 const int64_t* __restrict__ layouts, const cd* __restrict__ ws,
+                                                 const cd* __restrict__ fields, const cd* __restrict__ R,
+                                                 const cd* __restrict__ T, const FieldGroup1D* __restrict__ groups,
+                                                 const int* __restrict__ cols, const double* __restrict__ zs, int nz,
+                                                 cd* __restrict__ out, cd* __restrict__ scratch) {
+    const FieldGroup1D g = groups[blockIdx.x];
+    const Inst1D in = *inst;
+    const double* st = stacks + in.stack_off;
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N, M = in.M, L = in.L, cnt = g.count;
+    const int64_t NC = (int64_t)N * cnt, NZ = (int64_t)N * nz;
+    const double k0 = 2.0 * kPi / in.wavelength;
+    const cd* kx = ws + lay[LAYOUT_KX];
+    cd* Sm = scratch + g.scratch;
+    cd* Um = Sm + NC;
+    cd* Ez = Um + NC;
+    const cd *Sf = Sm, *Uf = Um;
+    cd eps_scale{0.0, 0.0};          // Ez = eps_scale Kx S outside grating layers
+    if (g.region >= 0) {
+        const int j = g.region, r = j + 1;
+        const cd* q = ws + lay[layout_q(r)];
+        const cd* p = ws + lay[layout_p(r)];
+        const int64_t lb = lay[layout_layer(L, j)];
+        const cd* ap = fields + in.field_off + (int64_t)j * N;
+        const cd* am = fields + in.field_off + (int64_t)(L + j) * N;
+        const double d = g.d;
+        for (int64_t e = threadIdx.x; e < NC; e += NT) {
+            const int i = (int)(e % N), col = (int)(e / N);
+            const double z = fmax(0.0, fmin(d, zs[cols[g.first + col]] - g.z_top));
+            const cd a = ap[i], b = am[i], qi = q[i];
+            cd S, U;
+            if (qi.x == 0.0 && qi.y == 0.0) {
+                // the finite-zero basis: linear in z, distances formed first (rcwa1d_fields.hpp:124-131)
+                const double den = 1.0 + 0.5 * k0 * d;
+                const double from_bottom = (1.0 + k0 * (d - z)) / den;
+                const double from_top = (1.0 + k0 * z) / den;
+                S = from_bottom * a + from_top * b;
+                U = cdiv(-a, den) + cdiv(b, den);
+            } else {
+                // each term decays from its own anchor; a zero amplitude takes no exponential
+                const bool az = a.x == 0.0 && a.y == 0.0, bz = b.x == 0.0 && b.y == 0.0;
+                const cd pp = az ? cd{0.0, 0.0} : a * cexp((-k0 * z) * qi);
+                const cd mm = bz ? cd{0.0, 0.0} : b * cexp((-k0 * (d - z)) * qi);
+                S = pp + mm;
+                U = mm - pp;
+            }
+            Sm[e] = S;
+            Um[e] = lb < 0 ? U * p[i] : U;
+        }
+        __syncthreads();
+        if (lb >= 0) {
+            const cd* W = ws + lb;
+            const cd* V = W + (int64_t)N * N;
+            cd* S2 = Ez + NC;
+            cd* U2 = S2 + NC;
+            wg_gemm<GEMM_SET, false>(N, cnt, N, W, N, Sm, N, S2, N);
+            wg_gemm<GEMM_SET, false>(N, cnt, N, V, N, Um, N, U2, N);
+            Sf = S2;
+            Uf = U2;
+        } else {
+            LayerCursor cur(st);
+            LayerView lv = cur.next();
+            for (int k = 0; k < j; ++k) lv = cur.next();
+            eps_scale = cdiv(cd{1.0, 0.0}, lv.n * lv.n);
+        }
+    } else {
+        const int r = g.region == -1 ? 0 : L + 1;
+        const cd* p = ws + lay[layout_p(r)];
+        const cd* qI = ws + lay[layout_q(0)];
+        const cd* qII = ws + lay[layout_q(L + 1)];
+        const bool top = g.region == -1;
+        // kz = i k0 q of the two outer media (the field metadata's kzI, kzII)
+        const cd kz0{-k0 * qI[M].y, k0 * qI[M].x};
+        for (int64_t e = threadIdx.x; e < NC; e += NT) {
+            const int i = (int)(e % N), col = (int)(e / N);
+            const double z = zs[cols[g.first + col]];
+            const cd zero{0.0, 0.0};
+            cd pp = zero, rr = zero, tt = zero;
+            if (top) {
+                const cd kzi{-k0 * qI[i].y, k0 * qI[i].x};
+                if (i == M) pp = cexp(cd{-kz0.y, kz0.x} * z);
+                if (R[i].x != 0.0 || R[i].y != 0.0) rr = R[i] * cexp(cd{kzi.y, -kzi.x} * z);
+            } else if (T[i].x != 0.0 || T[i].y != 0.0) {
+                const cd kzt{-k0 * qII[i].y, k0 * qII[i].x};
+                tt = T[i] * cexp(cd{-kzt.y, kzt.x} * (z - g.z_bottom));
+            }
+            Sm[e] = pp + rr + tt;
+            Um[e] = p[i] * ((rr - pp) - tt);
+        }
+        __syncthreads();
+        const cd n = top ? cd{st[1], st[2]} : cd{st[3], st[4]};
+        eps_scale = cdiv(cd{1.0, 0.0}, n * n);
+    }
+    // a polarization's three other components are zero (make_harmonic_fields starts every component at zero)
+    const cd zero{0.0, 0.0};
+    if (in.pol == 0) {
+        for (int64_t e = threadIdx.x; e < NC; e += NT) {
+            const int i = (int)(e % N), iz = cols[g.first + (int)(e / N)];
+            const cd S = Sf[e], U = Uf[e];
+            out[0 * NZ + i + (int64_t)iz * N] = zero;
+            out[1 * NZ + i + (int64_t)iz * N] = S;
+            out[2 * NZ + i + (int64_t)iz * N] = zero;
+            out[3 * NZ + i + (int64_t)iz * N] = cd{-U.y, U.x};
+            out[4 * NZ + i + (int64_t)iz * N] = zero;
+            out[5 * NZ + i + (int64_t)iz * N] = cd{-kx[i].x * S.x, -kx[i].x * S.y};
+        }
+        return;
+    }
+    for (int64_t e = threadIdx.x; e < NC; e += NT) {
+        const int i = (int)(e % N);
+        const cd v = kx[i].x * Sf[e];
+        Ez[e] = g.tm_grating ? v : v * eps_scale;
+    }
+    __syncthreads();
+    if (g.tm_grating) {
+        // [[eps]] of the layer, LU-factored as in the TM assembly (rcwa1d.hpp:455-462)
+        cd* E = Ez + 3 * NC;
+        int* piv = reinterpret_cast<int*>(E + (int64_t)N * N);
+        cd* coef = E + (int64_t)N * N + ((N + 15) / 16) * 16;
+        cd* F = coef + ((4 * M + 1 + 15) / 16) * 16;
+        LayerCursor cur(st);
+        LayerView lv = cur.next();
+        for (int k = 0; k < g.region; ++k) lv = cur.next();
+        layer_coefficients(lv, st[0], M, false, coef, F);
+        toeplitz(coef, N, E);
+        wg_lu(N, E, N, piv);
+        wg_lu_solve(N, E, N, piv, Ez, N, cnt);
+    }
+    for (int64_t e = threadIdx.x; e < NC; e += NT) {
+        const int i = (int)(e % N), iz = cols[g.first + (int)(e / N)];
+        const cd U = Uf[e];
+        out[0 * NZ + i + (int64_t)iz * N] = cd{U.y, -U.x};
+        out[1 * NZ + i + (int64_t)iz * N] = zero;
+        out[2 * NZ + i + (int64_t)iz * N] = Ez[e];
+        out[3 * NZ + i + (int64_t)iz * N] = zero;
+        out[4 * NZ + i + (int64_t)iz * N] = Sf[e];
+        out[5 * NZ + i + (int64_t)iz * N] = zero;
+    }
+}
+
+// A harmonic field set on an x grid (reconstruct_field_grid, rcwa_fields.hpp:326-355): component c at
+// (x_ix, z_iz) = sum_j exp(-i kxm_j x_ix) H[c][j + iz N], x_ix = ix period / nx; out[c][ix + iz nx].
+__global__ __launch_bounds__(NT) void k_fieldmap1d(const Inst1D* __restrict__ inst, const double* __restrict__ stacks,
+                                                   const int64_t* __restrict__ layouts, const cd* __restrict__ ws,
+                                                   const cd* __restrict__ H, int nz, int nx, cd* __restrict__ out) {
+    const Inst1D in = *inst;
+    const int N = in.N;
+    const int64_t cells = (int64_t)nx * nz;
+    const int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x;
+    if (e >= 6 * cells) return;
+    const int c = (int)(e / cells);
+    const int64_t k = e % cells;
+    const int ix = (int)(k % nx), iz = (int)(k / nx);
+    const double period = stacks[in.stack_off];
+    const double k0 = 2.0 * kPi / in.wavelength;
+    const cd* kx = ws + layouts[in.layout_off + LAYOUT_KX];
+    const double x = (static_cast<double>(ix) / nx) * period;
+    const cd* h = H + (int64_t)c * N * nz + (int64_t)iz * N;
+    cd s{0.0, 0.0};
+    for (int j = 0; j < N; ++j) {
+        double sn, cs;
+        sincos((k0 * kx[j].x) * x, &sn, &cs);
+        s = s + cd{cs, -sn} * h[j];
+    }
+    out[e] = s;
+}
+
+void launch_fields1d(hipStream_t s, int n_groups, const Inst1D* inst, const double* stacks, const int64_t* layouts,
+                     const cd* ws, const cd* fields, const cd* R, const cd* T, const FieldGroup1D* groups,
+                     const int* cols, const double* zs, int nz, cd* out, cd* scratch) {
+    k_fields1d<<<n_groups, NT, 0, s>>>(inst, stacks, layouts, ws, fields, R, T, groups, cols, zs, nz, out, scratch);
+}
+void launch_fieldmap1d(hipStream_t s, const Inst1D* inst, const double* stacks, const int64_t* layouts, const cd* ws,
+                       const cd* H, int nz, int nx, cd* out) {
+    const int64_t total = 6 * (int64_t)nx * nz;
+    k_fieldmap1d<<<(unsigned)((total + NT - 1) / NT), NT, 0, s>>>(inst, stacks, layouts, ws, H, nz, nx, out);
+}
+
 // host launchers
 void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                    const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
@@ -492,7 +703,8 @@ void launch_planar(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, 
     if (part & 2) {
         if (ev) hipEventRecord(ev[3], s);
         if (n_tasks > 0) k_modes1d<<<n_tasks, NT, 0, s>>>(insts, tasks, layouts, ws, task_info, ops.descs, ops.ops);
-        smatrix1d_launch(n_inst, s, insts, stacks, layouts, ws, task_info, out);
+        if (out.fields) smatrix1d_fields_launch(n_inst, s, insts, stacks, layouts, ws, task_info, out);
+        else smatrix1d_launch(n_inst, s, insts, stacks, layouts, ws, task_info, out);
         if (ev) hipEventRecord(ev[4], s);
     }
 }

@@ -739,11 +739,19 @@ DEV DOp op_keep(SmatCtx& c, DOp a, cd* slot) {
 // region(r) returns the modes of region r (0 incidence, j+1 layer j, L+1 exit).  Writes the
 // reflected (up) and transmitted (down) amplitudes, N x nrhs each.  On return c.fail says whether
 // the instance failed.
+//
+// Near-field capture (planar, nrhs = 1; rcwa1d.hpp:968-1023): with fields != null the reflection below
+// each layer is kept as the sweep passes it, and the downward sweep writes each layer's amplitudes,
+// a+ at the layer top (fields + j N) and a- = r_below (the amplitudes at the layer bottom) (fields +
+// (L + j) N).  pure: every layer is a uniform film; then only order keep_order of these vectors is kept
+// (the others are not excited), as the reference clears them.
 template <class RegionFn>
 DEV void run_recursion(SmatCtx& c, int L, double k0, const double* thick, RegionFn region, int nrhs, const int* cols,
-                       cd* up_out, cd* down_out) {
+                       cd* up_out, cd* down_out, cd* fields = nullptr, bool pure = false, int keep_order = 0) {
     __shared__ int g_kind[SMAT_MAX_BLOCKS];
     __shared__ cd* g_ptr[SMAT_MAX_BLOCKS];
+    __shared__ int rb_kind[SMAT_MAX_LAYERS];
+    __shared__ cd* rb_ptr[SMAT_MAX_LAYERS];
     const int N = c.N;
     const int64_t NN = (int64_t)N * N;
     const int nb = 2 * L + 1;
@@ -764,6 +772,17 @@ DEV void run_recursion(SmatCtx& c, int L, double k0, const double* thick, Region
         }
         c.check();
         if (c.fail) break;
+        if (fields && (b & 1)) {
+            // the reflection below layer j (rcwa1d.hpp:978), kept for its a- (slots from the kept end)
+            DOp kept = r;
+            if (r.kind != OP_ZERO) kept = op_keep(c, r, c.bump.keep(r.kind == OP_DIAG ? N : NN));
+            c.check();
+            if (c.fail) break;
+            if (threadIdx.x == 0) {
+                rb_kind[j] = kept.kind;
+                rb_ptr[j] = kept.p;
+            }
+        }
         DOp g, rn;
         stack_on(c, blk, r, g, rn);
         if (c.fail) break;
@@ -851,11 +870,30 @@ DEV void run_recursion(SmatCtx& c, int L, double k0, const double* thick, Region
     op_apply_cols(c, Tuu, tmpv, up_out, nrhs);
     for (int64_t e = threadIdx.x; e < NR; e += NT) up_out[e] = rud[e] + up_out[e];
     __syncthreads();
+    cd* bottom = fields ? c.bump.tmp(N) : nullptr;
+    c.check();
+    if (c.fail) return;
+    // pure films: only the incident order is excited (rcwa1d.hpp:1011-1020)
+    auto excited = [&](int i) { return !pure || i == keep_order; };
     for (int b = 1; b < nb; ++b) {
+        const int j = b / 2;
+        if (fields && (b & 1))
+            for (int i = threadIdx.x; i < N; i += NT) fields[(int64_t)j * N + i] = excited(i) ? down[i] : cd{0.0, 0.0};
         op_apply_cols(c, DOp{g_kind[b], g_ptr[b]}, down, tmpv, nrhs);
         cd* sw = down;
         down = tmpv;
         tmpv = sw;
+        if (fields && (b & 1)) {
+            for (int i = threadIdx.x; i < N; i += NT) bottom[i] = excited(i) ? down[i] : cd{0.0, 0.0};
+            __syncthreads();
+            cd* am = fields + (int64_t)(L + j) * N;
+            op_apply_cols(c, DOp{rb_kind[j], rb_ptr[j]}, bottom, am, 1);
+            if (pure) {
+                for (int i = threadIdx.x; i < N; i += NT)
+                    if (!excited(i)) am[i] = cd{0.0, 0.0};
+                __syncthreads();
+            }
+        }
     }
     for (int64_t e = threadIdx.x; e < NR; e += NT) down_out[e] = down[e];
     __syncthreads();

@@ -17,6 +17,8 @@
 
 #include "../../include/srcwa.h"
 #include "batch.hpp"
+#include <numeric>
+
 #include "context.hpp"
 #include "tuned.hpp"
 #include "cplx.hpp"
@@ -31,6 +33,11 @@ void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts,
 void launch_crossed(hipStream_t s, int n_inst, int n_tasks, const Inst2D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
                     int part, int flags, hipEvent_t* ev, const int* eig_order, const OpsArgs& ops);
+void launch_fields1d(hipStream_t s, int n_groups, const Inst1D* inst, const double* stacks, const int64_t* layouts,
+                     const cd* ws, const cd* fields, const cd* R, const cd* T, const FieldGroup1D* groups,
+                     const int* cols, const double* zs, int nz, cd* out, cd* scratch);
+void launch_fieldmap1d(hipStream_t s, const Inst1D* inst, const double* stacks, const int64_t* layouts, const cd* ws,
+                       const cd* H, int nz, int nx, cd* out);
 void launch_zgemm_batched(hipStream_t s, int count, const DenseItem* items, cd* arena);
 void launch_zgesv_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info);
 #if SRCWA_MIXED_PRECISION
@@ -501,6 +508,14 @@ struct srcwa_plan {
     DevBuf<double> diag;            // DIAG_WORDS per instance
     DevBuf<TaskDiag> task_diag;     // per layer task
     DevBuf<double> item_res;        // conical: two eigen residuals per layer task
+    // planar plans with PLAN_FIELDS: the layer amplitudes a+, a- (2 L N complex per problem), a per-problem
+    // flag that the capture is whole, and host copies of what the samplers need
+    DevBuf<cd> fields;
+    DevBuf<int> field_ok;
+    std::vector<Inst1D> h_insts;
+    std::vector<std::vector<LayerInfo>> h_layers;   // per problem
+    std::vector<double> h_stack_words;
+    bool executed = false;
     int flags = 0;                  // SRCWA_PLAN_* flags (from the context at creation)
     // 5 timing events per chunk: front start, after assembly, after eigensolves (front stream); back
     // start, back end (back stream); then the final event on the front stream
@@ -549,6 +564,8 @@ struct srcwa_plan {
         diag.pool = p;
         task_diag.pool = p;
         item_res.pool = p;
+        fields.pool = p;
+        field_ok.pool = p;
     }
     void alloc_diag() {
         diag.alloc(size_t(DIAG_WORDS) * n_inst);
@@ -618,6 +635,10 @@ void enqueue(srcwa_plan* plan, bool timing) {
         Out1D out{plan->R.p, plan->T.p, plan->DEr.p, plan->DEt.p, plan->status.p + ch.i0, plan->eig_info.p + ch.i0,
                   plan->kind == 1 ? nullptr : plan->geometry.p, plan->diag.p + int64_t(DIAG_WORDS) * ch.i0,
                   plan->task_diag.p + ch.t0, plan->item_res.p + 2 * int64_t(ch.t0)};
+        if (plan->kind == 1 && (plan->flags & PLAN_FIELDS)) {
+            out.fields = reinterpret_cast<double*>(plan->fields.p);
+            out.field_ok = plan->field_ok.p + ch.i0;
+        }
         cd* ws = plan->ws.p + (nc > 1 ? static_cast<int64_t>(c % 2) * plan->region_words : 0);
         hipEvent_t* ev = timing ? plan->ev.data() + 5 * c : nullptr;
         const int ni = ch.i1 - ch.i0, nt = ch.t1 - ch.t0;
@@ -727,7 +748,7 @@ int srcwa_pool_destroy(srcwa_pool* pool) {
 int srcwa_context_set_plan_flags(srcwa_context* ctx, int flags) {
     return guarded([&] {
         if (!ctx) invalid("srcwa_context_set_plan_flags: null context");
-        if (flags & ~int(SRCWA_PLAN_DIAGNOSTICS | SRCWA_PLAN_MIXED | SRCWA_PLAN_POINT_OPS))
+        if (flags & ~int(SRCWA_PLAN_DIAGNOSTICS | SRCWA_PLAN_MIXED | SRCWA_PLAN_POINT_OPS | SRCWA_PLAN_FIELDS))
             invalid("srcwa_context_set_plan_flags: unknown flag");
         if ((flags & SRCWA_PLAN_MIXED) && !SRCWA_MIXED_PRECISION) throw Failure{SRCWA_EUNSUPPORTED, kMixedOff};
         ctx->plan_flags = flags;
@@ -768,8 +789,10 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         plan->n_inst = n_problems;
         plan->flags = ctx->plan_flags;
         const bool diag = plan->flags & PLAN_DIAGNOSTICS;
+        const bool fields = plan->flags & PLAN_FIELDS;
         const int64_t budget = max_workspace_bytes > 0 ? max_workspace_bytes : (int64_t(64) << 30);
         const int64_t budget_words = budget / int64_t(sizeof(cd));
+        int64_t field_words = 0;
 
         std::vector<Inst1D> h_insts(n_problems);
         std::vector<LayerTask> h_tasks;
@@ -814,6 +837,8 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
                 }
             const int64_t ws_lo = w;
             w += cost.bump_words + (diag ? 3 * NN1 + 64 : 0) + ((plan->flags & PLAN_MIXED) ? mixed_solve_words(N, N) : 0);
+            // near fields: the reflection below every layer is kept, and one more vector
+            if (fields) w += int64_t(L) * align16(NN1) + align16(N) + 64;
             // admission into the current chunk by workspace footprint
             if (i > cur.i0 && chunk_ws + w > budget_words) {
                 cur.i1 = i;
@@ -853,6 +878,11 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
             in.out_off = orders;
             in.ws_lo = chunk_ws + ws_lo;
             in.ws_hi = chunk_ws + w;
+            in.field_off = field_words;
+            if (fields) {
+                field_words += 2 * int64_t(L) * N;
+                plan->h_layers.push_back(li);
+            }
             chunk_ws += w;
             orders += N;
         }
@@ -860,6 +890,9 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         cur.t1 = static_cast<int>(h_tasks.size());
         plan->chunks.push_back(cur);
         max_ws = std::max(max_ws, chunk_ws);
+        // the samplers read the modes where the solve left them, so a field plan runs as one chunk
+        if (fields && plan->chunks.size() > 1)
+            invalid("rcwa: a plan with near fields must fit one workspace chunk (raise max_workspace_bytes)");
         plan->n_tasks = static_cast<int>(h_tasks.size());
         plan->total_orders = orders;
         std::vector<LayerOps> h_ops;
@@ -871,11 +904,18 @@ static int plan1d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
                     b(h_stacks.size() * sizeof(double)) + b(h_layouts.size() * sizeof(int64_t)) +
                     b((plan->chunks.size() > 1 ? 2 : 1) * max_ws * sizeof(cd)) + b(std::max(1, plan->n_tasks) * sizeof(int)) +
                     2 * b(2 * orders * sizeof(double)) + 2 * b(orders * sizeof(double)) + 2 * b(n_problems * sizeof(int)) +
-                    diag_bytes(n_problems, plan->n_tasks) + ops_bytes(h_ops, op_words);
+                    diag_bytes(n_problems, plan->n_tasks) + ops_bytes(h_ops, op_words) +
+                    (fields ? b(std::max<int64_t>(1, field_words) * sizeof(cd)) + b(n_problems * sizeof(int)) : 0);
             return;
         }
         hipStream_t st = ctx->stream;
         plan->insts.upload(h_insts, st);
+        if (fields) {
+            plan->fields.alloc(static_cast<size_t>(std::max<int64_t>(1, field_words)));
+            plan->field_ok.alloc(n_problems);
+            plan->h_insts = h_insts;
+            plan->h_stack_words = h_stacks;
+        }
         plan->tasks.upload(h_tasks, st);
         plan->upload_ops(h_ops, op_words, st);
         plan->stacks.upload(h_stacks, st);
@@ -932,6 +972,8 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
         plan->kind = 2;
         plan->n_inst = n_problems;
         plan->flags = ctx->plan_flags;
+        if (plan->flags & PLAN_FIELDS)
+            throw Failure{SRCWA_EUNSUPPORTED, "rcwa: near fields are captured for planar (TE/TM) plans only in this build"};
         const bool diag = plan->flags & PLAN_DIAGNOSTICS;
         const int64_t budget = max_workspace_bytes > 0 ? max_workspace_bytes : (int64_t(64) << 30);
         const int64_t budget_words = budget / int64_t(sizeof(cd));
@@ -1116,6 +1158,7 @@ int srcwa_plan_execute(srcwa_plan* plan) {
         }
         hip_check(hipEventRecord(plan->ev.back(), front), "hipEventRecord");
         plan->timed = true;
+        plan->executed = true;
     });
 }
 
@@ -1203,6 +1246,161 @@ int srcwa_plan_results(srcwa_plan* plan, double* R, double* T, double* DEr, doub
         if (status) copy_out(status, plan->status.p, plan->n_inst * sizeof(int), hipMemcpyDeviceToHost, "D2H", plan);
         if (eig_info)
             copy_out(eig_info, plan->eig_info.p, plan->n_inst * sizeof(int), hipMemcpyDeviceToHost, "D2H", plan);
+    });
+}
+
+// ---- near fields of planar plans -----------------------------------------------------------------
+namespace {
+// The planar capture of one problem is usable: the plan keeps fields, has run, the solve succeeded and
+// every captured mode and amplitude is finite (rcwa1d.hpp:1057-1066).
+bool field_ready(srcwa_plan* plan, int problem) {
+    int st = 1, ok = 0;
+    copy_out(&st, plan->status.p + problem, sizeof(int), hipMemcpyDeviceToHost, "D2H", plan);
+    copy_out(&ok, plan->field_ok.p + problem, sizeof(int), hipMemcpyDeviceToHost, "D2H", plan);
+    return st == ST_OK && ok == 1;
+}
+
+void field_plan_checks(srcwa_plan* plan, int problem, const char* what) {
+    if (!plan) invalid(std::string(what) + ": null plan");
+    if (plan->kind != 1 || !(plan->flags & PLAN_FIELDS))
+        invalid(std::string(what) + ": the plan keeps no near fields (create it with SRCWA_PLAN_FIELDS)");
+    if (problem < 0 || problem >= plan->n_inst) invalid(std::string(what) + ": problem index out of range");
+    if (!plan->executed) invalid(std::string(what) + ": the plan has not been executed");
+    hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
+    for (hipStream_t st : {plan->ctx->stream, plan->ctx->back})
+        if (st) hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+// Harmonic fields of problem `problem` at nz depths into the device buffer H (6 x N x nz complex), with the
+// reference's depth groups (field_depth_groups, rcwa_fields.hpp:253-279): z < 0 the incidence side,
+// z > the stack bottom (or any z with no layer of positive thickness) the exit side, else the first layer
+// whose bottom is at or below z + 1e-12.
+void sample_fields(srcwa_plan* plan, int problem, int nz, const double* zs, DevBuf<cd>& H) {
+    const Inst1D& in = plan->h_insts[problem];
+    const std::vector<LayerInfo>& li = plan->h_layers[problem];
+    const int L = in.L, N = in.N;
+    for (int iz = 0; iz < nz; ++iz)
+        if (!std::isfinite(zs[iz])) invalid("rcwa: field depths must be finite");
+    std::vector<double> z_top(L), z_bot(L);
+    double z = 0.0;
+    bool finite_layer = false;
+    for (int j = 0; j < L; ++j) {
+        z_top[j] = z;
+        z += li[j].d;
+        z_bot[j] = z;
+        finite_layer = finite_layer || li[j].d > 0.0;
+    }
+    const double z_bottom = z;
+    std::vector<std::vector<int>> members(L + 2);   // 0 incidence, 1 exit, 2 + j layer j
+    for (int iz = 0; iz < nz; ++iz) {
+        const double zi = zs[iz];
+        int grp = zi < 0.0 ? 0 : 1;
+        if (zi >= 0.0 && zi <= z_bottom && finite_layer)
+            for (int j = 0; j < L; ++j)
+                if (zi <= z_bot[j] + 1e-12) {
+                    grp = 2 + j;
+                    break;
+                }
+        members[grp].push_back(iz);
+    }
+    std::vector<FieldGroup1D> groups;
+    std::vector<int> cols;
+    int64_t scr = 0;
+    const int64_t NN = int64_t(N) * N;
+    for (int gi = 0; gi < L + 2; ++gi) {
+        if (members[gi].empty()) continue;
+        FieldGroup1D g{};
+        g.region = gi == 0 ? -1 : gi == 1 ? -2 : gi - 2;
+        g.first = static_cast<int>(cols.size());
+        g.count = static_cast<int>(members[gi].size());
+        g.tm_grating = in.pol == 1 && g.region >= 0 && li[g.region].grating;
+        g.z_top = g.region >= 0 ? z_top[g.region] : 0.0;
+        g.d = g.region >= 0 ? li[g.region].d : 0.0;
+        g.z_bottom = z_bottom;
+        g.scratch = scr;
+        const int64_t nc = int64_t(N) * g.count;
+        scr += 5 * align16(nc);
+        if (g.tm_grating) scr += align16(NN) + align16(N) + align16(4 * in.M + 1) + fft_scratch_words(li[g.region], in.M) + 16;
+        cols.insert(cols.end(), members[gi].begin(), members[gi].end());
+        groups.push_back(g);
+    }
+    hipStream_t st = plan->ctx->stream;
+    H.alloc(static_cast<size_t>(6 * int64_t(N) * nz));
+    DevBuf<FieldGroup1D> d_groups;
+    DevBuf<int> d_cols;
+    DevBuf<double> d_zs;
+    DevBuf<cd> d_scr;
+    const std::vector<double> h_zs(zs, zs + nz);     // lives until the launch below has been synchronized
+    d_groups.upload(groups, st);
+    d_cols.upload(cols, st);
+    d_zs.upload(h_zs, st);
+    d_scr.alloc(static_cast<size_t>(std::max<int64_t>(16, scr)));
+    const int64_t oo = std::accumulate(plan->h_insts.begin(), plan->h_insts.begin() + problem, int64_t(0),
+                                       [](int64_t a, const Inst1D& x) { return a + x.N; });
+    launch_fields1d(st, static_cast<int>(groups.size()), plan->insts.p + problem, plan->stacks.p, plan->layouts.p,
+                    plan->ws.p, plan->fields.p, reinterpret_cast<const cd*>(plan->R.p) + oo,
+                    reinterpret_cast<const cd*>(plan->T.p) + oo, d_groups.p, d_cols.p, d_zs.p, nz, H.p, d_scr.p);
+    hip_check(hipGetLastError(), "kernel launch");
+    hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+bool all_finite(const std::vector<double>& v) {
+    for (double x : v)
+        if (!std::isfinite(x)) return false;
+    return true;
+}
+}  // namespace
+
+int srcwa_plan1d_field_available(srcwa_plan* plan, int problem, int* available) {
+    return guarded([&] {
+        field_plan_checks(plan, problem, "srcwa_plan1d_field_available");
+        if (!available) invalid("srcwa_plan1d_field_available: null output");
+        *available = field_ready(plan, problem) ? 1 : 0;
+    });
+}
+
+int srcwa_plan1d_field_at(srcwa_plan* plan, int problem, int nz, const double* zs, double* out) {
+    return guarded([&] {
+        field_plan_checks(plan, problem, "srcwa_plan1d_field_at");
+        if (!field_ready(plan, problem)) throw Failure{SRCWA_EUNAVAILABLE, "rcwa: planar field context is unavailable"};
+        if (nz < 0 || (nz > 0 && (!zs || !out))) invalid("srcwa_plan1d_field_at: bad depth arguments");
+        if (nz == 0) return;
+        DevBuf<cd> H;
+        H.pool = plan->ctx->pool;
+        sample_fields(plan, problem, nz, zs, H);
+        const size_t words = 12 * size_t(plan->h_insts[problem].N) * nz;
+        std::vector<double> h(words);
+        copy_out(h.data(), H.p, words * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
+        if (!all_finite(h)) throw Failure{SRCWA_ENUMERICAL, "rcwa: planar field propagation produced nonfinite values"};
+        std::memcpy(out, h.data(), words * sizeof(double));
+    });
+}
+
+int srcwa_plan1d_field_map(srcwa_plan* plan, int problem, int nz, const double* zs, int nx, double y, double* out) {
+    return guarded([&] {
+        if (nx <= 0) invalid("rcwa: field nx must be positive");
+        if (!std::isfinite(y)) invalid("rcwa: field y must be finite");
+        field_plan_checks(plan, problem, "srcwa_plan1d_field_map");
+        if (!field_ready(plan, problem)) throw Failure{SRCWA_EUNAVAILABLE, "rcwa: planar field context is unavailable"};
+        if (nz < 0 || (nz > 0 && (!zs || !out))) invalid("srcwa_plan1d_field_map: bad depth arguments");
+        if (nz == 0) return;
+        DevBuf<cd> H;
+        H.pool = plan->ctx->pool;
+        sample_fields(plan, problem, nz, zs, H);
+        const int N = plan->h_insts[problem].N;
+        std::vector<double> h(12 * size_t(N) * nz);
+        copy_out(h.data(), H.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
+        if (!all_finite(h)) throw Failure{SRCWA_ENUMERICAL, "rcwa: planar field propagation produced nonfinite values"};
+        DevBuf<cd> G;
+        G.pool = plan->ctx->pool;
+        G.alloc(static_cast<size_t>(6 * int64_t(nx) * nz));
+        launch_fieldmap1d(plan->ctx->stream, plan->insts.p + problem, plan->stacks.p, plan->layouts.p, plan->ws.p, H.p,
+                          nz, nx, G.p);
+        hip_check(hipGetLastError(), "kernel launch");
+        std::vector<double> g(12 * size_t(nx) * nz);
+        copy_out(g.data(), G.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
+        if (!all_finite(g)) throw Failure{SRCWA_ENUMERICAL, "rcwa: field reconstruction produced nonfinite values"};
+        std::memcpy(out, g.data(), g.size() * sizeof(double));
     });
 }
 
@@ -1693,6 +1891,8 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         plan->kind = 3;
         plan->n_inst = n_problems;
         plan->flags = ctx->plan_flags;
+        if (plan->flags & PLAN_FIELDS)
+            throw Failure{SRCWA_EUNSUPPORTED, "rcwa: near fields are captured for planar (TE/TM) plans only in this build"};
         const bool diag = plan->flags & PLAN_DIAGNOSTICS;
         const int64_t budget = max_workspace_bytes > 0 ? max_workspace_bytes : (int64_t(64) << 30);
         const int64_t budget_words = budget / int64_t(sizeof(cd));

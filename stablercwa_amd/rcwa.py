@@ -114,6 +114,31 @@ class PlanarBatch:
         _lib.check(self.lib.srcwa_plan_refinement(self.handle, _lib.ptr(out)))
         return out
 
+    # ---- near fields (a context created with plan_flags=PLAN_FIELDS; rcwa1d_fields.hpp:55-206) ----
+    def field_available(self, i: int) -> bool:
+        """PlanarFieldContext::available() of problem i after execute."""
+        v = ctypes.c_int(0)
+        _lib.check(self.lib.srcwa_plan1d_field_available(self.handle, int(i), ctypes.byref(v)))
+        return bool(v.value)
+
+    def field_at(self, i: int, zs) -> np.ndarray:
+        """Harmonic fields of problem i at the depths zs: array [6, N, nz] complex (Ex, Ey, Ez, Hx, Hy, Hz; H is
+        Z0 H), as HarmonicFields::components (rcwa_fields.hpp:108-128)."""
+        z = np.ascontiguousarray(np.atleast_1d(np.asarray(zs, dtype=np.float64)))
+        N = int(self.N[i])
+        out = np.zeros(12 * N * z.size)
+        _lib.check(self.lib.srcwa_plan1d_field_at(self.handle, int(i), int(z.size), _lib.ptr(z), _lib.ptr(out)))
+        return out.view(np.complex128).reshape(6, z.size, N).transpose(0, 2, 1)
+
+    def field_map(self, i: int, zs, nx: int = 256, y: float = 0.0) -> np.ndarray:
+        """The fields of problem i on the grid x_k = k period / nx (k < nx) at the depths zs: array [6, nz, nx]
+        complex, as FieldGridXZ::components (data()[iz nx + ix], rcwa_fields.hpp:130-157)."""
+        z = np.ascontiguousarray(np.atleast_1d(np.asarray(zs, dtype=np.float64)))
+        out = np.zeros(12 * max(int(nx), 0) * z.size)
+        _lib.check(self.lib.srcwa_plan1d_field_map(self.handle, int(i), int(z.size), _lib.ptr(z), int(nx), float(y),
+                                                   _lib.ptr(out)))
+        return out.view(np.complex128).reshape(6, z.size, max(int(nx), 0))
+
     def stage_ms(self) -> np.ndarray:
         ms = np.zeros(4)
         _lib.check(self.lib.srcwa_plan_stage_ms(self.handle, _lib.ptr(ms)))

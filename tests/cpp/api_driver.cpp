@@ -7,6 +7,8 @@
 //   planar_diag <pol> <M> <wavelength> <theta> <n> <flat record>   (Options::diagnostics = true)
 //   conical <M> <wavelength> <theta> <phi> <n> <flat record>
 //   crossed <M1> <M2> <wavelength> <theta> <phi> <Es.re> <Es.im> <Ep.re> <Ep.im> <laurent> <n> <flat record>
+//   fields  <pol> <M> <wavelength> <theta> <nz> <z>* <nx> <y> <n> <flat record>   (solve_with_fields, field_at,
+//           field_map; prints the 6 harmonic components order-major per depth, then the 6 grid components)
 //
 // Output per problem: "ok" and labelled arrays, or "invalid_argument <message>" / "unsupported <message>".
 #include <cstdio>
@@ -15,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "stablercwa_amd/fields.hpp"
 #include "stablercwa_amd/rcwa.hpp"
 
 namespace rcwa = srcwa_amd::rcwa;
@@ -189,6 +192,38 @@ int main() {
                 put("Tp", r.Tp);
                 put("Kx", r.Kx);
                 put("Ky", r.Ky);
+            } else if (kind == "fields") {
+                int pol, M, nz, nx;
+                double wl, th, y;
+                in >> pol >> M >> wl >> th >> nz;
+                srcwa_amd::RealVector zs(nz);
+                for (double& z : zs) in >> z;
+                in >> nx >> y;
+                const auto st = planar_stack(read_record(in));
+                const auto res = rcwa::solve_with_fields(st, wl, th, M, pol ? rcwa::Polarization::TM : rcwa::Polarization::TE);
+                std::printf("ok %d %zu %d\n", res.result.eig_info, res.result.DEr.size(), res.fields.available() ? 1 : 0);
+                put("DEr", res.result.DEr);
+                if (res.fields.available()) {
+                    const rcwa::PlanarFieldContext copy = res.fields;       // copies share the capture
+                    const auto h = rcwa::field_at(copy, zs);
+                    const auto g = rcwa::field_map(res, zs, nx, y);
+                    std::vector<cplx> hv, gv;
+                    for (int c = 0; c < 6; ++c) {
+                        hv.insert(hv.end(), h.components[c].data(), h.components[c].data() + h.components[c].rows() * h.components[c].cols());
+                        gv.insert(gv.end(), g.components[c].data(), g.components[c].data() + g.components[c].rows() * g.components[c].cols());
+                    }
+                    put("harm", hv);
+                    put("grid", gv);
+                    put("kzI", h.metadata.kzI);
+                    std::printf("z_bottom %.17g\n", h.metadata.z_bottom);
+                } else {
+                    try {
+                        (void)rcwa::field_at(res.fields, zs);
+                        std::printf("field_at did not throw\n");
+                    } catch (const std::logic_error& e) {
+                        std::printf("logic_error %s\n", e.what());
+                    }
+                }
             } else {
                 std::printf("invalid_argument unknown problem kind\n");
             }

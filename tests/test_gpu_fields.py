@@ -1,0 +1,163 @@
+"""GPU: planar near fields (SRCWA_PLAN_FIELDS; srcwa_plan1d_field_at / _field_map) against the reference's own
+rcwa::solve_with_fields + field_at + field_map compiled from its headers (oracle/_ref), whose fields are pinned
+by flux conservation and tangential continuity in tests/test_oracle_fields.py.
+
+* harmonic fields (6 components x N orders x depths) and the x-z grid within 1e-9 of the reference (relative to
+  max(1, |field|)) for TE and TM: the reference's field-test stack (two gratings around an absorbing film,
+  tests/test_rcwa_public_fields.cpp:28-33) at its depths (in both outer media, inside each layer, on and just
+  beside every interface, repeated depths), a grating on a 20-film mirror at M = 20, a pure film stack, and an
+  FFT-sampled grating (TM: the [[eps]] solve of Ez)
+* TE leaves Ex, Ez, Hy and TM leaves Ey, Hx, Hz exactly zero (every component of the output is written)
+* depths keep their order and repeats; one problem of a multi-problem plan samples the same as alone
+* flux: the GPU fields carry the same z-flux at every depth of a lossless stack
+* the failure contract: a solve that fails (exit-side grazing) leaves the context unavailable; non-finite depths,
+  nx <= 0 and a non-finite y are invalid arguments (rcwa_fields.hpp:253-292)
+"""
+import numpy as np
+import pytest
+
+from test_oracle_fields import ZS, flux, lossless_stack, lossy_stack
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-9
+
+
+@pytest.fixture(scope="module")
+def fctx():
+    from stablercwa_amd import _lib
+    return _lib.Context(0, plan_flags=_lib.PLAN_FIELDS)
+
+
+def mirror_stack():
+    from stablercwa_amd.types import Layer, Stack
+    films = [Layer.uniform(1.053 / (4 * n), n) for _ in range(10) for n in (2.1, 1.46)]
+    return Stack(1.0, [Layer.grating(0.38, [0.0, 0.3 * 0.574, 0.574], [1.46, 1.0])] + films, 1.46, 0.574)
+
+
+def films_stack():
+    from stablercwa_amd.types import Layer, Stack
+    return Stack(1.0, [Layer.uniform(0.12, 2.1), Layer.uniform(0.2, 1.46 + 0.01j), Layer.uniform(0.07, 2.1)], 1.5, 0.6)
+
+
+def fft_stack():
+    from stablercwa_amd.types import FourierMethod, Layer, Stack
+    g = Layer.grating(0.25, [0.0, 0.2, 0.45, 0.6], [1.9, 1.0, 1.5 + 0.02j])
+    g.fourier_method = FourierMethod.FFT
+    return Stack(1.1, [g, Layer.uniform(0.1, 1.7)], 1.45, 0.6)
+
+
+def _plan(ctx, stack, problems):
+    from stablercwa_amd import rcwa
+    b = rcwa.PlanarBatch([stack], [(0,) + p for p in problems], ctx)
+    b.execute()
+    b.sync()
+    return b
+
+
+def _close(a, b, what):
+    a = np.asarray(a); b = np.asarray(b)
+    err = np.max(np.abs(a - b) / np.maximum(1.0, np.abs(b))) if b.size else 0.0
+    assert err <= TOL, (what, err)
+
+
+CASES = [
+    ("reference field stack", lossy_stack, .83, 27., 3, ZS),
+    ("grating on a mirror, M=20", mirror_stack, 1.053, 66.36, 20, np.linspace(-0.2, 1.6, 23)),
+    ("pure films", films_stack, .9, 35., 4, np.array([-0.1, 0.0, 0.06, 0.12, 0.25, 0.32, 0.39, 0.5])),
+    ("FFT grating", fft_stack, .8, 12., 6, np.array([-0.05, 0.1, 0.25, 0.3, 0.35, 0.6])),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("pol", [0, 1])
+def test_fields_match_reference(fctx, ref_lib, case, pol):
+    from oracle import ref
+    name, make, lam, theta, M, zs = case
+    st = make()
+    b = _plan(fctx, st, [(lam, theta, M, pol)])
+    assert b.field_available(0)
+    o = ref.planar_fields(ref_lib, st.flat(), lam, theta, M, pol, zs, nx=11, y=.11)
+    assert o["available"]
+    _close(b.result(0).DEr, o["DEr"], name + " DEr")
+    h = b.field_at(0, zs)
+    _close(h, o["harm"], name + " harmonics")
+    _close(b.field_map(0, zs, 11, .11), o["grid"], name + " grid")
+    # the polarization's other three components are exactly zero, as the reference's are
+    assert not np.any(h[[0, 2, 4] if pol == 0 else [1, 3, 5]])
+
+
+def test_order_repeats_and_batch_independence(fctx):
+    st = lossy_stack()
+    zs = np.array([.2, -.12, .2, 100., .083, .42, -100., .083])
+    alone = _plan(fctx, st, [(.83, 27., 3, 1)])
+    many = _plan(fctx, st, [(.9, 10., 5, 0), (.83, 27., 3, 1), (.7, 40., 2, 1)])
+    h = alone.field_at(0, zs)
+    assert np.array_equal(h, many.field_at(1, zs))
+    assert np.array_equal(h[:, :, 0], h[:, :, 2]) and np.array_equal(h[:, :, 4], h[:, :, 7])
+    one = np.stack([alone.field_at(0, z)[:, :, 0] for z in zs], axis=2)
+    assert np.array_equal(h, one)
+    assert alone.field_at(0, []).shape == (6, 7, 0)
+    assert alone.field_map(0, [], 5).shape == (6, 0, 5)
+
+
+@pytest.mark.parametrize("pol", [0, 1])
+def test_gpu_fields_conserve_flux(fctx, pol):
+    b = _plan(fctx, lossless_stack(), [(.83, 27., 3, pol)])
+    S = flux(b.field_at(0, ZS))
+    assert np.max(np.abs(S / S[0] - 1.0)) <= 1e-10
+
+
+def test_failed_solve_is_unavailable_and_arguments_are_checked(fctx):
+    from stablercwa_amd import _lib
+    from stablercwa_amd.types import Layer, Stack
+    p, f = 0.5747, 0.30
+    grazing = Stack(1.0, [Layer.grating(0.38, [-f * p / 2, f * p / 2, p - f * p / 2], [1.45, 1.0])], 2.0, p)
+    b = _plan(fctx, grazing, [(p, 0.0, 8, 0)])
+    assert not b.field_available(0)
+    with pytest.raises(_lib.FieldUnavailable):
+        b.field_at(0, [0.1])
+    with pytest.raises(_lib.FieldUnavailable):
+        b.field_at(0, [])
+    ok = _plan(fctx, lossy_stack(), [(.83, 27., 3, 0)])
+    for bad in ([np.inf], [0.0, np.nan]):
+        with pytest.raises(ValueError):
+            ok.field_at(0, bad)
+    with pytest.raises(ValueError):
+        ok.field_map(0, ZS, 0)
+    with pytest.raises(ValueError):
+        ok.field_map(0, ZS, -1)
+    with pytest.raises(ValueError):
+        ok.field_map(0, ZS, 3, np.inf)
+
+
+def test_plans_without_the_flag_keep_no_fields(ctx):
+    b = _plan(ctx, lossy_stack(), [(.83, 27., 3, 0)])
+    with pytest.raises(ValueError):
+        b.field_at(0, [0.1])
+
+
+def test_cpp_solve_with_fields_matches_reference(ref_lib):
+    """The C++ API with the reference's signatures (include/stablercwa_amd/fields.hpp) through the C++ driver."""
+    from oracle import ref
+    from test_gpu_cpp_api import _drive, _fields, _rec
+    st = lossy_stack()
+    zs = ZS
+    lines = [f"fields {pol} 3 0.83 27.0 {len(zs)} " + " ".join(repr(float(z)) for z in zs) + f" 11 0.11 {_rec(st.flat())}"
+             for pol in (0, 1)]
+    for pol, block in zip((0, 1), _drive(lines)):
+        f = _fields(block)
+        assert f["head"][0] == "ok" and f["head"][3] == "1", block[0]
+        o = ref.planar_fields(ref_lib, st.flat(), .83, 27., 3, pol, zs, nx=11, y=.11)
+        harm = f["harm"].view(np.complex128).reshape(6, len(zs), 7).transpose(0, 2, 1)
+        grid = f["grid"].view(np.complex128).reshape(6, len(zs), 11)
+        _close(harm, o["harm"], "C++ harmonics")
+        _close(grid, o["grid"], "C++ grid")
+        assert abs(f["z_bottom"][0] - 0.42) <= 1e-15
+    # a failed solve: the context is unavailable and field_at throws std::logic_error
+    from stablercwa_amd.types import Layer, Stack
+    p, fl = 0.5747, 0.30
+    grazing = Stack(1.0, [Layer.grating(0.38, [-fl * p / 2, fl * p / 2, p - fl * p / 2], [1.45, 1.0])], 2.0, p)
+    block, = _drive([f"fields 0 8 {p!r} 0.0 1 0.1 5 0.0 {_rec(grazing.flat())}"])
+    assert block[0].split()[3] == "0"
+    assert any(ln.startswith("logic_error rcwa: planar field context is unavailable") for ln in block)
