@@ -182,6 +182,43 @@ def solve(stack: Stack, wavelength: float, theta_deg: float, M: int, pol: Polari
     return r
 
 
+class PlanarFieldContext:
+    """The near-field capture of one planar solve (PlanarFieldContext, rcwa1d_fields.hpp:16-47), kept on the
+    device by its plan.  field_at / field_map raise FieldUnavailable when the solve failed."""
+
+    def __init__(self, batch: PlanarBatch, available: bool):
+        self._batch = batch
+        self._available = available
+
+    def available(self) -> bool:
+        return self._available
+
+    def field_at(self, zs) -> np.ndarray:
+        """[6, N, nz] complex harmonic fields (Ex, Ey, Ez, Hx, Hy, Hz; H is Z0 H) at the depths zs."""
+        return self._batch.field_at(0, zs)
+
+    def field_map(self, zs, nx: int = 256, y: float = 0.0) -> np.ndarray:
+        """[6, nz, nx] complex fields on the grid x_k = k period / nx at the depths zs."""
+        return self._batch.field_map(0, zs, nx, y)
+
+
+_field_contexts = {}
+
+
+def solve_with_fields(stack: Stack, wavelength: float, theta_deg: float, M: int,
+                      pol: Polarization = Polarization.TE, ctx: _lib.Context | None = None):
+    """One planar solve and its near-field capture (rcwa::solve_with_fields, rcwa1d_fields.hpp:55-104): returns
+    (Result, PlanarFieldContext).  ctx, when given, must have been created with plan_flags including PLAN_FIELDS."""
+    if ctx is None:
+        ctx = _field_contexts.get(0)
+        if ctx is None:
+            ctx = _field_contexts[0] = _lib.Context(0, plan_flags=_lib.PLAN_FIELDS)
+    b = PlanarBatch([stack], [(0, wavelength, theta_deg, M, pol)], ctx)
+    b.execute()
+    r = b.result(0)
+    return r, PlanarFieldContext(b, b.field_available(0))
+
+
 def sweep(stack: Stack, points: Sequence[Point], M: int, pol: Polarization = Polarization.TE,
           ctx: _lib.Context | None = None) -> List[Result]:
     """Independent solves over points, one batched launch (rcwa::sweep, rcwa1d.hpp:1084-1091)."""

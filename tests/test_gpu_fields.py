@@ -161,3 +161,34 @@ def test_cpp_solve_with_fields_matches_reference(ref_lib):
     block, = _drive([f"fields 0 8 {p!r} 0.0 1 0.1 5 0.0 {_rec(grazing.flat())}"])
     assert block[0].split()[3] == "0"
     assert any(ln.startswith("logic_error rcwa: planar field context is unavailable") for ln in block)
+
+
+def test_python_solve_with_fields(ref_lib):
+    from oracle import ref
+    from stablercwa_amd import _lib, rcwa
+    st = lossy_stack()
+    r, f = rcwa.solve_with_fields(st, .83, 27., 3, rcwa.Polarization.TM)
+    assert f.available()
+    o = ref.planar_fields(ref_lib, st.flat(), .83, 27., 3, 1, ZS, nx=9, y=0.0)
+    _close(r.DEt, o["DEt"], "DEt")
+    _close(f.field_at(ZS), o["harm"], "harmonics")
+    _close(f.field_map(ZS, 9), o["grid"], "grid")
+    from stablercwa_amd.types import Layer, Stack
+    p, fl = 0.5747, 0.30
+    grazing = Stack(1.0, [Layer.grating(0.38, [-fl * p / 2, fl * p / 2, p - fl * p / 2], [1.45, 1.0])], 2.0, p)
+    r2, f2 = rcwa.solve_with_fields(grazing, p, 0.0, 8)
+    assert not f2.available() and np.all(np.isnan(r2.DEr))
+    with pytest.raises(_lib.FieldUnavailable):
+        f2.field_at([0.0])
+
+
+def test_field_plans_must_fit_one_chunk_and_be_planar(fctx):
+    """The samplers read the modes where the solve left them: a field plan is refused when its workspace would
+    be split into chunks; conical plans capture no fields in this build."""
+    from stablercwa_amd import rcwa
+    st = mirror_stack()
+    probs = [(0, 1.0 + 0.01 * i, 66.36, 20, 0) for i in range(4)]
+    with pytest.raises(ValueError, match="one workspace chunk"):
+        rcwa.PlanarBatch([st], probs, fctx, max_workspace_bytes=1 << 20)
+    with pytest.raises(NotImplementedError):
+        rcwa.ConicalBatch([lossy_stack()], [(0, .83, 27., 31., 3)], fctx)
