@@ -5,8 +5,9 @@ by flux conservation and tangential continuity in tests/test_oracle_fields.py.
 * harmonic fields (6 components x N orders x depths) and the x-z grid within 1e-9 of the reference (relative to
   max(1, |field|)) for TE and TM: the reference's field-test stack (two gratings around an absorbing film,
   tests/test_rcwa_public_fields.cpp:28-33) at its depths (in both outer media, inside each layer, on and just
-  beside every interface, repeated depths), a grating on a 20-film mirror at M = 20, a pure film stack, and an
-  FFT-sampled grating (TM: the [[eps]] solve of Ez)
+  beside every interface, repeated depths), a grating on a 20-film mirror at M = 20 and M = 60 (through the
+  whole 3.44-deep stack), a pure film stack, an FFT-sampled grating (TM: the [[eps]] solve of Ez), and normal
+  incidence on a symmetric grating (degenerate modes)
 * TE leaves Ex, Ez, Hy and TM leaves Ey, Hx, Hz exactly zero (every component of the output is written)
 * depths keep their order and repeats; one problem of a multi-problem plan samples the same as alone
 * flux: the GPU fields carry the same z-flux at every depth of a lossless stack
@@ -66,6 +67,11 @@ CASES = [
     ("grating on a mirror, M=20", mirror_stack, 1.053, 66.36, 20, np.linspace(-0.2, 1.6, 23)),
     ("pure films", films_stack, .9, 35., 4, np.array([-0.1, 0.0, 0.06, 0.12, 0.25, 0.32, 0.39, 0.5])),
     ("FFT grating", fft_stack, .8, 12., 6, np.array([-0.05, 0.1, 0.25, 0.3, 0.35, 0.6])),
+    ("grating on a mirror, M=60", mirror_stack, 1.053, 66.36, 60, np.linspace(-0.3, 3.6, 40)),
+    # normal incidence on a symmetric grating: degenerate eigenvalues, so the eigenvector bases of the GPU and
+    # the reference differ inside each degenerate pair; the fields are physical and must still agree
+    ("normal incidence, degenerate modes", lossless_stack, .83, 0.0, 5,
+     np.array([-0.2, 0.0, 0.1, 0.2, 0.2 + 5e-13, 0.25, 0.29, 0.3, 0.42, 0.42 + 5e-13, 0.7])),
 ]
 
 
