@@ -9,6 +9,9 @@ Contents
 * ``conical.py``  -- numpy restatement of the conical solve (rcwa1d_conical.hpp).
 * ``rcwa2d.py``   -- numpy restatement of the crossed-grating solve (rcwa2d.hpp, rcwa2d_operators.hpp).
 * ``ref.py``      -- ctypes loader for ``oracle/_ref/libsrcwa_ref.so``, the reference's own header-only
-                     C++ built from its sources by ``oracle/Makefile`` (absent where the reference is absent).
+                     C++ built from its sources by ``oracle/Makefile`` (absent where the reference is absent),
+                     including its planar near fields (``planar_fields``: solve_with_fields, field_at, field_map),
+                     pinned by flux conservation, tangential continuity and a planted anchor swap in
+                     ``tests/test_oracle_fields.py``.
 * ``flat.py``     -- decoder of the flat stack records that cross the C-ABI.
 """
