@@ -1330,6 +1330,9 @@ void sample_fields(srcwa_plan* plan, int problem, int nz, const double* zs, DevB
     DevBuf<int> d_cols;
     DevBuf<double> d_zs;
     DevBuf<cd> d_scr;
+    d_groups.pool = d_cols.pool = plan->ctx->pool;      // a pooled context takes these from its pool too
+    d_zs.pool = plan->ctx->pool;
+    d_scr.pool = plan->ctx->pool;
     const std::vector<double> h_zs(zs, zs + nz);     // lives until the launch below has been synchronized
     d_groups.upload(groups, st);
     d_cols.upload(cols, st);

@@ -198,3 +198,29 @@ def test_field_plans_must_fit_one_chunk_and_be_planar(fctx):
         rcwa.PlanarBatch([st], probs, fctx, max_workspace_bytes=1 << 20)
     with pytest.raises(NotImplementedError):
         rcwa.ConicalBatch([lossy_stack()], [(0, .83, 27., 31., 3)], fctx)
+
+
+def test_fields_on_a_pooled_context(ref_lib):
+    """A context with a device pool: the field plan and the samplers' buffers come from the pool, and the pool
+    gets every block back once the plan is destroyed."""
+    import ctypes
+    from oracle import ref
+    from stablercwa_amd import _lib, rcwa
+    lib = _lib.load()
+    pool = ctypes.c_void_p()
+    _lib.check(lib.srcwa_pool_create(0, 256 << 20, ctypes.byref(pool)))
+    ctx = _lib.Context(0, plan_flags=_lib.PLAN_FIELDS)
+    try:
+        _lib.check(lib.srcwa_context_set_pool(ctx.handle, pool))
+        free0 = lib.srcwa_pool_free_bytes(pool)
+        st = lossy_stack()
+        b = _plan(ctx, st, [(.83, 27., 3, 1)])
+        o = ref.planar_fields(ref_lib, st.flat(), .83, 27., 3, 1, ZS, nx=7)
+        for _ in range(3):
+            _close(b.field_at(0, ZS), o["harm"], "pooled harmonics")
+            _close(b.field_map(0, ZS, 7), o["grid"], "pooled grid")
+        b.close()
+        assert lib.srcwa_pool_free_bytes(pool) == free0
+    finally:
+        ctx.close()
+        lib.srcwa_pool_destroy(pool)
