@@ -224,3 +224,21 @@ def test_fields_on_a_pooled_context(ref_lib):
     finally:
         ctx.close()
         lib.srcwa_pool_destroy(pool)
+
+
+@pytest.mark.parametrize("extra", ["diagnostics", "point_ops"])
+def test_fields_with_other_plan_flags(ref_lib, extra):
+    """Near fields combine with the other plan flags: diagnostics (per-task operators, kept layer matrices) and
+    per-point operators give the same fields as the reference."""
+    from oracle import ref
+    from stablercwa_amd import _lib
+    flag = _lib.PLAN_DIAGNOSTICS if extra == "diagnostics" else _lib.PLAN_POINT_OPS
+    ctx = _lib.Context(0, plan_flags=_lib.PLAN_FIELDS | flag)
+    st = lossy_stack()
+    for pol in (0, 1):
+        b = _plan(ctx, st, [(.83, 27., 3, pol), (.9, 10., 3, pol)])
+        for i, (lam, th) in enumerate([(.83, 27.), (.9, 10.)]):
+            o = ref.planar_fields(ref_lib, st.flat(), lam, th, 3, pol, ZS)
+            _close(b.field_at(i, ZS), o["harm"], f"{extra} pol {pol} point {i}")
+        b.close()
+    ctx.close()
