@@ -242,3 +242,24 @@ def test_fields_with_other_plan_flags(ref_lib, extra):
             _close(b.field_at(i, ZS), o["harm"], f"{extra} pol {pol} point {i}")
         b.close()
     ctx.close()
+
+
+def test_fields_edge_stacks(fctx, ref_lib):
+    """No layers at all (z = 0 belongs to the exit side), a zero-thickness layer (its interface is owned by the
+    layer above, rcwa_fields.hpp:253-279), and many depths in one group."""
+    from oracle import ref
+    from stablercwa_amd.types import Layer, Stack
+    bare = Stack(1.3, [], 1.6, 0.7)
+    zero = Stack(1.0, [Layer.grating(0.2, [0.0, 0.3, 0.6], [1.8, 1.2]), Layer.uniform(0.0, 2.2),
+                       Layer.grating(0.0, [0.0, 0.1, 0.6], [1.4, 2.0]), Layer.uniform(0.15, 1.5)], 1.45, 0.6)
+    cases = [(bare, np.array([-0.5, -1e-12, 0.0, 1e-12, 0.4])),
+             (zero, np.array([-0.1, 0.0, 0.1, 0.2, 0.2 + 1e-13, 0.25, 0.35, 0.5])),
+             (zero, np.linspace(0.2, 0.35, 3000))]
+    for st, zs in cases:
+        for pol in (0, 1):
+            b = _plan(fctx, st, [(.9, 20., 4, pol)])
+            o = ref.planar_fields(ref_lib, st.flat(), .9, 20., 4, pol, zs, nx=5, y=.3)
+            assert b.field_available(0) == o["available"] == True
+            _close(b.field_at(0, zs), o["harm"], f"edge harmonics L={len(st.layers)} nz={len(zs)}")
+            _close(b.field_map(0, zs, 5, .3), o["grid"], "edge grid")
+            b.close()
