@@ -82,7 +82,8 @@ int srcwa_context_set_pool(srcwa_context* ctx, srcwa_pool* pool);
  * SRCWA_PLAN_FIELDS: planar (TE/TM) plans keep each solve's near-field capture, as rcwa::solve_with_fields
  *   does (rcwa1d.hpp:480-489, 1006-1023; rcwa1d_fields.hpp:55-104): the layer amplitudes a+ (layer top) and
  *   a- (layer bottom), beside the modes and R, T the solve leaves on the device; srcwa_plan1d_field_at and
- *   srcwa_plan1d_field_map evaluate them.  Such a plan must fit one workspace chunk.  Conical and crossed
+ *   srcwa_plan1d_field_map evaluate them.  Conical plans keep theirs the same way (rcwa1d_conical.hpp:859-970;
+ *   srcwa_plan_conical_field_at / _field_map).  Such a plan must fit one workspace chunk.  Crossed-grating
  *   plans refuse the flag (SRCWA_EUNSUPPORTED). */
 enum { SRCWA_PLAN_DIAGNOSTICS = 1, SRCWA_PLAN_MIXED = 2, SRCWA_PLAN_POINT_OPS = 4, SRCWA_PLAN_FIELDS = 8 };
 int srcwa_context_set_plan_flags(srcwa_context* ctx, int flags);
@@ -205,6 +206,18 @@ int64_t srcwa_plan_orders(const srcwa_plan* plan);
 int srcwa_plan1d_field_available(srcwa_plan* plan, int problem, int* available);
 int srcwa_plan1d_field_at(srcwa_plan* plan, int problem, int nz, const double* zs, double* out);
 int srcwa_plan1d_field_map(srcwa_plan* plan, int problem, int nz, const double* zs, int nx, double y, double* out);
+
+/* Near fields of problem `problem` of an executed conical plan created with SRCWA_PLAN_FIELDS, for the
+ * incident Jones vector jones = [Es.re, Es.im, Ep.re, Ep.im] (normalized first, rcwa_fields.hpp:218-230): as
+ * srcwa_plan1d_field_at / _field_map (ConicalFieldContext, field_at, field_map,
+ * rcwa1d_conical_fields.hpp:14-262), with the grid's transverse factor exp(-i (kxm x + ky y)).  Layers of
+ * zero thickness hold no fields (the depth groups skip them).  srcwa_plan1d_field_available serves conical
+ * plans too; a solve with a layer in the Schur basis (a degenerate layer) keeps no capture in this build, so
+ * its context is unavailable.  A Jones vector that is not finite or is zero: SRCWA_EINVAL. */
+int srcwa_plan_conical_field_at(srcwa_plan* plan, int problem, int nz, const double* zs, const double* jones,
+                                double* out);
+int srcwa_plan_conical_field_map(srcwa_plan* plan, int problem, int nz, const double* zs, int nx, double y,
+                                 const double* jones, double* out);
 int64_t srcwa_plan_device_bytes(const srcwa_plan* plan);
 int srcwa_plan_chunks(const srcwa_plan* plan);
 /* Shared operator sets of the plan (0 with SRCWA_PLAN_POINT_OPS, with diagnostics, or when no two points

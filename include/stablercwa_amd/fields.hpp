@@ -4,9 +4,10 @@
 //
 // The solve runs as a one-problem plan created with SRCWA_PLAN_FIELDS: the S-matrix kernel keeps each
 // layer's amplitudes a+ (layer top) and a- (layer bottom), the modes stay in the plan's device workspace,
-// and field_at / field_map evaluate them on the device (srcwa_plan1d_field_at / _field_map).  A context
-// shares its plan with its copies: the capture is never written after the solve, so a copy behaves as the
-// reference's deep copy.  Conical and crossed-grating near fields are not provided by this build.
+// and field_at / field_map evaluate them on the device (srcwa_plan1d_field_at / _field_map).  Conical solves
+// likewise (solve_conical_with_fields, ConicalFieldContext; rcwa1d_conical_fields.hpp), with the Jones vector
+// chosen per call.  A context shares its plan with its copies: the capture is never written after the solve,
+// so a copy behaves as the reference's deep copy.  Crossed-grating near fields are not provided by this build.
 #pragma once
 
 #include <array>
@@ -331,6 +332,206 @@ inline FieldGridXZ field_map(const PlanarFieldContext& context, const RealVector
 }
 inline FieldGridXZ field_map(const PlanarFieldResult& result, const RealVector& zs, int nx = 256, double y = 0.0) {
     return field_map(result.fields, zs, nx, y);
+}
+
+// ---- conical near fields (rcwa1d_conical_fields.hpp:14-262) ----------------------------------------------
+
+using JonesVector = std::array<cplx, 2>;   // [Es, Ep]
+
+// normalized_jones (rcwa_fields.hpp:218-230)
+inline JonesVector normalized_jones(JonesVector jones) {
+    double scale = 0.0;
+    for (cplx v : jones) {
+        if (!std::isfinite(v.real()) || !std::isfinite(v.imag()))
+            throw std::invalid_argument("rcwa: Jones input must be finite and nonzero");
+        scale = (std::max)(scale, (std::max)(std::abs(v.real()), std::abs(v.imag())));
+    }
+    if (scale == 0.0) throw std::invalid_argument("rcwa: Jones input must be finite and nonzero");
+    for (auto& v : jones) v /= scale;
+    const double norm = std::sqrt(std::norm(jones[0]) + std::norm(jones[1]));
+    for (auto& v : jones) v /= norm;
+    return jones;
+}
+
+// jones_from_angles (rcwa_fields.hpp:232-237)
+inline JonesVector jones_from_angles(double psi_deg = 90.0, double delta_deg = 0.0) {
+    if (!std::isfinite(psi_deg) || !std::isfinite(delta_deg))
+        throw std::invalid_argument("rcwa: Jones angles must be finite");
+    const double psi = psi_deg * ::srcwa_amd::detail::kPi / 180.0, delta = delta_deg * ::srcwa_amd::detail::kPi / 180.0;
+    return normalized_jones({std::sin(psi) * std::exp(cplx(0.0, 1.0) * delta), cplx(std::cos(psi), 0.0)});
+}
+
+struct ConicalFieldResult;
+class ConicalFieldContext;
+namespace detail {
+struct ConicalFieldAccess {
+    static srcwa_plan* plan(const ConicalFieldContext& c);
+};
+}  // namespace detail
+
+// The capture of one conical solve for both incident polarizations; a Jones vector is chosen per call.  A
+// solve with a degenerate (Schur-basis) layer leaves the context Unavailable in this build.
+class ConicalFieldContext {
+public:
+    ConicalFieldContext() = default;
+    ConicalFieldContext(const ConicalFieldContext&) = default;
+    ConicalFieldContext& operator=(const ConicalFieldContext&) = default;
+    ConicalFieldContext(ConicalFieldContext&& other) noexcept { *this = std::move(other); }
+    ConicalFieldContext& operator=(ConicalFieldContext&& other) noexcept {
+        if (this != &other) {
+            metadata_ = std::move(other.metadata_);
+            plan_ = std::move(other.plan_);
+            status_ = std::exchange(other.status_, FieldContextStatus::Empty);
+            other.metadata_ = FieldMetadata{};
+            other.plan_.reset();
+        }
+        return *this;
+    }
+    FieldContextStatus status() const noexcept { return status_; }
+    bool available() const noexcept { return status_ == FieldContextStatus::Ready; }
+    const FieldMetadata& metadata() const noexcept { return metadata_; }
+    void require_available() const {
+        if (!available()) throw std::logic_error("rcwa: conical field context is unavailable");
+    }
+
+private:
+    FieldMetadata metadata_;
+    std::shared_ptr<::srcwa_amd::detail::Plan> plan_;
+    FieldContextStatus status_ = FieldContextStatus::Empty;
+    friend struct detail::ConicalFieldAccess;
+    friend ConicalFieldResult solve_conical_with_fields(const Stack&, double, double, double, int, const Options&);
+};
+
+inline srcwa_plan* detail::ConicalFieldAccess::plan(const ConicalFieldContext& c) { return c.plan_ ? c.plan_->p : nullptr; }
+
+struct ConicalFieldResult {
+    ConicalResult result;
+    ConicalFieldContext fields;
+};
+
+// One conical solve with its capture for the s and p incidence (rcwa1d_conical_fields.hpp:56-98).
+inline ConicalFieldResult solve_conical_with_fields(const Stack& stack, double wavelength, double theta_deg,
+                                                   double phi_deg, int M, const Options& options = {}) {
+    ConicalFieldResult out;
+    std::shared_ptr<::srcwa_amd::detail::Plan> plan;
+    out.result = detail::conical_batch(stack, {ConicalPoint{wavelength, theta_deg, phi_deg}}, M, options,
+                                       SRCWA_PLAN_FIELDS, &plan)[0];
+    int ok = 0;
+    ::srcwa_amd::detail::check(srcwa_plan1d_field_available(plan->p, 0, &ok));
+    ConicalFieldContext& ctx = out.fields;
+    FieldMetadata& meta = ctx.metadata_;
+    meta.kind = FieldSolveKind::Conical;
+    meta.wavelength = wavelength;
+    meta.k0 = 2.0 * ::srcwa_amd::detail::kPi / wavelength;
+    meta.theta_deg = theta_deg;
+    meta.phi_deg = phi_deg;
+    meta.truncation_order = M;
+    meta.incident_basis_columns = 2;
+    meta.orders = out.result.orders;
+    meta.normalization.electric_incident_amplitude = 1.0;
+    meta.normalization.magnetic_incident_amplitude = stack.nI.real();
+    // geometry with zero-thickness layers skipped (copy_field_geometry, rcwa_fields.hpp:403-434)
+    meta.period = stack.period;
+    meta.nI = stack.nI;
+    meta.nII = stack.nII;
+    double z = 0.0;
+    int effective = 0;
+    for (std::size_t j = 0; j < stack.layers.size(); ++j) {
+        const Layer& l = stack.layers[j];
+        FieldLayerInfo info_j;
+        info_j.original_index = j;
+        info_j.effective_index = l.d == 0.0 ? -1 : effective++;
+        info_j.z_top = z;
+        info_j.thickness = l.d;
+        z += l.d;
+        info_j.z_bottom = z;
+        info_j.homogeneous = l.homogeneous;
+        info_j.uniform_index = l.n;
+        info_j.edges = l.edges;
+        info_j.n_values = l.n_values;
+        meta.layers.push_back(std::move(info_j));
+    }
+    meta.z_bottom = z;
+    meta.ky = meta.k0 * out.result.ky;
+    bool finite = std::isfinite(meta.k0) && std::isfinite(meta.ky);
+    for (std::size_t j = 0; j < out.result.kx.size(); ++j) {
+        meta.kxm.push_back(meta.k0 * out.result.kx[j]);
+        meta.kzI.push_back(meta.k0 * out.result.kzI[j]);
+        meta.kzII.push_back(meta.k0 * out.result.kzII[j]);
+        for (const cplx& v : {meta.kxm.back(), meta.kzI.back(), meta.kzII.back()})
+            finite = finite && std::isfinite(v.real()) && std::isfinite(v.imag());
+    }
+    ctx.status_ = ok && finite ? FieldContextStatus::Ready : FieldContextStatus::Unavailable;
+    if (ctx.available()) {
+        ctx.plan_ = std::move(plan);
+        const int modes = 2 * static_cast<int>(out.result.orders.size());
+        for (FieldLayerInfo& l : meta.layers) {
+            if (l.effective_index < 0) continue;
+            l.has_modes = true;
+            l.propagates = true;
+            l.analytic_modes = l.homogeneous;
+            l.mode_count = modes;
+            l.propagation = FieldPropagation::Exponential;
+        }
+    }
+    return out;
+}
+
+inline HarmonicFields field_at(const ConicalFieldContext& context, const RealVector& zs,
+                               JonesVector jones = {{1.0, 0.0}}) {
+    context.require_available();
+    jones = normalized_jones(jones);
+    for (double v : zs)
+        if (!std::isfinite(v)) throw std::invalid_argument("rcwa: field depths must be finite");
+    HarmonicFields out;
+    out.metadata = context.metadata();
+    out.orders = out.metadata.orders;
+    out.z = zs;
+    const int n = static_cast<int>(out.orders.size()), nz = static_cast<int>(zs.size());
+    const double jv[4] = {jones[0].real(), jones[0].imag(), jones[1].real(), jones[1].imag()};
+    std::vector<double> v(12 * static_cast<std::size_t>(n) * nz);
+    if (nz > 0)
+        ::srcwa_amd::detail::check(
+            srcwa_plan_conical_field_at(detail::ConicalFieldAccess::plan(context), 0, nz, zs.data(), jv, v.data()));
+    detail::fill_components(out.components, v, n, nz);
+    return out;
+}
+inline HarmonicFields field_at(const ConicalFieldContext& context, double z, JonesVector jones = {{1.0, 0.0}}) {
+    return field_at(context, RealVector{z}, jones);
+}
+inline HarmonicFields field_at(const ConicalFieldResult& result, const RealVector& zs, JonesVector jones = {{1.0, 0.0}}) {
+    return field_at(result.fields, zs, jones);
+}
+inline HarmonicFields field_at(const ConicalFieldResult& result, double z, JonesVector jones = {{1.0, 0.0}}) {
+    return field_at(result.fields, z, jones);
+}
+
+inline FieldGridXZ field_map(const ConicalFieldContext& context, const RealVector& zs, int nx = 256, double y = 0.0,
+                             JonesVector jones = {{1.0, 0.0}}) {
+    if (nx <= 0) throw std::invalid_argument("rcwa: field nx must be positive");
+    if (!std::isfinite(y)) throw std::invalid_argument("rcwa: field y must be finite");
+    context.require_available();
+    jones = normalized_jones(jones);
+    for (double v : zs)
+        if (!std::isfinite(v)) throw std::invalid_argument("rcwa: field depths must be finite");
+    FieldGridXZ out;
+    out.metadata = context.metadata();
+    out.z = zs;
+    out.y = y;
+    out.x.resize(nx);
+    for (int ix = 0; ix < nx; ++ix) out.x[ix] = (static_cast<double>(ix) / nx) * out.metadata.period;
+    const int nz = static_cast<int>(zs.size());
+    const double jv[4] = {jones[0].real(), jones[0].imag(), jones[1].real(), jones[1].imag()};
+    std::vector<double> v(12 * static_cast<std::size_t>(nx) * nz);
+    if (nz > 0)
+        ::srcwa_amd::detail::check(srcwa_plan_conical_field_map(detail::ConicalFieldAccess::plan(context), 0, nz,
+                                                                zs.data(), nx, y, jv, v.data()));
+    detail::fill_components(out.components, v, nx, nz);
+    return out;
+}
+inline FieldGridXZ field_map(const ConicalFieldResult& result, const RealVector& zs, int nx = 256, double y = 0.0,
+                             JonesVector jones = {{1.0, 0.0}}) {
+    return field_map(result.fields, zs, nx, y, jones);
 }
 
 }  // namespace rcwa

@@ -12,9 +12,9 @@
 // Error behaviour: invalid input throws std::invalid_argument with the reference's message; a numerical
 // failure returns NaN arrays with eig_info set (planar, conical) or status NumericalFailure (crossed);
 // options this build does not provide (custom hooks -- see stablercwa_amd/hooks.hpp for the other
-// direction --, LR eigensolver, non-pivoting or QR linear solvers, homogeneous_fast_path = false, conical and
+// direction --, LR eigensolver, non-pivoting or QR linear solvers, homogeneous_fast_path = false,
 // crossed-grating near-field capture) throw srcwa_amd::Unsupported; a HIP runtime error throws
-// std::runtime_error.  Planar near fields are in stablercwa_amd/fields.hpp.  FFT-sampled
+// std::runtime_error.  Planar and conical near fields are in stablercwa_amd/fields.hpp.  FFT-sampled
 // grating layers (FourierMethod::FFT) are computed on the GPU.  sweep* run every point in one batched GPU
 // launch; results keep the input order and each result is bitwise the same as its single solve.
 #pragma once
@@ -27,6 +27,7 @@
 #include <functional>
 #include <limits>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <utility>
@@ -555,8 +556,11 @@ inline std::vector<Result> planar_batch(const Stack& stack, const std::vector<Po
     return out;
 }
 
+// keep: when given, receives the executed plan (a near-field caller keeps its capture on the device);
+// extra_flags: plan flags beyond the options' (SRCWA_PLAN_FIELDS).
 inline std::vector<ConicalResult> conical_batch(const Stack& stack, const std::vector<ConicalPoint>& points, int M,
-                                                const Options& options) {
+                                                const Options& options, int extra_flags = 0,
+                                                std::shared_ptr<Plan>* keep = nullptr) {
     require_gpu_options(options, true);
     const auto t0 = Clock::now();
     std::vector<ConicalResult> out(points.size());
@@ -572,11 +576,13 @@ inline std::vector<ConicalResult> conical_batch(const Stack& stack, const std::v
         th[i] = points[i].theta_deg;
         ph[i] = points[i].phi_deg;
     }
-    Plan plan;
-    check(srcwa_plan_conical_create(context(options.device, plan_flags(options)), 1,
+    auto owned = std::make_shared<Plan>();
+    Plan& plan = *owned;
+    check(srcwa_plan_conical_create(context(options.device, plan_flags(options) | extra_flags), 1,
                                     rec.data(), &off, P, idx.data(), wl.data(), th.data(), ph.data(), Ms.data(),
                                     options.max_workspace_bytes, &plan.p));
     plan.run();
+    if (keep) *keep = owned;
     const PlanReport rep = report(plan.p, P);
     const std::int64_t orders = srcwa_plan_orders(plan.p);
     std::vector<double> J(8 * orders), Jt(8 * orders), DEr(4 * orders), DEt(4 * orders), geo(8 * orders);

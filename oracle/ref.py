@@ -169,3 +169,25 @@ def planar_fields(lib, stack_flat, wavelength, theta, M, pol, zs, nx=0, y=0.0):
     h = harm.view(np.complex128).reshape(6, nz, N).transpose(0, 2, 1) if ok else None
     g = grid.view(np.complex128).reshape(6, nz, nx) if ok and nx > 0 else None
     return dict(available=ok, harm=h, grid=g, DEr=DEr, DEt=DEt)
+
+
+def conical_fields(lib, stack_flat, wavelength, theta, phi, M, jones, zs, nx=0, y=0.0):
+    """rcwa::solve_conical_with_fields + field_at (+ field_map with nx > 0) of the reference for the Jones vector
+    jones = (Es, Ep) (rcwa1d_conical_fields.hpp:56-262).  Returns dict(available, n_schur, harm [6, N, nz],
+    grid [6, nz, nx] or None)."""
+    st = np.ascontiguousarray(stack_flat, dtype=np.float64)
+    z = np.ascontiguousarray(np.atleast_1d(np.asarray(zs, dtype=np.float64)))
+    j = np.array([complex(jones[0]).real, complex(jones[0]).imag, complex(jones[1]).real, complex(jones[1]).imag])
+    N, nz = 2 * M + 1, z.size
+    harm = np.zeros(12 * N * nz)
+    grid = np.zeros(12 * max(nx, 0) * nz)
+    avail = ctypes.c_int(0); ns = ctypes.c_int(0)
+    rc = lib.ref_conical_fields(_p(st), ctypes.c_double(wavelength), ctypes.c_double(theta), ctypes.c_double(phi),
+                                int(M), _p(j), int(nz), _p(z), int(nx), ctypes.c_double(y), ctypes.byref(avail),
+                                ctypes.byref(ns), _p(harm), _p(grid))
+    if rc != 0:
+        raise RuntimeError(lib.ref_last_error().decode())
+    ok = bool(avail.value)
+    h = harm.view(np.complex128).reshape(6, nz, N).transpose(0, 2, 1) if ok else None
+    g = grid.view(np.complex128).reshape(6, nz, nx) if ok and nx > 0 else None
+    return dict(available=ok, n_schur=ns.value, harm=h, grid=g)

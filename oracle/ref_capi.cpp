@@ -25,6 +25,7 @@
 #include "stablercwa/rcwa1d.hpp"
 #include "stablercwa/rcwa1d_fields.hpp"
 #include "stablercwa/rcwa1d_conical.hpp"
+#include "stablercwa/rcwa1d_conical_fields.hpp"
 #include "stablercwa/rcwa2d.hpp"
 #include "stablercwa/eig.hpp"
 #include "ref_decode.hpp"
@@ -145,6 +146,38 @@ int ref_planar_fields(const double* stack, double wavelength, double theta_deg, 
             }
         if (nx <= 0) return;
         const auto g = rcwa::field_map(res, z, nx, y);
+        for (int c = 0; c < 6; ++c)
+            for (int k = 0; k < nx * nz; ++k) {
+                const cplx v = g.components[c].data()[k];
+                grid[2 * (static_cast<std::size_t>(c) * nx * nz + k)] = v.real();
+                grid[2 * (static_cast<std::size_t>(c) * nx * nz + k) + 1] = v.imag();
+            }
+    });
+}
+
+// Near fields of one conical solve for the Jones vector jones = [Es.re, Es.im, Ep.re, Ep.im]:
+// rcwa::solve_conical_with_fields, then field_at and field_map (rcwa1d_conical_fields.hpp:56-262).  Layout and
+// availability as ref_planar_fields; n_schur: the solve's Schur-basis layers.
+int ref_conical_fields(const double* stack, double wavelength, double theta_deg, double phi_deg, int M,
+                       const double* jones, int nz, const double* zs, int nx, double y, int* available,
+                       int* n_schur, double* harm, double* grid) {
+    return guarded([&] {
+        const auto res = rcwa::solve_conical_with_fields(decode1d(stack), wavelength, theta_deg, phi_deg, M);
+        const int n = 2 * M + 1;
+        *available = res.fields.available() ? 1 : 0;
+        *n_schur = res.result.n_schur;
+        if (!*available) return;
+        const stablercwa::rcwa::JonesVector jv{{cplx(jones[0], jones[1]), cplx(jones[2], jones[3])}};
+        const stablercwa::RealVector z(zs, zs + nz);
+        const auto h = rcwa::field_at(res, z, jv);
+        for (int c = 0; c < 6; ++c)
+            for (int k = 0; k < n * nz; ++k) {
+                const cplx v = h.components[c].data()[k];
+                harm[2 * (static_cast<std::size_t>(c) * n * nz + k)] = v.real();
+                harm[2 * (static_cast<std::size_t>(c) * n * nz + k) + 1] = v.imag();
+            }
+        if (nx <= 0) return;
+        const auto g = rcwa::field_map(res, z, nx, y, jv);
         for (int c = 0; c < 6; ++c)
             for (int k = 0; k < nx * nz; ++k) {
                 const cplx v = g.components[c].data()[k];

@@ -72,6 +72,9 @@ PROTOTYPES = {
     "srcwa_plan1d_field_at": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     "srcwa_plan1d_field_map": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double,
                                               _vp]),
+    "srcwa_plan_conical_field_at": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
+    "srcwa_plan_conical_field_map": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
+                                                    ctypes.c_double, _vp, _vp]),
     "srcwa_plan_operator_sets": (ctypes.c_int, [_vp]),
     "srcwa_plan_geometry": (ctypes.c_int, [_vp, _vp]),
     "srcwa_plan_device_bytes": (ctypes.c_int64, [_vp]),

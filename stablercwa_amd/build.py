@@ -5,7 +5,7 @@
 
 Each family source (rcwa1d.hip, conical.hip, rcwa2d.hip, dense.hip) is compiled as several units: one
 "main" unit (its light kernels and host launcher), one for its S-matrix kernel (-DSRCWA_PART_ONLY=1)
-(rcwa1d.hip: also its near-field form, -DSRCWA_PART_ONLY=2) and one per eigen-kernel instantiation QC in
+(rcwa1d.hip and conical.hip: also their near-field forms, -DSRCWA_PART_ONLY=2) and one per eigen-kernel instantiation QC in
 {1, 2, 4, 8, 12, 16} (-DSRCWA_QC_ONLY=QC), so the slow units build in parallel.
 """
 from __future__ import annotations
@@ -33,7 +33,7 @@ FAMILIES = ["rcwa1d.hip", "conical.hip", "rcwa2d.hip", "dense.hip"]
 HOST_SOURCES = ["runtime.cpp", "server.cpp"]
 QCS = (1, 2, 4, 8, 12, 16)
 # (source, unit): 0 = main, "p1" = the S-matrix part, QC = that eigen instantiation
-UNITS = ([(f, 0) for f in FAMILIES] + [(f, "p1") for f in FAMILIES if f != "dense.hip"] + [("rcwa1d.hip", "p2")] +
+UNITS = ([(f, 0) for f in FAMILIES] + [(f, "p1") for f in FAMILIES if f != "dense.hip"] + [("rcwa1d.hip", "p2"), ("conical.hip", "p2")] +
          [(f, q) for f in FAMILIES for q in QCS] + [(f, 0) for f in HOST_SOURCES])
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -697,7 +697,11 @@ void SRCWA_QC_FN(conical_qc)(const QcLaunch& a) {
 #endif
 
 // ------------------------------------------------------------------------------------------------
-#if SRCWA_IN_PART(1)
+// FIELDS: the plan keeps the near-field capture (PLAN_FIELDS): per layer a+ and a- (2N x 2, columns s and p
+// incidence), then the outer responses up and down (2N x 2 each), at Inst1D::field_off.  The two forms are
+// instantiated in units of their own (part 1 without, part 2 with the capture).
+#if SRCWA_IN_PART(1) || SRCWA_IN_PART(2)
+template <bool FIELDS>
 __global__ SMAT_KERNEL_BOUNDS_OF(SRCWA_SMAT_OCC_CONICAL) void k_smatrix_conical(const Inst1D* __restrict__ insts,
                                                         const double* __restrict__ stacks,
                                                         const int64_t* __restrict__ layouts, cd* __restrict__ ws,
@@ -786,8 +790,34 @@ __global__ SMAT_KERNEL_BOUNDS_OF(SRCWA_SMAT_OCC_CONICAL) void k_smatrix_conical(
         cd* downm = c.bump.keep(2 * (int64_t)n);
         c.check();
         const int cols[2] = {M, N + M};
-        if (!c.fail) run_recursion(c, L, k0, s_thick, region, 2, cols, upm, downm);
+        cd* fields = FIELDS ? reinterpret_cast<cd*>(out.fields) + in.field_off : nullptr;
+        if (!c.fail) run_recursion(c, L, k0, s_thick, region, 2, cols, upm, downm, fields, pure, M, N);
         status = c.fail ? (c.fail == 2 ? ST_WORKSPACE : ST_NUMERICAL) : ST_OK;
+        if (FIELDS && status == ST_OK) {
+            // the outer responses (before the reflected p sign) and the capture's validity
+            // (rcwa1d_conical.hpp:232-241, 957-970); a Schur-basis layer is not captured by this build
+            cd* resp = fields + 4 * (int64_t)L * n;
+            int bad = 0;
+            for (int e = threadIdx.x; e < 2 * n; e += NT) {
+                const bool keep = !pure || (e % n) % N == M;
+                resp[e] = keep ? upm[e] : cd{0.0, 0.0};
+                resp[2 * n + e] = keep ? downm[e] : cd{0.0, 0.0};
+                bad |= !finite(resp[e]) || !finite(resp[2 * n + e]);
+            }
+            for (int64_t e = threadIdx.x; e < 4 * (int64_t)L * n; e += NT) bad |= !finite(fields[e]);
+            for (int r = 0; r < L + 2; ++r) {
+                const Region g = region(r);
+                if (g.X != nullptr) bad = 1;
+                for (int i = threadIdx.x; i < n; i += NT) {
+                    bad |= !finite(g.q[i]);
+                    if (g.analytic) bad |= !finite(g.w[i]) || !finite(g.p[i]);
+                }
+                if (!g.analytic)
+                    for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) bad |= !finite(g.W[e]) || !finite(g.V[e]);
+            }
+            bad = block_max_int(bad);
+            if (threadIdx.x == 0) out.field_ok[blockIdx.x] = !bad;
+        }
         if (status == ST_OK) {
             int bad = 0;
             for (int j = threadIdx.x; j < N; j += NT) {
@@ -824,6 +854,7 @@ __global__ SMAT_KERNEL_BOUNDS_OF(SRCWA_SMAT_OCC_CONICAL) void k_smatrix_conical(
         }
     }
     if (threadIdx.x == 0) {
+        if (FIELDS && status != ST_OK) out.field_ok[blockIdx.x] = 0;
         out.status[blockIdx.x] = status;
         out.eig_info[blockIdx.x] = eig_info;
         double* dg = out.diag + (int64_t)DIAG_WORDS * blockIdx.x;
@@ -835,16 +866,161 @@ __global__ SMAT_KERNEL_BOUNDS_OF(SRCWA_SMAT_OCC_CONICAL) void k_smatrix_conical(
         c.refine_record(dg);
     }
 }
+#endif
+#if SRCWA_IN_PART(1)
 void smatrix_conical_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out) {
-    k_smatrix_conical<<<grid, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
+    k_smatrix_conical<false><<<grid, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
+}
+#endif
+#if SRCWA_IN_PART(2)
+void smatrix_conical_fields_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out) {
+    k_smatrix_conical<true><<<grid, NT, 0, s>>>(insts, stacks, layouts, ws, task_info, out);
 }
 #endif
 
 // ------------------------------------------------------------------------------------------------
-// the S-matrix kernel is compiled in its own unit
+// the S-matrix kernel is compiled in units of its own
 void smatrix_conical_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out);
+void smatrix_conical_fields_launch(int grid, hipStream_t s, const Inst1D* insts, const double* stacks, const int64_t* layouts, cd* ws, const int* task_info, Out1D out);
 
 #if SRCWA_MAIN_TU
+// ------------------------------------------------------------------------------------------------
+// Near fields of one conical solve for one incident Jones vector (ConicalFieldSampler::sample,
+// rcwa1d_conical_fields.hpp:158-236): one workgroup per depth group.  The captured 2N x 2 amplitudes are
+// combined with the (normalized) Jones vector (j0, j1), propagated from their anchors, mapped to the
+// tangential fields F = W (p + m), G = V (m - p) (analytic regions: w, v scaling), rotated from the (s, t)
+// basis of each order to x, y, and Ez = [[eps]]^-1 (i (ky hx - kx hy)) (a uniform region: / eps).
+// out[c][i + iz N], c = Ex, Ey, Ez, Hx, Hy, Hz.
+__global__ __launch_bounds__(NT) void k_fields_conical(const Inst1D* __restrict__ inst, const double* __restrict__ stacks,
+                                                       const int64_t* __restrict__ layouts, const cd* __restrict__ ws,
+                                                       const cd* __restrict__ fields, const FieldGroup1D* __restrict__ groups,
+                                                       const int* __restrict__ cols, const double* __restrict__ zs, int nz,
+                                                       cd j0, cd j1, cd* __restrict__ out, cd* __restrict__ scratch) {
+    const FieldGroup1D g = groups[blockIdx.x];
+    const Inst1D in = *inst;
+    const double* st = stacks + in.stack_off;
+    const int64_t* lay = layouts + in.layout_off;
+    const int N = in.N, M = in.M, L = in.L, n = 2 * N, cnt = g.count;
+    const int64_t Nc = (int64_t)n * cnt, NC = (int64_t)N * cnt, NZ = (int64_t)N * nz;
+    const double k0 = 2.0 * kPi / in.wavelength;
+    const ConicalGeo geo = geo_of(const_cast<cd*>(ws), lay, N);
+    const double theta = in.theta_deg * kPi / 180.0, phi = in.phi_deg * kPi / 180.0;
+    const double ky = st[1] * sin(theta) * sin(phi);
+    cd* P = scratch + g.scratch;
+    cd* Q = P + Nc;
+    cd* F = Q + Nc;
+    cd* G = F + Nc;
+    cd* E1 = G + Nc;
+    cd* E2 = E1 + NC;
+    const cd* Ff = P;
+    const cd* Gf = Q;
+    const cd* IE = nullptr;           // [[eps]]^-1 of a grating region
+    cd eps_scale{0.0, 0.0};
+    if (g.region >= 0) {
+        const int j = g.region, r = j + 1;
+        const cd* q = ws + lay[conical_q(r)];
+        const int64_t sb = lay[conical_slot(L, j)];
+        const cd* ap = fields + in.field_off + (int64_t)j * 2 * n;
+        const cd* am = fields + in.field_off + (int64_t)(L + j) * 2 * n;
+        const double d = g.d;
+        const cd* w = sb < 0 ? ws + lay[conical_w(r)] : nullptr;
+        const cd* v = sb < 0 ? ws + lay[conical_v(r)] : nullptr;
+        for (int64_t e = threadIdx.x; e < Nc; e += NT) {
+            const int i = (int)(e % n), col = (int)(e / n);
+            const double from_top = fmax(0.0, fmin(d, zs[cols[g.first + col]] - g.z_top));
+            const double from_bottom = d - from_top;
+            const cd a = ap[i] * j0 + ap[n + i] * j1, b = am[i] * j0 + am[n + i] * j1;
+            const bool az = a.x == 0.0 && a.y == 0.0, bz = b.x == 0.0 && b.y == 0.0;
+            const cd pp = az ? cd{0.0, 0.0} : a * cexp((-k0 * from_top) * q[i]);
+            const cd mm = bz ? cd{0.0, 0.0} : b * cexp((-k0 * from_bottom) * q[i]);
+            if (sb < 0) {
+                P[e] = w[i] * (pp + mm);
+                Q[e] = v[i] * (mm - pp);
+            } else {
+                P[e] = pp + mm;
+                Q[e] = mm - pp;
+            }
+        }
+        __syncthreads();
+        if (sb >= 0) {
+            const ConicalSlots sl = slots_of(const_cast<cd*>(ws) + sb, N);
+            wg_gemm<GEMM_SET, false>(n, cnt, n, sl.W, n, P, n, F, n);
+            wg_gemm<GEMM_SET, false>(n, cnt, n, sl.V, n, Q, n, G, n);
+            Ff = F;
+            Gf = G;
+            IE = sl.IE;
+        } else {
+            const LayerView lv = layer_at(st, (int)lay[conical_orig(L, j)]);
+            const cd index = lv.homogeneous ? lv.n : cd{lv.vals[0], lv.vals[1]};
+            eps_scale = cdiv(cd{1.0, 0.0}, index * index);
+        }
+    } else {
+        const bool top = g.region == -1;
+        const int r = top ? 0 : L + 1;
+        const cd* w = ws + lay[conical_w(r)];
+        const cd* v = ws + lay[conical_v(r)];
+        const cd* resp = fields + in.field_off + 4 * (int64_t)L * n + (top ? 0 : 2 * n);
+        const cd kz0 = k0 * geo.kzI[M];
+        for (int64_t e = threadIdx.x; e < Nc; e += NT) {
+            const int i = (int)(e % n), col = (int)(e / n), o = i % N;
+            const double z = zs[cols[g.first + col]];
+            const cd rsp = resp[i] * j0 + resp[n + i] * j1;
+            const bool rz = rsp.x == 0.0 && rsp.y == 0.0;
+            const cd zero{0.0, 0.0};
+            cd a = zero, b = zero, t = zero;
+            if (top) {
+                if (o == M) a = (i < N ? j0 : j1) * cexp(cd{-kz0.y, kz0.x} * z);
+                if (!rz) {
+                    const cd kz = k0 * geo.kzI[o];
+                    b = rsp * cexp(cd{kz.y, -kz.x} * z);
+                }
+            } else if (!rz) {
+                const cd kz = k0 * geo.kzII[o];
+                t = rsp * cexp(cd{-kz.y, kz.x} * (z - g.z_bottom));
+            }
+            P[e] = w[i] * ((a + b) + t);
+            Q[e] = v[i] * ((b - a) - t);
+        }
+        __syncthreads();
+        const cd nn = top ? cd{st[1], st[2]} : cd{st[3], st[4]};
+        eps_scale = cdiv(cd{1.0, 0.0}, nn * nn);
+    }
+    // rotate (s, t) -> (x, y) per order (rcwa1d_conical_fields.hpp:210-220); Ez before [[eps]]^-1 into E1
+    for (int64_t e = threadIdx.x; e < NC; e += NT) {
+        const int jj = (int)(e % N), col = (int)(e / N);
+        const int iz = cols[g.first + col];
+        const double cs = geo.cs[jj].x, sn = geo.sn[jj].x, kx = geo.kx[jj].x;
+        const cd Fa = Ff[jj + (int64_t)col * n], Fb = Ff[N + jj + (int64_t)col * n];
+        const cd Ga = Gf[jj + (int64_t)col * n], Gb = Gf[N + jj + (int64_t)col * n];
+        const cd Ex = -sn * Fa + cs * Fb;
+        const cd Ey = cs * Fa + sn * Fb;
+        const cd hx = -cs * Ga - sn * Gb;
+        const cd hy = -sn * Ga + cs * Gb;
+        const cd u = ky * hx - kx * hy;
+        out[0 * NZ + jj + (int64_t)iz * N] = Ex;
+        out[1 * NZ + jj + (int64_t)iz * N] = Ey;
+        out[3 * NZ + jj + (int64_t)iz * N] = cd{hx.y, -hx.x};
+        out[4 * NZ + jj + (int64_t)iz * N] = cd{hy.y, -hy.x};
+        out[5 * NZ + jj + (int64_t)iz * N] = ky * Ex - kx * Ey;
+        E1[e] = cd{-u.y, u.x};
+    }
+    __syncthreads();
+    if (IE) {
+        wg_gemm<GEMM_SET, false>(N, cnt, N, IE, N, E1, N, E2, N);
+        for (int64_t e = threadIdx.x; e < NC; e += NT)
+            out[2 * NZ + (e % N) + (int64_t)cols[g.first + (int)(e / N)] * N] = E2[e];
+    } else {
+        for (int64_t e = threadIdx.x; e < NC; e += NT)
+            out[2 * NZ + (e % N) + (int64_t)cols[g.first + (int)(e / N)] * N] = E1[e] * eps_scale;
+    }
+}
+
+void launch_fields_conical(hipStream_t s, int n_groups, const Inst1D* inst, const double* stacks,
+                           const int64_t* layouts, const cd* ws, const cd* fields, const FieldGroup1D* groups,
+                           const int* cols, const double* zs, int nz, cd j0, cd j1, cd* out, cd* scratch) {
+    k_fields_conical<<<n_groups, NT, 0, s>>>(inst, stacks, layouts, ws, fields, groups, cols, zs, nz, j0, j1, out, scratch);
+}
+
 void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts, const LayerTask* tasks,
                     const double* stacks, const int64_t* layouts, cd* ws, int* task_info, Out1D out, int max_iter, int min_n, int max_n,
                     int part, int flags, hipEvent_t* ev, const int* eig_order, const OpsArgs& ops) {
@@ -899,7 +1075,8 @@ void launch_conical(hipStream_t s, int n_inst, int n_tasks, const Inst1D* insts,
             a.grid = n_tasks;
             conical_qc(eig_columns_per_thread(2 * max_n), a);
         }
-        smatrix_conical_launch(n_inst, s, insts, stacks, layouts, ws, task_info, out);
+        if (out.fields) smatrix_conical_fields_launch(n_inst, s, insts, stacks, layouts, ws, task_info, out);
+        else smatrix_conical_launch(n_inst, s, insts, stacks, layouts, ws, task_info, out);
         if (ev) hipEventRecord(ev[4], s);
     }
 }

@@ -388,7 +388,7 @@ __global__ SMAT_KERNEL_BOUNDS_OF(SRCWA_SMAT_OCC_PLANAR) void k_smatrix1d(const I
         __syncthreads();
         const int cols[1] = {M};
         cd* fields = FIELDS ? reinterpret_cast<cd*>(out.fields) + in.field_off : nullptr;
-        run_recursion(c, L, k0, s_thick, region, 1, cols, R, T, fields, pure, M);
+        run_recursion(c, L, k0, s_thick, region, 1, cols, R, T, fields, pure, M, N);
         status = c.fail ? (c.fail == 2 ? ST_WORKSPACE : ST_NUMERICAL) : ST_OK;
         if (FIELDS && status == ST_OK) {
             // the capture is usable when every mode set and amplitude is finite (rcwa1d.hpp:496-508)
@@ -622,10 +622,13 @@ __global__ __launch_bounds__(NT) void k_fields1d(const Inst1D* __restrict__ inst
 }
 
 // A harmonic field set on an x grid (reconstruct_field_grid, rcwa_fields.hpp:326-355): component c at
-// (x_ix, z_iz) = sum_j exp(-i kxm_j x_ix) H[c][j + iz N], x_ix = ix period / nx; out[c][ix + iz nx].
+// (x_ix, z_iz) = sum_j exp(-i (kxm_j x_ix + ky y)) H[c][j + iz N], x_ix = ix period / nx; out[c][ix + iz nx].
+// kxm_j = k0 kx_j from the first N words of the instance's layout block (planar kx, conical geometry);
+// kyy = ky y (0 for planar solves).
 __global__ __launch_bounds__(NT) void k_fieldmap1d(const Inst1D* __restrict__ inst, const double* __restrict__ stacks,
                                                    const int64_t* __restrict__ layouts, const cd* __restrict__ ws,
-                                                   const cd* __restrict__ H, int nz, int nx, cd* __restrict__ out) {
+                                                   const cd* __restrict__ H, int nz, int nx, double kyy,
+                                                   cd* __restrict__ out) {
     const Inst1D in = *inst;
     const int N = in.N;
     const int64_t cells = (int64_t)nx * nz;
@@ -642,7 +645,7 @@ __global__ __launch_bounds__(NT) void k_fieldmap1d(const Inst1D* __restrict__ in
     cd s{0.0, 0.0};
     for (int j = 0; j < N; ++j) {
         double sn, cs;
-        sincos((k0 * kx[j].x) * x, &sn, &cs);
+        sincos((k0 * kx[j].x) * x + kyy, &sn, &cs);
         s = s + cd{cs, -sn} * h[j];
     }
     out[e] = s;
@@ -654,9 +657,9 @@ void launch_fields1d(hipStream_t s, int n_groups, const Inst1D* inst, const doub
     k_fields1d<<<n_groups, NT, 0, s>>>(inst, stacks, layouts, ws, fields, R, T, groups, cols, zs, nz, out, scratch);
 }
 void launch_fieldmap1d(hipStream_t s, const Inst1D* inst, const double* stacks, const int64_t* layouts, const cd* ws,
-                       const cd* H, int nz, int nx, cd* out) {
+                       const cd* H, int nz, int nx, double kyy, cd* out) {
     const int64_t total = 6 * (int64_t)nx * nz;
-    k_fieldmap1d<<<(unsigned)((total + NT - 1) / NT), NT, 0, s>>>(inst, stacks, layouts, ws, H, nz, nx, out);
+    k_fieldmap1d<<<(unsigned)((total + NT - 1) / NT), NT, 0, s>>>(inst, stacks, layouts, ws, H, nz, nx, kyy, out);
 }
 
 // host launchers

@@ -740,14 +740,16 @@ DEV DOp op_keep(SmatCtx& c, DOp a, cd* slot) {
 // reflected (up) and transmitted (down) amplitudes, N x nrhs each.  On return c.fail says whether
 // the instance failed.
 //
-// Near-field capture (planar, nrhs = 1; rcwa1d.hpp:968-1023): with fields != null the reflection below
-// each layer is kept as the sweep passes it, and the downward sweep writes each layer's amplitudes,
-// a+ at the layer top (fields + j N) and a- = r_below (the amplitudes at the layer bottom) (fields +
-// (L + j) N).  pure: every layer is a uniform film; then only order keep_order of these vectors is kept
-// (the others are not excited), as the reference clears them.
+// Near-field capture (rcwa1d.hpp:968-1023, rcwa1d_conical.hpp:859-919): with fields != null the reflection
+// below each layer is kept as the sweep passes it, and the downward sweep writes each layer's amplitudes
+// (N x nrhs each, column-major), a+ at the layer top (fields + j N nrhs) and a- = r_below (the amplitudes
+// at the layer bottom) (fields + (L + j) N nrhs).  pure: every layer is a uniform film; then only the rows
+// of diffraction order keep_order (row i is order i mod orders) are kept, as the reference clears the
+// others (they are not excited).
 template <class RegionFn>
 DEV void run_recursion(SmatCtx& c, int L, double k0, const double* thick, RegionFn region, int nrhs, const int* cols,
-                       cd* up_out, cd* down_out, cd* fields = nullptr, bool pure = false, int keep_order = 0) {
+                       cd* up_out, cd* down_out, cd* fields = nullptr, bool pure = false, int keep_order = 0,
+                       int orders = 1) {
     __shared__ int g_kind[SMAT_MAX_BLOCKS];
     __shared__ cd* g_ptr[SMAT_MAX_BLOCKS];
     __shared__ int rb_kind[SMAT_MAX_LAYERS];
@@ -870,27 +872,27 @@ DEV void run_recursion(SmatCtx& c, int L, double k0, const double* thick, Region
     op_apply_cols(c, Tuu, tmpv, up_out, nrhs);
     for (int64_t e = threadIdx.x; e < NR; e += NT) up_out[e] = rud[e] + up_out[e];
     __syncthreads();
-    cd* bottom = fields ? c.bump.tmp(N) : nullptr;
+    cd* bottom = fields ? c.bump.tmp(NR) : nullptr;
     c.check();
     if (c.fail) return;
     // pure films: only the incident order is excited (rcwa1d.hpp:1011-1020)
-    auto excited = [&](int i) { return !pure || i == keep_order; };
+    auto excited = [&](int64_t e) { return !pure || (int)(e % N) % orders == keep_order; };
     for (int b = 1; b < nb; ++b) {
         const int j = b / 2;
         if (fields && (b & 1))
-            for (int i = threadIdx.x; i < N; i += NT) fields[(int64_t)j * N + i] = excited(i) ? down[i] : cd{0.0, 0.0};
+            for (int64_t e = threadIdx.x; e < NR; e += NT) fields[j * NR + e] = excited(e) ? down[e] : cd{0.0, 0.0};
         op_apply_cols(c, DOp{g_kind[b], g_ptr[b]}, down, tmpv, nrhs);
         cd* sw = down;
         down = tmpv;
         tmpv = sw;
         if (fields && (b & 1)) {
-            for (int i = threadIdx.x; i < N; i += NT) bottom[i] = excited(i) ? down[i] : cd{0.0, 0.0};
+            for (int64_t e = threadIdx.x; e < NR; e += NT) bottom[e] = excited(e) ? down[e] : cd{0.0, 0.0};
             __syncthreads();
-            cd* am = fields + (int64_t)(L + j) * N;
-            op_apply_cols(c, DOp{rb_kind[j], rb_ptr[j]}, bottom, am, 1);
+            cd* am = fields + (L + j) * NR;
+            op_apply_cols(c, DOp{rb_kind[j], rb_ptr[j]}, bottom, am, nrhs);
             if (pure) {
-                for (int i = threadIdx.x; i < N; i += NT)
-                    if (!excited(i)) am[i] = cd{0.0, 0.0};
+                for (int64_t e = threadIdx.x; e < NR; e += NT)
+                    if (!excited(e)) am[e] = cd{0.0, 0.0};
                 __syncthreads();
             }
         }

@@ -37,7 +37,10 @@ void launch_fields1d(hipStream_t s, int n_groups, const Inst1D* inst, const doub
                      const cd* ws, const cd* fields, const cd* R, const cd* T, const FieldGroup1D* groups,
                      const int* cols, const double* zs, int nz, cd* out, cd* scratch);
 void launch_fieldmap1d(hipStream_t s, const Inst1D* inst, const double* stacks, const int64_t* layouts, const cd* ws,
-                       const cd* H, int nz, int nx, cd* out);
+                       const cd* H, int nz, int nx, double kyy, cd* out);
+void launch_fields_conical(hipStream_t s, int n_groups, const Inst1D* inst, const double* stacks,
+                           const int64_t* layouts, const cd* ws, const cd* fields, const FieldGroup1D* groups,
+                           const int* cols, const double* zs, int nz, cd j0, cd j1, cd* out, cd* scratch);
 void launch_zgemm_batched(hipStream_t s, int count, const DenseItem* items, cd* arena);
 void launch_zgesv_batched(hipStream_t s, int count, const DenseItem* items, cd* arena, int* info);
 #if SRCWA_MIXED_PRECISION
@@ -635,7 +638,7 @@ void enqueue(srcwa_plan* plan, bool timing) {
         Out1D out{plan->R.p, plan->T.p, plan->DEr.p, plan->DEt.p, plan->status.p + ch.i0, plan->eig_info.p + ch.i0,
                   plan->kind == 1 ? nullptr : plan->geometry.p, plan->diag.p + int64_t(DIAG_WORDS) * ch.i0,
                   plan->task_diag.p + ch.t0, plan->item_res.p + 2 * int64_t(ch.t0)};
-        if (plan->kind == 1 && (plan->flags & PLAN_FIELDS)) {
+        if (plan->kind <= 2 && (plan->flags & PLAN_FIELDS)) {
             out.fields = reinterpret_cast<double*>(plan->fields.p);
             out.field_ok = plan->field_ok.p + ch.i0;
         }
@@ -972,9 +975,9 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
         plan->kind = 2;
         plan->n_inst = n_problems;
         plan->flags = ctx->plan_flags;
-        if (plan->flags & PLAN_FIELDS)
-            throw Failure{SRCWA_EUNSUPPORTED, "rcwa: near fields are captured for planar (TE/TM) plans only in this build"};
         const bool diag = plan->flags & PLAN_DIAGNOSTICS;
+        const bool fields = plan->flags & PLAN_FIELDS;
+        int64_t field_words = 0;
         const int64_t budget = max_workspace_bytes > 0 ? max_workspace_bytes : (int64_t(64) << 30);
         const int64_t budget_words = budget / int64_t(sizeof(cd));
         std::vector<Inst1D> h_insts(n_problems);
@@ -983,7 +986,8 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
         int64_t orders = 0, chunk_ws = 0, max_ws = 0;
         PlanChunk cur{0, 0, 0, 0};
         OpSharing sharing;
-        sharing.enabled = !(plan->flags & (PLAN_DIAGNOSTICS | PLAN_POINT_OPS));
+        // a field plan keeps every layer's [[eps]]^-1 in its own slots, where the sampler reads it
+        sharing.enabled = !(plan->flags & (PLAN_DIAGNOSTICS | PLAN_POINT_OPS | PLAN_FIELDS));
         for (int i = 0; i < n_problems; ++i) {
             const int s = stack_index[i];
             if (s < 0 || s >= n_stacks) invalid("rcwa: stack index out of range");
@@ -1035,6 +1039,8 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
             const int64_t ws_lo = w;
             w += cost.bump_words + (diag ? 12 * NN1 + 64 : 0) +
                  ((plan->flags & PLAN_MIXED) ? mixed_solve_words(2 * N, 2 * N) : 0);
+            // near fields: the reflection below every layer (order 2N) is kept, and one more 2N x 2 block
+            if (fields) w += int64_t(L) * align16(4 * NN1) + align16(4 * int64_t(N)) + 64;
             if (i > cur.i0 && chunk_ws + w > budget_words) {
                 cur.i1 = i;
                 cur.t1 = static_cast<int>(h_tasks.size());
@@ -1079,6 +1085,12 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
             in.out_off = orders;
             in.ws_lo = chunk_ws + ws_lo;
             in.ws_hi = chunk_ws + w;
+            in.field_off = field_words;
+            if (fields) {
+                // per layer a+ and a- (2N x 2 each), then the outer responses up and down (2N x 2 each)
+                field_words += 8 * int64_t(N) * (L + 1);
+                plan->h_layers.push_back(all);
+            }
             chunk_ws += w;
             orders += N;
         }
@@ -1086,6 +1098,8 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
         cur.t1 = static_cast<int>(h_tasks.size());
         plan->chunks.push_back(cur);
         max_ws = std::max(max_ws, chunk_ws);
+        if (fields && plan->chunks.size() > 1)
+            invalid("rcwa: a plan with near fields must fit one workspace chunk (raise max_workspace_bytes)");
         plan->n_tasks = static_cast<int>(h_tasks.size());
         plan->total_orders = orders;
         std::vector<LayerOps> h_ops;
@@ -1097,11 +1111,18 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
                     b((plan->chunks.size() > 1 ? 2 : 1) * max_ws * sizeof(cd)) + b(4 * std::max(1, plan->n_tasks) * sizeof(int)) +
                     2 * b(8 * orders * sizeof(double)) + 2 * b(4 * orders * sizeof(double)) +
                     b(GEO_WORDS * orders * sizeof(double)) + 2 * b(n_problems * sizeof(int)) +
-                    diag_bytes(n_problems, plan->n_tasks) + ops_bytes(h_ops, op_words);
+                    diag_bytes(n_problems, plan->n_tasks) + ops_bytes(h_ops, op_words) +
+                    (fields ? b(std::max<int64_t>(1, field_words) * sizeof(cd)) + b(n_problems * sizeof(int)) : 0);
             return;
         }
         hipStream_t st = ctx->stream;
         plan->insts.upload(h_insts, st);
+        if (fields) {
+            plan->fields.alloc(static_cast<size_t>(std::max<int64_t>(1, field_words)));
+            plan->field_ok.alloc(n_problems);
+            plan->h_insts = h_insts;
+            plan->h_stack_words = ps.words;
+        }
         plan->tasks.upload(h_tasks, st);
         plan->upload_ops(h_ops, op_words, st);
         plan->eig_order.upload(longest_first(plan->chunks, h_tasks, h_insts), st);
@@ -1249,10 +1270,10 @@ int srcwa_plan_results(srcwa_plan* plan, double* R, double* T, double* DEr, doub
     });
 }
 
-// ---- near fields of planar plans -----------------------------------------------------------------
+// ---- near fields of planar and conical plans -----------------------------------------------------
 namespace {
-// The planar capture of one problem is usable: the plan keeps fields, has run, the solve succeeded and
-// every captured mode and amplitude is finite (rcwa1d.hpp:1057-1066).
+// The capture of one problem is usable: the plan keeps fields, has run, the solve succeeded and every
+// captured mode and amplitude is finite (rcwa1d.hpp:1057-1066, rcwa1d_conical.hpp:957-970).
 bool field_ready(srcwa_plan* plan, int problem) {
     int st = 1, ok = 0;
     copy_out(&st, plan->status.p + problem, sizeof(int), hipMemcpyDeviceToHost, "D2H", plan);
@@ -1260,10 +1281,11 @@ bool field_ready(srcwa_plan* plan, int problem) {
     return st == ST_OK && ok == 1;
 }
 
-void field_plan_checks(srcwa_plan* plan, int problem, const char* what) {
+// kind: 1 planar, 2 conical, 0 either
+void field_plan_checks(srcwa_plan* plan, int problem, const char* what, int kind) {
     if (!plan) invalid(std::string(what) + ": null plan");
-    if (plan->kind != 1 || !(plan->flags & PLAN_FIELDS))
-        invalid(std::string(what) + ": the plan keeps no near fields (create it with SRCWA_PLAN_FIELDS)");
+    if (plan->kind > 2 || (kind && plan->kind != kind) || !(plan->flags & PLAN_FIELDS))
+        invalid(std::string(what) + ": the plan keeps no near fields of this kind (create it with SRCWA_PLAN_FIELDS)");
     if (problem < 0 || problem >= plan->n_inst) invalid(std::string(what) + ": problem index out of range");
     if (!plan->executed) invalid(std::string(what) + ": the plan has not been executed");
     hip_check(hipSetDevice(plan->ctx->device), "hipSetDevice");
@@ -1271,25 +1293,50 @@ void field_plan_checks(srcwa_plan* plan, int problem, const char* what) {
         if (st) hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
 
+// normalized_jones (rcwa_fields.hpp:218-230): scaled by its largest component, then to unit norm
+void normalize_jones(const double* jones, cd& j0, cd& j1) {
+    if (!jones) invalid("rcwa: Jones input must be finite and nonzero");
+    std::complex<double> v[2] = {{jones[0], jones[1]}, {jones[2], jones[3]}};
+    double scale = 0.0;
+    for (const auto& c : v) {
+        if (!std::isfinite(c.real()) || !std::isfinite(c.imag())) invalid("rcwa: Jones input must be finite and nonzero");
+        scale = std::max(scale, std::max(std::abs(c.real()), std::abs(c.imag())));
+    }
+    if (scale == 0.0) invalid("rcwa: Jones input must be finite and nonzero");
+    for (auto& c : v) c /= scale;
+    const double norm = std::sqrt(std::norm(v[0]) + std::norm(v[1]));
+    for (auto& c : v) c /= norm;
+    j0 = cd{v[0].real(), v[0].imag()};
+    j1 = cd{v[1].real(), v[1].imag()};
+}
+
 // Harmonic fields of problem `problem` at nz depths into the device buffer H (6 x N x nz complex), with the
-// reference's depth groups (field_depth_groups, rcwa_fields.hpp:253-279): z < 0 the incidence side,
-// z > the stack bottom (or any z with no layer of positive thickness) the exit side, else the first layer
-// whose bottom is at or below z + 1e-12.
-void sample_fields(srcwa_plan* plan, int problem, int nz, const double* zs, DevBuf<cd>& H) {
+// reference's depth groups (field_depth_groups, rcwa_fields.hpp:253-279): z < 0 the incidence side, else the
+// exit side unless some layer with modes has its bottom at or below z + 1e-12 (the first such layer; an
+// interface belongs to the layer above).  Planar plans keep every layer; conical plans only the layers of
+// positive thickness (copy_field_geometry with skip_zero, rcwa_fields.hpp:403-434).
+void sample_fields(srcwa_plan* plan, int problem, int nz, const double* zs, DevBuf<cd>& H, cd j0, cd j1) {
     const Inst1D& in = plan->h_insts[problem];
     const std::vector<LayerInfo>& li = plan->h_layers[problem];
-    const int L = in.L, N = in.N;
+    const bool conical = plan->kind == 2;
+    const int N = in.N, all = static_cast<int>(li.size());
     for (int iz = 0; iz < nz; ++iz)
         if (!std::isfinite(zs[iz])) invalid("rcwa: field depths must be finite");
-    std::vector<double> z_top(L), z_bot(L);
+    // the layers with modes: planar all, conical those with d > 0 (region index = position in this list)
+    std::vector<int> idx;
+    std::vector<double> z_top, z_bot;
     double z = 0.0;
     bool finite_layer = false;
-    for (int j = 0; j < L; ++j) {
-        z_top[j] = z;
+    for (int j = 0; j < all; ++j) {
+        const double top = z;
         z += li[j].d;
-        z_bot[j] = z;
+        if (conical && !(li[j].d > 0.0)) continue;
+        idx.push_back(j);
+        z_top.push_back(top);
+        z_bot.push_back(z);
         finite_layer = finite_layer || li[j].d > 0.0;
     }
+    const int L = static_cast<int>(idx.size());
     const double z_bottom = z;
     std::vector<std::vector<int>> members(L + 2);   // 0 incidence, 1 exit, 2 + j layer j
     for (int iz = 0; iz < nz; ++iz) {
@@ -1313,14 +1360,19 @@ void sample_fields(srcwa_plan* plan, int problem, int nz, const double* zs, DevB
         g.region = gi == 0 ? -1 : gi == 1 ? -2 : gi - 2;
         g.first = static_cast<int>(cols.size());
         g.count = static_cast<int>(members[gi].size());
-        g.tm_grating = in.pol == 1 && g.region >= 0 && li[g.region].grating;
+        g.tm_grating = !conical && in.pol == 1 && g.region >= 0 && li[idx[g.region]].grating;
         g.z_top = g.region >= 0 ? z_top[g.region] : 0.0;
-        g.d = g.region >= 0 ? li[g.region].d : 0.0;
+        g.d = g.region >= 0 ? li[idx[g.region]].d : 0.0;
         g.z_bottom = z_bottom;
         g.scratch = scr;
         const int64_t nc = int64_t(N) * g.count;
-        scr += 5 * align16(nc);
-        if (g.tm_grating) scr += align16(NN) + align16(N) + align16(4 * in.M + 1) + fft_scratch_words(li[g.region], in.M) + 16;
+        if (conical) {
+            scr += 4 * align16(2 * nc) + 2 * align16(nc) + 16;
+        } else {
+            scr += 5 * align16(nc);
+            if (g.tm_grating)
+                scr += align16(NN) + align16(N) + align16(4 * in.M + 1) + fft_scratch_words(li[idx[g.region]], in.M) + 16;
+        }
         cols.insert(cols.end(), members[gi].begin(), members[gi].end());
         groups.push_back(g);
     }
@@ -1338,11 +1390,17 @@ void sample_fields(srcwa_plan* plan, int problem, int nz, const double* zs, DevB
     d_cols.upload(cols, st);
     d_zs.upload(h_zs, st);
     d_scr.alloc(static_cast<size_t>(std::max<int64_t>(16, scr)));
-    const int64_t oo = std::accumulate(plan->h_insts.begin(), plan->h_insts.begin() + problem, int64_t(0),
-                                       [](int64_t a, const Inst1D& x) { return a + x.N; });
-    launch_fields1d(st, static_cast<int>(groups.size()), plan->insts.p + problem, plan->stacks.p, plan->layouts.p,
-                    plan->ws.p, plan->fields.p, reinterpret_cast<const cd*>(plan->R.p) + oo,
-                    reinterpret_cast<const cd*>(plan->T.p) + oo, d_groups.p, d_cols.p, d_zs.p, nz, H.p, d_scr.p);
+    if (conical) {
+        launch_fields_conical(st, static_cast<int>(groups.size()), plan->insts.p + problem, plan->stacks.p,
+                              plan->layouts.p, plan->ws.p, plan->fields.p, d_groups.p, d_cols.p, d_zs.p, nz, j0, j1,
+                              H.p, d_scr.p);
+    } else {
+        const int64_t oo = std::accumulate(plan->h_insts.begin(), plan->h_insts.begin() + problem, int64_t(0),
+                                           [](int64_t a, const Inst1D& x) { return a + x.N; });
+        launch_fields1d(st, static_cast<int>(groups.size()), plan->insts.p + problem, plan->stacks.p, plan->layouts.p,
+                        plan->ws.p, plan->fields.p, reinterpret_cast<const cd*>(plan->R.p) + oo,
+                        reinterpret_cast<const cd*>(plan->T.p) + oo, d_groups.p, d_cols.p, d_zs.p, nz, H.p, d_scr.p);
+    }
     hip_check(hipGetLastError(), "kernel launch");
     hip_check(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
@@ -1352,59 +1410,98 @@ bool all_finite(const std::vector<double>& v) {
         if (!std::isfinite(x)) return false;
     return true;
 }
+
+const char* unavailable_msg(const srcwa_plan* plan) {
+    return plan->kind == 2 ? "rcwa: conical field context is unavailable" : "rcwa: planar field context is unavailable";
+}
+const char* nonfinite_msg(const srcwa_plan* plan) {
+    return plan->kind == 2 ? "rcwa: conical field propagation produced nonfinite values"
+                           : "rcwa: planar field propagation produced nonfinite values";
+}
+
+// field_at of either kind; jones: conical only
+void field_at_impl(srcwa_plan* plan, int problem, int nz, const double* zs, const double* jones, double* out,
+                   const char* what, int kind) {
+    field_plan_checks(plan, problem, what, kind);
+    if (!field_ready(plan, problem)) throw Failure{SRCWA_EUNAVAILABLE, unavailable_msg(plan)};
+    cd j0{1.0, 0.0}, j1{0.0, 0.0};
+    if (kind == 2) normalize_jones(jones, j0, j1);
+    if (nz < 0 || (nz > 0 && (!zs || !out))) invalid(std::string(what) + ": bad depth arguments");
+    if (nz == 0) return;
+    DevBuf<cd> H;
+    H.pool = plan->ctx->pool;
+    sample_fields(plan, problem, nz, zs, H, j0, j1);
+    const size_t words = 12 * size_t(plan->h_insts[problem].N) * nz;
+    std::vector<double> h(words);
+    copy_out(h.data(), H.p, words * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
+    if (!all_finite(h)) throw Failure{SRCWA_ENUMERICAL, nonfinite_msg(plan)};
+    std::memcpy(out, h.data(), words * sizeof(double));
+}
+
+void field_map_impl(srcwa_plan* plan, int problem, int nz, const double* zs, int nx, double y, const double* jones,
+                    double* out, const char* what, int kind) {
+    if (nx <= 0) invalid("rcwa: field nx must be positive");
+    if (!std::isfinite(y)) invalid("rcwa: field y must be finite");
+    field_plan_checks(plan, problem, what, kind);
+    if (!field_ready(plan, problem)) throw Failure{SRCWA_EUNAVAILABLE, unavailable_msg(plan)};
+    cd j0{1.0, 0.0}, j1{0.0, 0.0};
+    if (kind == 2) normalize_jones(jones, j0, j1);
+    if (nz < 0 || (nz > 0 && (!zs || !out))) invalid(std::string(what) + ": bad depth arguments");
+    if (nz == 0) return;
+    DevBuf<cd> H;
+    H.pool = plan->ctx->pool;
+    sample_fields(plan, problem, nz, zs, H, j0, j1);
+    const Inst1D& in = plan->h_insts[problem];
+    const int N = in.N;
+    std::vector<double> h(12 * size_t(N) * nz);
+    copy_out(h.data(), H.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
+    if (!all_finite(h)) throw Failure{SRCWA_ENUMERICAL, nonfinite_msg(plan)};
+    // the transverse phase ky y of conical solves (the field metadata's ky = k0 nI sin(theta) sin(phi))
+    double kyy = 0.0;
+    if (kind == 2) {
+        const double* st = plan->h_stack_words.data() + in.stack_off;
+        const double k0 = 2.0 * 3.14159265358979323846 / in.wavelength;
+        const double ky = st[1] * std::sin(in.theta_deg * 3.14159265358979323846 / 180.0) *
+                          std::sin(in.phi_deg * 3.14159265358979323846 / 180.0);
+        kyy = (k0 * ky) * y;
+    }
+    DevBuf<cd> G;
+    G.pool = plan->ctx->pool;
+    G.alloc(static_cast<size_t>(6 * int64_t(nx) * nz));
+    launch_fieldmap1d(plan->ctx->stream, plan->insts.p + problem, plan->stacks.p, plan->layouts.p, plan->ws.p, H.p,
+                      nz, nx, kyy, G.p);
+    hip_check(hipGetLastError(), "kernel launch");
+    std::vector<double> g(12 * size_t(nx) * nz);
+    copy_out(g.data(), G.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
+    if (!all_finite(g)) throw Failure{SRCWA_ENUMERICAL, "rcwa: field reconstruction produced nonfinite values"};
+    std::memcpy(out, g.data(), g.size() * sizeof(double));
+}
 }  // namespace
 
 int srcwa_plan1d_field_available(srcwa_plan* plan, int problem, int* available) {
     return guarded([&] {
-        field_plan_checks(plan, problem, "srcwa_plan1d_field_available");
+        field_plan_checks(plan, problem, "srcwa_plan1d_field_available", 0);
         if (!available) invalid("srcwa_plan1d_field_available: null output");
         *available = field_ready(plan, problem) ? 1 : 0;
     });
 }
 
 int srcwa_plan1d_field_at(srcwa_plan* plan, int problem, int nz, const double* zs, double* out) {
-    return guarded([&] {
-        field_plan_checks(plan, problem, "srcwa_plan1d_field_at");
-        if (!field_ready(plan, problem)) throw Failure{SRCWA_EUNAVAILABLE, "rcwa: planar field context is unavailable"};
-        if (nz < 0 || (nz > 0 && (!zs || !out))) invalid("srcwa_plan1d_field_at: bad depth arguments");
-        if (nz == 0) return;
-        DevBuf<cd> H;
-        H.pool = plan->ctx->pool;
-        sample_fields(plan, problem, nz, zs, H);
-        const size_t words = 12 * size_t(plan->h_insts[problem].N) * nz;
-        std::vector<double> h(words);
-        copy_out(h.data(), H.p, words * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
-        if (!all_finite(h)) throw Failure{SRCWA_ENUMERICAL, "rcwa: planar field propagation produced nonfinite values"};
-        std::memcpy(out, h.data(), words * sizeof(double));
-    });
+    return guarded([&] { field_at_impl(plan, problem, nz, zs, nullptr, out, "srcwa_plan1d_field_at", 1); });
 }
 
 int srcwa_plan1d_field_map(srcwa_plan* plan, int problem, int nz, const double* zs, int nx, double y, double* out) {
-    return guarded([&] {
-        if (nx <= 0) invalid("rcwa: field nx must be positive");
-        if (!std::isfinite(y)) invalid("rcwa: field y must be finite");
-        field_plan_checks(plan, problem, "srcwa_plan1d_field_map");
-        if (!field_ready(plan, problem)) throw Failure{SRCWA_EUNAVAILABLE, "rcwa: planar field context is unavailable"};
-        if (nz < 0 || (nz > 0 && (!zs || !out))) invalid("srcwa_plan1d_field_map: bad depth arguments");
-        if (nz == 0) return;
-        DevBuf<cd> H;
-        H.pool = plan->ctx->pool;
-        sample_fields(plan, problem, nz, zs, H);
-        const int N = plan->h_insts[problem].N;
-        std::vector<double> h(12 * size_t(N) * nz);
-        copy_out(h.data(), H.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
-        if (!all_finite(h)) throw Failure{SRCWA_ENUMERICAL, "rcwa: planar field propagation produced nonfinite values"};
-        DevBuf<cd> G;
-        G.pool = plan->ctx->pool;
-        G.alloc(static_cast<size_t>(6 * int64_t(nx) * nz));
-        launch_fieldmap1d(plan->ctx->stream, plan->insts.p + problem, plan->stacks.p, plan->layouts.p, plan->ws.p, H.p,
-                          nz, nx, G.p);
-        hip_check(hipGetLastError(), "kernel launch");
-        std::vector<double> g(12 * size_t(nx) * nz);
-        copy_out(g.data(), G.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, "D2H", plan);
-        if (!all_finite(g)) throw Failure{SRCWA_ENUMERICAL, "rcwa: field reconstruction produced nonfinite values"};
-        std::memcpy(out, g.data(), g.size() * sizeof(double));
-    });
+    return guarded([&] { field_map_impl(plan, problem, nz, zs, nx, y, nullptr, out, "srcwa_plan1d_field_map", 1); });
+}
+
+int srcwa_plan_conical_field_at(srcwa_plan* plan, int problem, int nz, const double* zs, const double* jones,
+                                double* out) {
+    return guarded([&] { field_at_impl(plan, problem, nz, zs, jones, out, "srcwa_plan_conical_field_at", 2); });
+}
+
+int srcwa_plan_conical_field_map(srcwa_plan* plan, int problem, int nz, const double* zs, int nx, double y,
+                                 const double* jones, double* out) {
+    return guarded([&] { field_map_impl(plan, problem, nz, zs, nx, y, jones, out, "srcwa_plan_conical_field_map", 2); });
 }
 
 int64_t srcwa_plan_orders(const srcwa_plan* plan) { return plan ? plan->total_orders : -1; }
@@ -1895,7 +1992,7 @@ static int plan2d_create_impl(srcwa_context* ctx, int n_stacks, const double* st
         plan->n_inst = n_problems;
         plan->flags = ctx->plan_flags;
         if (plan->flags & PLAN_FIELDS)
-            throw Failure{SRCWA_EUNSUPPORTED, "rcwa: near fields are captured for planar (TE/TM) plans only in this build"};
+            throw Failure{SRCWA_EUNSUPPORTED, "rcwa: near fields are captured for planar and conical plans only in this build"};
         const bool diag = plan->flags & PLAN_DIAGNOSTICS;
         const int64_t budget = max_workspace_bytes > 0 ? max_workspace_bytes : (int64_t(64) << 30);
         const int64_t budget_words = budget / int64_t(sizeof(cd));

@@ -286,6 +286,27 @@ class ConicalBatch(PlanarBatch):
                              DEr=res["DEr"][a:b].copy(), DEt=res["DEt"][a:b].copy(),
                              eig_info=int(res["eig_info"][i]), phi_m_deg=g[:, 7].copy())
 
+    # ---- near fields (a context created with plan_flags=PLAN_FIELDS; rcwa1d_conical_fields.hpp:56-262) ----
+    def field_at(self, i: int, zs, jones=(1.0, 0.0)) -> np.ndarray:
+        """Harmonic fields of problem i at the depths zs for the incident Jones vector (Es, Ep), normalized first:
+        array [6, N, nz] complex (Ex, Ey, Ez, Hx, Hy, Hz; H is Z0 H)."""
+        z = np.ascontiguousarray(np.atleast_1d(np.asarray(zs, dtype=np.float64)))
+        j = np.array([complex(jones[0]).real, complex(jones[0]).imag, complex(jones[1]).real, complex(jones[1]).imag])
+        N = int(self.N[i])
+        out = np.zeros(12 * N * z.size)
+        _lib.check(self.lib.srcwa_plan_conical_field_at(self.handle, int(i), int(z.size), _lib.ptr(z), _lib.ptr(j),
+                                                        _lib.ptr(out)))
+        return out.view(np.complex128).reshape(6, z.size, N).transpose(0, 2, 1)
+
+    def field_map(self, i: int, zs, nx: int = 256, y: float = 0.0, jones=(1.0, 0.0)) -> np.ndarray:
+        """The fields of problem i on the grid x_k = k period / nx at height y: array [6, nz, nx] complex."""
+        z = np.ascontiguousarray(np.atleast_1d(np.asarray(zs, dtype=np.float64)))
+        j = np.array([complex(jones[0]).real, complex(jones[0]).imag, complex(jones[1]).real, complex(jones[1]).imag])
+        out = np.zeros(12 * max(int(nx), 0) * z.size)
+        _lib.check(self.lib.srcwa_plan_conical_field_map(self.handle, int(i), int(z.size), _lib.ptr(z), int(nx),
+                                                         float(y), _lib.ptr(j), _lib.ptr(out)))
+        return out.view(np.complex128).reshape(6, z.size, max(int(nx), 0))
+
 
 def solve_conical(stack: Stack, wavelength: float, theta_deg: float, phi_deg: float, M: int,
                   ctx: _lib.Context | None = None) -> ConicalResult:
@@ -310,7 +331,6 @@ def sweep_conical(stack: Stack, points: Sequence[ConicalPoint], M: int,
 
 # ---- crossed gratings ------------------------------------------------------------------------------
 _STATUS2D = {ST_OK: Status2D.Ok, ST_NUMERICAL: Status2D.NumericalFailure, ST_RAYLEIGH: Status2D.RayleighAnomaly}
-
 
 class CrossedBatch(PlanarBatch):
     """A device-resident batch of crossed-grating solves.

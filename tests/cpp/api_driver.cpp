@@ -9,6 +9,8 @@
 //   crossed <M1> <M2> <wavelength> <theta> <phi> <Es.re> <Es.im> <Ep.re> <Ep.im> <laurent> <n> <flat record>
 //   fields  <pol> <M> <wavelength> <theta> <nz> <z>* <nx> <y> <n> <flat record>   (solve_with_fields, field_at,
 //           field_map; prints the 6 harmonic components order-major per depth, then the 6 grid components)
+//   conical_fields <M> <wavelength> <theta> <phi> <Es.re> <Es.im> <Ep.re> <Ep.im> <nz> <z>* <nx> <y> <n> <record>
+//           (solve_conical_with_fields, field_at and field_map for that Jones vector)
 //
 // Output per problem: "ok" and labelled arrays, or "invalid_argument <message>" / "unsupported <message>".
 #include <cstdio>
@@ -223,6 +225,29 @@ int main() {
                     } catch (const std::logic_error& e) {
                         std::printf("logic_error %s\n", e.what());
                     }
+                }
+            } else if (kind == "conical_fields") {
+                int M, nz, nx;
+                double wl, th, ph, a, b, c, d, y;
+                in >> M >> wl >> th >> ph >> a >> b >> c >> d >> nz;
+                srcwa_amd::RealVector zs(nz);
+                for (double& z : zs) in >> z;
+                in >> nx >> y;
+                const auto st = planar_stack(read_record(in));
+                const auto res = rcwa::solve_conical_with_fields(st, wl, th, ph, M);
+                std::printf("ok %d %zu %d\n", res.result.eig_info, res.result.kx.size(), res.fields.available() ? 1 : 0);
+                if (res.fields.available()) {
+                    const rcwa::JonesVector jones{{cplx{a, b}, cplx{c, d}}};
+                    const auto h = rcwa::field_at(res, zs, jones);
+                    const auto g = rcwa::field_map(res.fields, zs, nx, y, jones);
+                    std::vector<cplx> hv, gv;
+                    for (int k = 0; k < 6; ++k) {
+                        hv.insert(hv.end(), h.components[k].data(), h.components[k].data() + h.components[k].rows() * h.components[k].cols());
+                        gv.insert(gv.end(), g.components[k].data(), g.components[k].data() + g.components[k].rows() * g.components[k].cols());
+                    }
+                    put("harm", hv);
+                    put("grid", gv);
+                    std::printf("ky %.17g\n", h.metadata.ky);
                 }
             } else {
                 std::printf("invalid_argument unknown problem kind\n");

@@ -188,16 +188,20 @@ def test_python_solve_with_fields(ref_lib):
         f2.field_at([0.0])
 
 
-def test_field_plans_must_fit_one_chunk_and_be_planar(fctx):
+def test_field_plans_must_fit_one_chunk_and_be_planar_or_conical(fctx):
     """The samplers read the modes where the solve left them: a field plan is refused when its workspace would
-    be split into chunks; conical plans capture no fields in this build."""
+    be split into chunks; crossed-grating plans capture no fields in this build."""
     from stablercwa_amd import rcwa
     st = mirror_stack()
     probs = [(0, 1.0 + 0.01 * i, 66.36, 20, 0) for i in range(4)]
     with pytest.raises(ValueError, match="one workspace chunk"):
         rcwa.PlanarBatch([st], probs, fctx, max_workspace_bytes=1 << 20)
+    with pytest.raises(ValueError, match="one workspace chunk"):
+        rcwa.ConicalBatch([st], [(0, 1.0 + 0.01 * i, 40., 30., 20) for i in range(4)], fctx, max_workspace_bytes=1 << 20)
+    from stablercwa_amd.types import Layer2D, Stack2D
     with pytest.raises(NotImplementedError):
-        rcwa.ConicalBatch([lossy_stack()], [(0, .83, 27., 31., 3)], fctx)
+        rcwa.CrossedBatch([Stack2D(nI=1.0, nII=1.45, layers=[Layer2D.uniform(0.1, 1.5)], Lx=1.0, Ly=1.0)],
+                          [(0, 1.3, 10., 20., 1, 1, (1.0, 0.0))], fctx)
 
 
 def test_fields_on_a_pooled_context(ref_lib):
@@ -263,3 +267,96 @@ def test_fields_edge_stacks(fctx, ref_lib):
             _close(b.field_at(0, zs), o["harm"], f"edge harmonics L={len(st.layers)} nz={len(zs)}")
             _close(b.field_map(0, zs, 5, .3), o["grid"], "edge grid")
             b.close()
+
+
+# ---- conical near fields (srcwa_plan_conical_field_at / _field_map) ----------------------------------------
+CONICAL_CASES = [
+    ("reference field stack", lossy_stack, .83, 27., 31., 3, ZS),
+    ("grating on a mirror, M=20", mirror_stack, 1.053, 40., 30., 20, np.linspace(-0.2, 3.6, 25)),
+    ("pure films", films_stack, .9, 35., 20., 4, np.array([-0.1, 0.0, 0.06, 0.12, 0.25, 0.32, 0.39, 0.5])),
+    ("FFT grating", fft_stack, .8, 12., 50., 6, np.array([-0.05, 0.1, 0.25, 0.3, 0.35, 0.6])),
+]
+JONES = [(1.0, 0.0), (0.0, 1.0), (0.3 + 0.2j, -0.7 + 0.5j)]
+
+
+def _conical_plan(ctx, stack, problems):
+    from stablercwa_amd import rcwa
+    b = rcwa.ConicalBatch([stack], [(0,) + p for p in problems], ctx)
+    b.execute()
+    b.sync()
+    return b
+
+
+@pytest.mark.parametrize("case", CONICAL_CASES, ids=[c[0] for c in CONICAL_CASES])
+def test_conical_fields_match_reference(fctx, ref_lib, case):
+    from oracle import ref
+    name, make, lam, theta, phi, M, zs = case
+    st = make()
+    b = _conical_plan(fctx, st, [(lam, theta, phi, M)])
+    assert b.field_available(0)
+    for jones in JONES:
+        o = ref.conical_fields(ref_lib, st.flat(), lam, theta, phi, M, jones, zs, nx=9, y=.37)
+        assert o["available"] and o["n_schur"] == 0
+        _close(b.field_at(0, zs, jones), o["harm"], f"conical {name} {jones} harmonics")
+        _close(b.field_map(0, zs, 9, .37, jones), o["grid"], f"conical {name} {jones} grid")
+    b.close()
+
+
+def test_conical_fields_zero_thickness_layers_and_batches(fctx, ref_lib):
+    """Zero-thickness layers hold no fields (skipped by the depth groups); one problem of a batch samples the
+    same as alone; the GPU fields conserve flux through a lossless stack."""
+    from oracle import ref
+    from stablercwa_amd.types import Layer, Stack
+    st = Stack(1.0, [Layer.grating(0.2, [0.0, 0.3, 0.6], [1.8, 1.2]), Layer.uniform(0.0, 2.2),
+                     Layer.grating(0.0, [0.0, 0.1, 0.6], [1.4, 2.0]), Layer.uniform(0.15, 1.5)], 1.45, 0.6)
+    zs = np.array([-0.1, 0.0, 0.1, 0.2, 0.2 + 1e-13, 0.25, 0.35, 0.5])
+    alone = _conical_plan(fctx, st, [(.9, 20., 35., 4)])
+    many = _conical_plan(fctx, st, [(.8, 10., 5., 3), (.9, 20., 35., 4)])
+    o = ref.conical_fields(ref_lib, st.flat(), .9, 20., 35., 4, (0.6, 0.8j), zs)
+    h = alone.field_at(0, zs, (0.6, 0.8j))
+    _close(h, o["harm"], "conical zero-thickness layers")
+    assert np.array_equal(h, many.field_at(1, zs, (0.6, 0.8j)))
+    b = _conical_plan(fctx, lossless_stack(), [(.83, 27., 31., 3)])
+    S = flux(b.field_at(0, ZS, (0.3 + 0.2j, -0.7 + 0.5j)))
+    assert np.max(np.abs(S / S[0] - 1.0)) <= 1e-10
+
+
+def test_conical_field_contract(fctx):
+    """A degenerate (Schur-basis) layer leaves this build's conical context unavailable while the solve itself
+    succeeds; Jones vectors that are zero or not finite and non-finite depths are invalid arguments; the planar
+    entry points refuse a conical plan."""
+    from conftest import load_golden
+    from stablercwa_amd import _lib
+    from stablercwa_amd.types import Stack
+    c = [c for c in load_golden("conical_ref.json")["cases"] if c["n_schur"] > 0][0]
+    b = _conical_plan(fctx, Stack.from_flat(np.array(c["stack"])), [(c["wavelength"], c["theta"], c["phi"], c["M"])])
+    assert b.results()["status"][0] == 0 and not b.field_available(0)
+    with pytest.raises(_lib.FieldUnavailable):
+        b.field_at(0, [0.1])
+    ok = _conical_plan(fctx, lossy_stack(), [(.83, 27., 31., 3)])
+    for bad in [(0.0, 0.0), (np.inf, 1.0), (1.0, complex(0, np.nan))]:
+        with pytest.raises(ValueError):
+            ok.field_at(0, [0.1], bad)
+    with pytest.raises(ValueError):
+        ok.field_at(0, [np.nan])
+    with pytest.raises(ValueError):
+        ok.field_map(0, ZS, 0)
+    out = np.zeros(12 * 7)
+    z = np.array([0.1])
+    with pytest.raises(ValueError):
+        _lib.check(ok.lib.srcwa_plan1d_field_at(ok.handle, 0, 1, _lib.ptr(z), _lib.ptr(out)))
+
+
+def test_cpp_solve_conical_with_fields_matches_reference(ref_lib):
+    from oracle import ref
+    from test_gpu_cpp_api import _drive, _fields, _rec
+    st = lossy_stack()
+    es, ep = 0.3 + 0.2j, -0.7 + 0.5j
+    line = (f"conical_fields 3 0.83 27.0 31.0 {es.real!r} {es.imag!r} {ep.real!r} {ep.imag!r} {len(ZS)} "
+            + " ".join(repr(float(z)) for z in ZS) + f" 9 0.37 {_rec(st.flat())}")
+    block, = _drive([line])
+    f = _fields(block)
+    assert f["head"][0] == "ok" and f["head"][3] == "1", block[0]
+    o = ref.conical_fields(ref_lib, st.flat(), .83, 27., 31., 3, (es, ep), ZS, nx=9, y=.37)
+    _close(f["harm"].view(np.complex128).reshape(6, len(ZS), 7).transpose(0, 2, 1), o["harm"], "C++ conical harmonics")
+    _close(f["grid"].view(np.complex128).reshape(6, len(ZS), 9), o["grid"], "C++ conical grid")

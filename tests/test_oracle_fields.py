@@ -114,3 +114,54 @@ def test_planted_amplitude_swap_breaks_the_flux_invariant(ref_lib):
     swapped = am * np.exp(-k0 * d * q) + ap
     assert np.max(np.abs(good - Ey[:, 1])) <= 1e-9 * np.max(np.abs(Ey))
     assert np.max(np.abs(swapped - Ey[:, 1])) > 1e-3 * np.max(np.abs(Ey))
+
+
+# ---- conical near fields (rcwa1d_conical_fields.hpp) -------------------------------------------------------
+CONICAL_ZS = np.array([-0.3, -1e-9, 0.0, 0.1, 0.2 - 1e-10, 0.2 + 1e-10, 0.25, 0.29 - 1e-10, 0.29 + 1e-10, 0.42, 0.6])
+
+
+def test_conical_flux_is_conserved_for_any_jones_vector(ref_lib):
+    from oracle import ref
+    for jones in [(1, 0), (0, 1), (0.3 + 0.2j, -0.7 + 0.5j)]:
+        o = ref.conical_fields(ref_lib, lossless_stack().flat(), .83, 27., 31., 3, jones, CONICAL_ZS)
+        assert o["available"]
+        S = flux(o["harm"])
+        assert np.max(np.abs(S / S[0] - 1.0)) <= 1e-10, (jones, S / S[0])
+
+
+def test_conical_fields_are_linear_in_the_jones_vector(ref_lib):
+    """The capture holds the s and p responses; the fields of (Es, Ep) / |(Es, Ep)| are that combination."""
+    from oracle import ref
+    st = lossy_stack().flat()
+    hs = ref.conical_fields(ref_lib, st, .83, 27., 31., 3, (1, 0), CONICAL_ZS)["harm"]
+    hp = ref.conical_fields(ref_lib, st, .83, 27., 31., 3, (0, 1), CONICAL_ZS)["harm"]
+    es, ep = 0.3 + 0.2j, -0.7 + 0.5j
+    nrm = np.hypot(abs(es), abs(ep))
+    h = ref.conical_fields(ref_lib, st, .83, 27., 31., 3, (es, ep), CONICAL_ZS)["harm"]
+    want = (es * hs + ep * hp) / nrm
+    assert np.max(np.abs(h - want)) <= 1e-12 * np.max(np.abs(want))
+
+
+def test_conical_tangential_fields_are_continuous(ref_lib):
+    from oracle import ref
+    zs = [s + e for s in (0.0, 0.2, 0.29, 0.42) for e in (-1e-10, 1e-10)]
+    h = ref.conical_fields(ref_lib, lossy_stack().flat(), .83, 27., 31., 3, (0.6, 0.8j), zs)["harm"]
+    for c in (0, 1, 3, 4):            # Ex, Ey, Hx, Hy
+        above, below = h[c][:, 0::2], h[c][:, 1::2]
+        assert np.max(np.abs(above - below)) <= 1e-7 * max(1.0, np.max(np.abs(above)))
+
+
+def test_conical_grid_carries_the_ky_phase(ref_lib):
+    """field_map at height y: sum_j exp(-i (kxm_j x + ky y)) h_j with ky = k0 nI sin(theta) sin(phi)."""
+    from oracle import ref
+    lam, th, ph, M, nx, y = .83, 27., 31., 3, 9, .37
+    o = ref.conical_fields(ref_lib, lossy_stack().flat(), lam, th, ph, M, (1, 0), CONICAL_ZS, nx=nx, y=y)
+    k0 = 2 * np.pi / lam
+    s = 1.2 * np.sin(np.radians(th))
+    kxm = k0 * (s * np.cos(np.radians(ph)) - np.arange(-M, M + 1) * lam / .6)
+    ky = k0 * s * np.sin(np.radians(ph))
+    x = np.arange(nx) / nx * .6
+    carrier = np.exp(-1j * (np.outer(x, kxm) + ky * y))
+    for c in range(6):
+        want = (carrier @ o["harm"][c]).T
+        assert np.max(np.abs(o["grid"][c] - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
