@@ -188,8 +188,9 @@ def single_solve_latency(reps=40):
 
 
 def near_fields_leg(nz=256, nx=256, reps=5):
-    """Planar near fields (solve_with_fields + field_at + field_map): the grating on the 20-film mirror at M = 60,
-    lambda 1.053, theta 66.36, TE and TM, sampled at nz depths through the whole stack and on an nx x nz grid;
+    """Near fields (solve_with_fields / solve_conical_with_fields + field_at + field_map): the grating on the
+    20-film mirror at M = 60, lambda 1.053, theta 66.36 (planar TE and TM) and theta 40, phi 30 (conical, one Jones
+    vector), sampled at nz depths into the stack and on an nx x nz grid;
     GPU time per call (capture kept on the device) and the reference's own field_at / field_map on one thread,
     with the largest field difference between them."""
     from stablercwa_amd import _lib, rcwa
@@ -219,6 +220,30 @@ def near_fields_leg(nz=256, nx=256, reps=5):
                                                      float(np.max(np.abs(g - o["grid"])))) / scale
         b.close()
         out["TE" if pol == 0 else "TM"] = leg
+    # conical: the same stack at phi = 30 degrees, one Jones vector per call
+    b = rcwa.ConicalBatch([st], [(0, 1.053, 40.0, 30.0, 60)], ctx)
+    b.execute()
+    b.sync()
+    jones = (0.6, 0.8j)
+    h = b.field_at(0, zs, jones)
+    ta, tm = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter(); h = b.field_at(0, zs, jones); ta.append(time.perf_counter() - t0)
+        t0 = time.perf_counter(); g = b.field_map(0, zs, nx, 0.0, jones); tm.append(time.perf_counter() - t0)
+    leg = dict(field_at_ms=float(np.median(ta) * 1e3), field_map_ms=float(np.median(tm) * 1e3),
+               point="theta 40, phi 30, Jones (0.6, 0.8i)")
+    from oracle import ref
+    if ref.available():
+        lib = ref.load()
+        lib.ref_set_threads(1)
+        t0 = time.perf_counter()
+        o = ref.conical_fields(lib, st.flat(), 1.053, 40.0, 30.0, 60, jones, zs, nx=nx)
+        leg["reference_solve_field_at_field_map_ms_1thread"] = (time.perf_counter() - t0) * 1e3
+        scale = max(1.0, float(np.max(np.abs(o["harm"]))))
+        leg["max_abs_dfield_vs_reference"] = max(float(np.max(np.abs(h - o["harm"]))),
+                                                 float(np.max(np.abs(g - o["grid"])))) / scale
+    b.close()
+    out["conical"] = leg
     return out
 
 
