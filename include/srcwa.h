@@ -211,9 +211,9 @@ int srcwa_plan1d_field_map(srcwa_plan* plan, int problem, int nz, const double* 
  * incident Jones vector jones = [Es.re, Es.im, Ep.re, Ep.im] (normalized first, rcwa_fields.hpp:218-230): as
  * srcwa_plan1d_field_at / _field_map (ConicalFieldContext, field_at, field_map,
  * rcwa1d_conical_fields.hpp:14-262), with the grid's transverse factor exp(-i (kxm x + ky y)).  Layers of
- * zero thickness hold no fields (the depth groups skip them).  srcwa_plan1d_field_available serves conical
- * plans too; a solve with a layer in the Schur basis (a degenerate layer) keeps no capture in this build, so
- * its context is unavailable.  A Jones vector that is not finite or is zero: SRCWA_EINVAL. */
+ * zero thickness hold no fields (the depth groups skip them); degenerate layers are sampled in their Schur
+ * basis (exp(-k0 z G) per depth).  srcwa_plan1d_field_available serves conical plans too.  A Jones vector
+ * that is not finite or is zero: SRCWA_EINVAL. */
 int srcwa_plan_conical_field_at(srcwa_plan* plan, int problem, int nz, const double* zs, const double* jones,
                                 double* out);
 int srcwa_plan_conical_field_map(srcwa_plan* plan, int problem, int nz, const double* zs, int nx, double y,

@@ -369,8 +369,8 @@ struct ConicalFieldAccess {
 };
 }  // namespace detail
 
-// The capture of one conical solve for both incident polarizations; a Jones vector is chosen per call.  A
-// solve with a degenerate (Schur-basis) layer leaves the context Unavailable in this build.
+// The capture of one conical solve for both incident polarizations; a Jones vector is chosen per call.
+// Degenerate layers are sampled in their Schur basis.
 class ConicalFieldContext {
 public:
     ConicalFieldContext() = default;

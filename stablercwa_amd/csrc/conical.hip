@@ -102,6 +102,14 @@ DEV LayerView layer_at(const double* st, int index) {
     return lv;
 }
 
+// Field plans (PLAN_FIELDS): the Schur generator G (2N x 2N) of a degenerate layer, kept for sampling at any
+// depth, after the layer's slots, diagnostics copies and FFT samples (the host sizes the slot to match).
+DEV cd* schur_generator_keep(cd* ws, const int64_t* lay, const Inst1D& in, int layer, const LayerView& lv) {
+    const int64_t NN = (int64_t)in.N * in.N;
+    const int64_t fft = (!lv.homogeneous && lv.method == 1) ? 2 * (int64_t)fft_samples_for(in.M, lv.samples) : 0;
+    return ws + lay[conical_slot(in.L, layer)] + conical_slot_words(in.N) + ((in.flags & PLAN_DIAGNOSTICS) ? 4 * NN : 0) + fft;
+}
+
 // A conical operator set (layout: conical_ops_words): [[eps]], [[eps]]^-1, C = [[1/eps]]^-1.
 struct ConicalOps {
     cd *E, *IE, *C, *Z, *coef, *F;
@@ -558,6 +566,10 @@ DEV int schur_modes(const Inst1D& in, const double* st, const int64_t* lay, cd* 
     const double scale = wg_norm1(n, E1), err = wg_norm1(n, E4);
     if (!isfinite(scale) || !isfinite(err) || err > 1e-10 * fmax(scale, 2.2250738585072014e-308)) return 1;
     for (int i = threadIdx.x; i < n; i += NT) q[i] = E3[i + (int64_t)i * n];
+    if (in.flags & PLAN_FIELDS) {
+        cd* keep = schur_generator_keep(ws, lay, in, layer, lv);
+        for (int64_t e = threadIdx.x; e < nn; e += NT) keep[e] = E3[e];
+    }
     __syncthreads();
     // V = Q Z G^-1 (right triangular solve, one thread per row)
     wg_gemm<GEMM_SET, false>(n, n, n, E2, n, s.W, n, s.V, n);
@@ -795,7 +807,7 @@ __global__ SMAT_KERNEL_BOUNDS_OF(SRCWA_SMAT_OCC_CONICAL) void k_smatrix_conical(
         status = c.fail ? (c.fail == 2 ? ST_WORKSPACE : ST_NUMERICAL) : ST_OK;
         if (FIELDS && status == ST_OK) {
             // the outer responses (before the reflected p sign) and the capture's validity
-            // (rcwa1d_conical.hpp:232-241, 957-970); a Schur-basis layer is not captured by this build
+            // (rcwa1d_conical.hpp:232-241, 957-970); a Schur-basis layer's generator G is kept by its modes kernel
             cd* resp = fields + 4 * (int64_t)L * n;
             int bad = 0;
             for (int e = threadIdx.x; e < 2 * n; e += NT) {
@@ -805,9 +817,17 @@ __global__ SMAT_KERNEL_BOUNDS_OF(SRCWA_SMAT_OCC_CONICAL) void k_smatrix_conical(
                 bad |= !finite(resp[e]) || !finite(resp[2 * n + e]);
             }
             for (int64_t e = threadIdx.x; e < 4 * (int64_t)L * n; e += NT) bad |= !finite(fields[e]);
+            cd* marks = fields + 4 * (int64_t)(L + 1) * n;       // 1 for a Schur-basis layer, then its G is kept
             for (int r = 0; r < L + 2; ++r) {
                 const Region g = region(r);
-                if (g.X != nullptr) bad = 1;
+                if (r >= 1 && r <= L) {
+                    if (threadIdx.x == 0) marks[r - 1] = cd{g.X != nullptr ? 1.0 : 0.0, 0.0};
+                    if (g.X != nullptr) {
+                        const LayerView lv = layer_at(st, (int)lay[conical_orig(L, r - 1)]);
+                        const cd* G = schur_generator_keep(ws, lay, in, r - 1, lv);
+                        for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += NT) bad |= !finite(G[e]);
+                    }
+                }
                 for (int i = threadIdx.x; i < n; i += NT) {
                     bad |= !finite(g.q[i]);
                     if (g.analytic) bad |= !finite(g.w[i]) || !finite(g.p[i]);
@@ -925,6 +945,60 @@ __global__ __launch_bounds__(NT) void k_fields_conical(const Inst1D* __restrict_
         const double d = g.d;
         const cd* w = sb < 0 ? ws + lay[conical_w(r)] : nullptr;
         const cd* v = sb < 0 ? ws + lay[conical_v(r)] : nullptr;
+        const cd* marks = fields + in.field_off + 4 * (int64_t)(L + 1) * n;
+        if (marks[j].x != 0.0) {
+            // a Schur-basis layer: p = exp(-k0 z G) a, m = exp(-k0 (d - z) G) b, one triangular exponential per
+            // depth and side (field_schur_action, rcwa1d_conical_fields.hpp:106-156)
+            const int64_t nn = (int64_t)n * n;
+            cd* av = E2 + NC;
+            cd* bv = av + n;
+            cd* dg = bv + n;
+            cd* S1 = dg + n;
+            cd* S2 = S1 + nn;
+            cd* S3 = S2 + nn;
+            cd* S4 = S3 + nn;
+            const LayerView lv = layer_at(st, (int)lay[conical_orig(L, j)]);
+            const cd* Gm = schur_generator_keep(const_cast<cd*>(ws), lay, in, j, lv);
+            int live = 0;
+            for (int i = threadIdx.x; i < n; i += NT) {
+                av[i] = ap[i] * j0 + ap[n + i] * j1;
+                bv[i] = am[i] * j0 + am[n + i] * j1;
+                live |= (av[i].x != 0.0 || av[i].y != 0.0 ? 1 : 0) | (bv[i].x != 0.0 || bv[i].y != 0.0 ? 2 : 0);
+            }
+            int any = 0;
+            for (int bit = 1; bit <= 2; bit <<= 1) any |= block_max_int(live & bit);
+            int fail = 0;
+            for (int col = 0; col < cnt; ++col) {
+                const double from_top = fmax(0.0, fmin(d, zs[cols[g.first + col]] - g.z_top));
+                for (int side = 0; side < 2; ++side) {
+                    const cd* anc = side ? bv : av;
+                    cd* dst = (side ? Q : P) + (int64_t)col * n;
+                    if (!(any & (side ? 2 : 1))) {
+                        for (int i = threadIdx.x; i < n; i += NT) dst[i] = cd{0.0, 0.0};
+                        __syncthreads();
+                        continue;
+                    }
+                    const cd f{-k0 * (side ? d - from_top : from_top), 0.0};
+                    for (int64_t e = threadIdx.x; e < nn; e += NT) S1[e] = Gm[e] * f;
+                    __syncthreads();
+                    for (int i = threadIdx.x; i < n; i += NT) dg[i] = S1[i + (int64_t)i * n];
+                    __syncthreads();
+                    fail |= wg_triangular_exp(n, S1, dg, S2, S3, S4);
+                    for (int i = threadIdx.x; i < n; i += NT) {
+                        cd acc{0.0, 0.0};
+                        for (int k = 0; k < n; ++k) acc = acc + S2[i + (int64_t)k * n] * anc[k];
+                        dst[i] = acc;
+                    }
+                    __syncthreads();
+                }
+            }
+            const double nan = fail ? __builtin_nan("") : 0.0;
+            for (int64_t e = threadIdx.x; e < Nc; e += NT) {
+                const cd pp = P[e], mm = Q[e];
+                P[e] = fail ? cd{nan, nan} : pp + mm;
+                Q[e] = fail ? cd{nan, nan} : mm - pp;
+            }
+        } else
         for (int64_t e = threadIdx.x; e < Nc; e += NT) {
             const int i = (int)(e % n), col = (int)(e / n);
             const double from_top = fmax(0.0, fmin(d, zs[cols[g.first + col]] - g.z_top));

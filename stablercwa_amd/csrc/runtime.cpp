@@ -1034,7 +1034,8 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
             for (int j = 0; j < L; ++j)
                 if (general[j + 1]) {
                     lay[conical_slot(L, j)] = w;
-                    w += align16(conical_slot_words(N) + (diag ? 4 * NN1 : 0) + fft_scratch_words(all[eff[j]], M[i]));
+                    w += align16(conical_slot_words(N) + (diag ? 4 * NN1 : 0) + fft_scratch_words(all[eff[j]], M[i]) +
+                                 (fields ? 4 * NN1 : 0));      // field plans keep a Schur generator here
                 }
             const int64_t ws_lo = w;
             w += cost.bump_words + (diag ? 12 * NN1 + 64 : 0) +
@@ -1087,8 +1088,9 @@ static int plan_conical_create_impl(srcwa_context* ctx, int n_stacks, const doub
             in.ws_hi = chunk_ws + w;
             in.field_off = field_words;
             if (fields) {
-                // per layer a+ and a- (2N x 2 each), then the outer responses up and down (2N x 2 each)
-                field_words += 8 * int64_t(N) * (L + 1);
+                // per layer a+ and a- (2N x 2 each), then the outer responses up and down (2N x 2 each), then one
+                // Schur-basis mark per layer
+                field_words += 8 * int64_t(N) * (L + 1) + L;
                 plan->h_layers.push_back(all);
             }
             chunk_ws += w;
@@ -1368,6 +1370,8 @@ void sample_fields(srcwa_plan* plan, int problem, int nz, const double* zs, DevB
         const int64_t nc = int64_t(N) * g.count;
         if (conical) {
             scr += 4 * align16(2 * nc) + 2 * align16(nc) + 16;
+            // a layer group may sample a Schur-basis layer: its anchors and four (2N)^2 exponential buffers
+            if (g.region >= 0) scr += 4 * align16(4 * NN) + align16(6 * int64_t(N)) + 64;
         } else {
             scr += 5 * align16(nc);
             if (g.tm_grating)

@@ -321,18 +321,34 @@ def test_conical_fields_zero_thickness_layers_and_batches(fctx, ref_lib):
     assert np.max(np.abs(S / S[0] - 1.0)) <= 1e-10
 
 
-def test_conical_field_contract(fctx):
-    """A degenerate (Schur-basis) layer leaves this build's conical context unavailable while the solve itself
-    succeeds; Jones vectors that are zero or not finite and non-finite depths are invalid arguments; the planar
-    entry points refuse a conical plan."""
+def test_conical_fields_through_schur_layers(fctx, ref_lib):
+    """Degenerate layers propagate in the Schur basis: the field plan keeps their generator G and samples
+    exp(-k0 z G) per depth; the reference's cases with Schur layers (tests/golden/conical_ref.json) agree."""
     from conftest import load_golden
-    from stablercwa_amd import _lib
+    from oracle import ref
     from stablercwa_amd.types import Stack
-    c = [c for c in load_golden("conical_ref.json")["cases"] if c["n_schur"] > 0][0]
-    b = _conical_plan(fctx, Stack.from_flat(np.array(c["stack"])), [(c["wavelength"], c["theta"], c["phi"], c["M"])])
-    assert b.results()["status"][0] == 0 and not b.field_available(0)
-    with pytest.raises(_lib.FieldUnavailable):
-        b.field_at(0, [0.1])
+    cases = [c for c in load_golden("conical_ref.json")["cases"] if c["n_schur"] > 0][:4]
+    assert cases
+    for c in cases:
+        st = Stack.from_flat(np.array(c["stack"]))
+        total = sum(l.d for l in st.layers)
+        zs = np.concatenate([[-0.2, 0.0], np.linspace(0.0, total, 9)[1:], [total + 0.3]])
+        b = _conical_plan(fctx, st, [(c["wavelength"], c["theta"], c["phi"], c["M"])])
+        assert b.results()["status"][0] == 0 and b.field_available(0), c["name"]
+        for jones in JONES[1:]:
+            o = ref.conical_fields(ref_lib, c["stack"], c["wavelength"], c["theta"], c["phi"], c["M"], jones, zs,
+                                   nx=5, y=.2)
+            assert o["available"] and o["n_schur"] > 0, c["name"]
+            _close(b.field_at(0, zs, jones), o["harm"], f"schur {c['name']} harmonics")
+            _close(b.field_map(0, zs, 5, .2, jones), o["grid"], f"schur {c['name']} grid")
+        b.close()
+
+
+def test_conical_field_contract(fctx):
+    """Jones vectors that are zero or not finite and non-finite depths are invalid arguments; the planar entry
+    points refuse a conical plan; a failed conical solve leaves the context unavailable."""
+    from stablercwa_amd import _lib
+    from stablercwa_amd.types import Layer, Stack
     ok = _conical_plan(fctx, lossy_stack(), [(.83, 27., 31., 3)])
     for bad in [(0.0, 0.0), (np.inf, 1.0), (1.0, complex(0, np.nan))]:
         with pytest.raises(ValueError):
@@ -345,6 +361,20 @@ def test_conical_field_contract(fctx):
     z = np.array([0.1])
     with pytest.raises(ValueError):
         _lib.check(ok.lib.srcwa_plan1d_field_at(ok.handle, 0, 1, _lib.ptr(z), _lib.ptr(out)))
+    # an eigensolve stopped after one sweep: eig_info > 0, NaN results, no fields
+    st = Stack(1.0, [Layer.grating(0.38, [0.0, 0.1722, 0.574], [1.46, 1.0])], 1.46, 0.574)
+    b = rcwa_conical_batch(fctx, st, (1.0, 30.0, 20.0, 6))
+    b.set_max_iter(1)
+    b.execute()
+    b.sync()
+    assert b.results()["eig_info"][0] > 0 and not b.field_available(0)
+    with pytest.raises(_lib.FieldUnavailable):
+        b.field_at(0, [0.1])
+
+
+def rcwa_conical_batch(ctx, stack, point):
+    from stablercwa_amd import rcwa
+    return rcwa.ConicalBatch([stack], [(0,) + tuple(point)], ctx)
 
 
 def test_cpp_solve_conical_with_fields_matches_reference(ref_lib):

@@ -165,3 +165,24 @@ def test_conical_grid_carries_the_ky_phase(ref_lib):
     for c in range(6):
         want = (carrier @ o["harm"][c]).T
         assert np.max(np.abs(o["grid"][c] - want)) <= 1e-12 * max(1.0, np.max(np.abs(want)))
+
+
+def test_conical_schur_layer_fields_are_continuous_and_linear(ref_lib):
+    """The reference's degenerate (Schur-basis) conical cases: tangential Ex, Ey, Hx, Hy continuous at every
+    interface and the fields linear in the Jones vector, so the Schur-basis propagation is pinned too."""
+    from conftest import load_golden
+    from oracle import ref
+    from stablercwa_amd.types import Stack
+    for c in [c for c in load_golden("conical_ref.json")["cases"] if c["n_schur"] > 0][:4]:
+        st = Stack.from_flat(np.array(c["stack"]))
+        tops = np.cumsum([0.0] + [l.d for l in st.layers])
+        zs = [t + e for t in tops for e in (-1e-10, 1e-10)]
+        args = (ref_lib, c["stack"], c["wavelength"], c["theta"], c["phi"], c["M"])
+        hs = ref.conical_fields(*args, (1, 0), zs)
+        assert hs["available"] and hs["n_schur"] > 0
+        hp = ref.conical_fields(*args, (0, 1), zs)["harm"]
+        h = ref.conical_fields(*args, (0.6, 0.8j), zs)["harm"]
+        assert np.max(np.abs(h - (0.6 * hs["harm"] + 0.8j * hp))) <= 1e-12 * np.max(np.abs(h))
+        for comp in (0, 1, 3, 4):
+            above, below = h[comp][:, 0::2], h[comp][:, 1::2]
+            assert np.max(np.abs(above - below)) <= 1e-6 * max(1.0, np.max(np.abs(above))), (c["name"], comp)
