@@ -1464,9 +1464,8 @@ void field_map_impl(srcwa_plan* plan, int problem, int nz, const double* zs, int
     double kyy = 0.0;
     if (kind == 2) {
         const double* st = plan->h_stack_words.data() + in.stack_off;
-        const double k0 = 2.0 * 3.14159265358979323846 / in.wavelength;
-        const double ky = st[1] * std::sin(in.theta_deg * 3.14159265358979323846 / 180.0) *
-                          std::sin(in.phi_deg * 3.14159265358979323846 / 180.0);
+        const double k0 = 2.0 * kPi / in.wavelength;
+        const double ky = st[1] * std::sin(in.theta_deg * kPi / 180.0) * std::sin(in.phi_deg * kPi / 180.0);
         kyy = (k0 * ky) * y;
     }
     DevBuf<cd> G;
