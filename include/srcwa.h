@@ -216,6 +216,10 @@ int srcwa_plan1d_field_map(srcwa_plan* plan, int problem, int nz, const double* 
  * that is not finite or is zero: SRCWA_EINVAL. */
 int srcwa_plan_conical_field_at(srcwa_plan* plan, int problem, int nz, const double* zs, const double* jones,
                                 double* out);
+/* Per layer of positive thickness (in stack order) of an available conical capture: 1 when the layer
+ * propagates in the Schur basis (FieldPropagation::Schur of the field metadata, rcwa1d_conical_fields.hpp:89-96),
+ * else 0; schur holds one int per such layer. */
+int srcwa_plan_conical_field_layers(srcwa_plan* plan, int problem, int* schur);
 int srcwa_plan_conical_field_map(srcwa_plan* plan, int problem, int nz, const double* zs, int nx, double y,
                                  const double* jones, double* out);
 int64_t srcwa_plan_device_bytes(const srcwa_plan* plan);

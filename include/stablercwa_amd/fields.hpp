@@ -463,6 +463,8 @@ inline ConicalFieldResult solve_conical_with_fields(const Stack& stack, double w
     }
     ctx.status_ = ok && finite ? FieldContextStatus::Ready : FieldContextStatus::Unavailable;
     if (ctx.available()) {
+        std::vector<int> schur(static_cast<std::size_t>(effective) + 1, 0);
+        ::srcwa_amd::detail::check(srcwa_plan_conical_field_layers(plan->p, 0, schur.data()));
         ctx.plan_ = std::move(plan);
         const int modes = 2 * static_cast<int>(out.result.orders.size());
         for (FieldLayerInfo& l : meta.layers) {
@@ -471,7 +473,7 @@ inline ConicalFieldResult solve_conical_with_fields(const Stack& stack, double w
             l.propagates = true;
             l.analytic_modes = l.homogeneous;
             l.mode_count = modes;
-            l.propagation = FieldPropagation::Exponential;
+            l.propagation = schur[l.effective_index] ? FieldPropagation::Schur : FieldPropagation::Exponential;
         }
     }
     return out;

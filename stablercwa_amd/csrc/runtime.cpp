@@ -1497,6 +1497,21 @@ int srcwa_plan1d_field_map(srcwa_plan* plan, int problem, int nz, const double* 
     return guarded([&] { field_map_impl(plan, problem, nz, zs, nx, y, nullptr, out, "srcwa_plan1d_field_map", 1); });
 }
 
+int srcwa_plan_conical_field_layers(srcwa_plan* plan, int problem, int* schur) {
+    return guarded([&] {
+        field_plan_checks(plan, problem, "srcwa_plan_conical_field_layers", 2);
+        if (!schur) invalid("srcwa_plan_conical_field_layers: null output");
+        if (!field_ready(plan, problem)) throw Failure{SRCWA_EUNAVAILABLE, unavailable_msg(plan)};
+        const Inst1D& in = plan->h_insts[problem];
+        const int L = in.L, n = 2 * in.N;
+        std::vector<cd> marks(static_cast<size_t>(std::max(1, L)));
+        if (L > 0)
+            copy_out(marks.data(), plan->fields.p + in.field_off + 4 * int64_t(L + 1) * n, L * sizeof(cd),
+                     hipMemcpyDeviceToHost, "D2H", plan);
+        for (int j = 0; j < L; ++j) schur[j] = marks[j].x != 0.0 ? 1 : 0;
+    });
+}
+
 int srcwa_plan_conical_field_at(srcwa_plan* plan, int problem, int nz, const double* zs, const double* jones,
                                 double* out) {
     return guarded([&] { field_at_impl(plan, problem, nz, zs, jones, out, "srcwa_plan_conical_field_at", 2); });

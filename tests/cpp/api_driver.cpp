@@ -248,6 +248,10 @@ int main() {
                     put("harm", hv);
                     put("grid", gv);
                     std::printf("ky %.17g\n", h.metadata.ky);
+                    std::printf("schur_layers");
+                    for (const auto& l : h.metadata.layers)
+                        if (l.has_modes) std::printf(" %d", l.propagation == rcwa::FieldPropagation::Schur ? 1 : 0);
+                    std::printf("\n");
                 }
             } else {
                 std::printf("invalid_argument unknown problem kind\n");

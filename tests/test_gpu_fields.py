@@ -390,3 +390,19 @@ def test_cpp_solve_conical_with_fields_matches_reference(ref_lib):
     o = ref.conical_fields(ref_lib, st.flat(), .83, 27., 31., 3, (es, ep), ZS, nx=9, y=.37)
     _close(f["harm"].view(np.complex128).reshape(6, len(ZS), 7).transpose(0, 2, 1), o["harm"], "C++ conical harmonics")
     _close(f["grid"].view(np.complex128).reshape(6, len(ZS), 9), o["grid"], "C++ conical grid")
+
+
+def test_cpp_conical_metadata_marks_schur_layers(ref_lib):
+    """FieldLayerInfo::propagation is Schur for the layers the solve propagated in the Schur basis, as many as
+    the reference's n_schur."""
+    from conftest import load_golden
+    from oracle import ref
+    from test_gpu_cpp_api import _drive, _rec
+    c = [c for c in load_golden("conical_ref.json")["cases"] if c["n_schur"] > 0][0]
+    line = (f"conical_fields {c['M']} {c['wavelength']!r} {c['theta']!r} {c['phi']!r} 1.0 0.0 0.0 0.0 1 0.01 3 0.0 "
+            f"{_rec(c['stack'])}")
+    block, = _drive([line])
+    assert block[0].split()[3] == "1", block[0]
+    marks = [int(x) for ln in block if ln.startswith("schur_layers") for x in ln.split()[1:]]
+    o = ref.conical_fields(ref_lib, c["stack"], c["wavelength"], c["theta"], c["phi"], c["M"], (1, 0), [0.01])
+    assert sum(marks) == o["n_schur"] > 0, (marks, o["n_schur"])
