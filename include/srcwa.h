@@ -206,6 +206,9 @@ int64_t srcwa_plan_orders(const srcwa_plan* plan);
 int srcwa_plan1d_field_available(srcwa_plan* plan, int problem, int* available);
 int srcwa_plan1d_field_at(srcwa_plan* plan, int problem, int nz, const double* zs, double* out);
 int srcwa_plan1d_field_map(srcwa_plan* plan, int problem, int nz, const double* zs, int nx, double y, double* out);
+/* Per layer of an available planar capture: its modes with q exactly zero, which propagate in the linear
+ * finite-zero basis (FieldLayerInfo::finite_zero_mode_count, rcwa1d_fields.hpp:94-103); one int per layer. */
+int srcwa_plan1d_field_layers(srcwa_plan* plan, int problem, int* zero_modes);
 
 /* Near fields of problem `problem` of an executed conical plan created with SRCWA_PLAN_FIELDS, for the
  * incident Jones vector jones = [Es.re, Es.im, Ep.re, Ep.im] (normalized first, rcwa_fields.hpp:218-230): as

@@ -275,13 +275,17 @@ inline PlanarFieldResult solve_with_fields(const Stack& stack, double wavelength
     }
     ctx.status_ = ok && finite ? FieldContextStatus::Ready : FieldContextStatus::Unavailable;
     if (ctx.available()) {
+        std::vector<int> zeros(meta.layers.size() + 1, 0);
+        ::srcwa_amd::detail::check(srcwa_plan1d_field_layers(plan->p, 0, zeros.data()));
         ctx.plan_ = std::move(plan);
-        for (FieldLayerInfo& l : meta.layers) {
+        for (std::size_t j = 0; j < meta.layers.size(); ++j) {
+            FieldLayerInfo& l = meta.layers[j];
             l.has_modes = true;
             l.propagates = l.thickness > 0.0;
             l.analytic_modes = l.homogeneous;
             l.mode_count = N;
-            l.propagation = FieldPropagation::Exponential;
+            l.finite_zero_mode_count = zeros[j];
+            l.propagation = zeros[j] ? FieldPropagation::FiniteZero : FieldPropagation::Exponential;
         }
     }
     return out;

@@ -74,6 +74,7 @@ PROTOTYPES = {
                                               _vp]),
     "srcwa_plan_conical_field_at": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
     "srcwa_plan_conical_field_layers": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
+    "srcwa_plan1d_field_layers": (ctypes.c_int, [_vp, ctypes.c_int, _vp]),
     "srcwa_plan_conical_field_map": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
                                                     ctypes.c_double, _vp, _vp]),
     "srcwa_plan_operator_sets": (ctypes.c_int, [_vp]),

@@ -1497,6 +1497,26 @@ int srcwa_plan1d_field_map(srcwa_plan* plan, int problem, int nz, const double* 
     return guarded([&] { field_map_impl(plan, problem, nz, zs, nx, y, nullptr, out, "srcwa_plan1d_field_map", 1); });
 }
 
+int srcwa_plan1d_field_layers(srcwa_plan* plan, int problem, int* zero_modes) {
+    return guarded([&] {
+        field_plan_checks(plan, problem, "srcwa_plan1d_field_layers", 1);
+        if (!zero_modes) invalid("srcwa_plan1d_field_layers: null output");
+        if (!field_ready(plan, problem)) throw Failure{SRCWA_EUNAVAILABLE, unavailable_msg(plan)};
+        const Inst1D& in = plan->h_insts[problem];
+        const int L = in.L, N = in.N;
+        std::vector<int64_t> lay(static_cast<size_t>(layout_size(L)));
+        copy_out(lay.data(), plan->layouts.p + in.layout_off, lay.size() * sizeof(int64_t), hipMemcpyDeviceToHost,
+                 "D2H", plan);
+        std::vector<cd> q(static_cast<size_t>(N));
+        for (int j = 0; j < L; ++j) {
+            copy_out(q.data(), plan->ws.p + lay[layout_q(j + 1)], N * sizeof(cd), hipMemcpyDeviceToHost, "D2H", plan);
+            int zeros = 0;
+            for (const cd& v : q) zeros += v.x == 0.0 && v.y == 0.0;
+            zero_modes[j] = zeros;
+        }
+    });
+}
+
 int srcwa_plan_conical_field_layers(srcwa_plan* plan, int problem, int* schur) {
     return guarded([&] {
         field_plan_checks(plan, problem, "srcwa_plan_conical_field_layers", 2);

@@ -406,3 +406,25 @@ def test_cpp_conical_metadata_marks_schur_layers(ref_lib):
     marks = [int(x) for ln in block if ln.startswith("schur_layers") for x in ln.split()[1:]]
     o = ref.conical_fields(ref_lib, c["stack"], c["wavelength"], c["theta"], c["phi"], c["M"], (1, 0), [0.01])
     assert sum(marks) == o["n_schur"] > 0, (marks, o["n_schur"])
+
+
+@pytest.mark.parametrize("pol", [0, 1])
+def test_fields_through_a_grazing_film_order(fctx, ref_lib, pol):
+    """Normal incidence with lambda / period = 1.5 exactly and a film of index 1.5: orders +-1 have q = 0 exactly in
+    the film, which then propagates in the linear finite-zero basis (rcwa1d_fields.hpp:124-131); the metadata
+    counts those two modes."""
+    import ctypes
+    from oracle import ref
+    from stablercwa_amd import _lib
+    from stablercwa_amd.types import Layer, Stack
+    st = Stack(1.0, [Layer.grating(0.2, [0.0, 0.2, 0.5], [1.9, 1.0]), Layer.uniform(0.1, 1.5),
+                     Layer.uniform(0.05, 2.0)], 1.45, 0.5)
+    zs = np.array([-0.1, 0.1, 0.2, 0.23, 0.27, 0.3, 0.33, 0.4])
+    b = _plan(fctx, st, [(0.75, 0.0, 3, pol)])
+    o = ref.planar_fields(ref_lib, st.flat(), 0.75, 0.0, 3, pol, zs, nx=5)
+    assert b.field_available(0) and o["available"]
+    _close(b.field_at(0, zs), o["harm"], "finite-zero film harmonics")
+    _close(b.field_map(0, zs, 5), o["grid"], "finite-zero film grid")
+    zeros = np.zeros(3, dtype=np.int32)
+    _lib.check(b.lib.srcwa_plan1d_field_layers(b.handle, 0, _lib.ptr(zeros)))
+    assert zeros.tolist() == [0, 2, 0]
