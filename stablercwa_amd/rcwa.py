@@ -318,6 +318,38 @@ def solve_conical(stack: Stack, wavelength: float, theta_deg: float, phi_deg: fl
     return r
 
 
+class ConicalFieldContext:
+    """The near-field capture of one conical solve for the s and p incidence (ConicalFieldContext,
+    rcwa1d_conical_fields.hpp:16-52), kept on the device by its plan; a Jones vector is chosen per call."""
+
+    def __init__(self, batch: ConicalBatch, available: bool):
+        self._batch = batch
+        self._available = available
+
+    def available(self) -> bool:
+        return self._available
+
+    def field_at(self, zs, jones=(1.0, 0.0)) -> np.ndarray:
+        return self._batch.field_at(0, zs, jones)
+
+    def field_map(self, zs, nx: int = 256, y: float = 0.0, jones=(1.0, 0.0)) -> np.ndarray:
+        return self._batch.field_map(0, zs, nx, y, jones)
+
+
+def solve_conical_with_fields(stack: Stack, wavelength: float, theta_deg: float, phi_deg: float, M: int,
+                              ctx: _lib.Context | None = None):
+    """One conical solve and its near-field capture (rcwa::solve_conical_with_fields,
+    rcwa1d_conical_fields.hpp:56-98): returns (ConicalResult, ConicalFieldContext)."""
+    if ctx is None:
+        ctx = _field_contexts.get(0)
+        if ctx is None:
+            ctx = _field_contexts[0] = _lib.Context(0, plan_flags=_lib.PLAN_FIELDS)
+    b = ConicalBatch([stack], [(0, wavelength, theta_deg, phi_deg, M)], ctx)
+    b.execute()
+    r = b.result(0)
+    return r, ConicalFieldContext(b, b.field_available(0))
+
+
 def sweep_conical(stack: Stack, points: Sequence[ConicalPoint], M: int,
                   ctx: _lib.Context | None = None) -> List[ConicalResult]:
     """Independent conical solves over points in one batched launch (rcwa1d_conical.hpp:992-999)."""

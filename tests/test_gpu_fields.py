@@ -428,3 +428,14 @@ def test_fields_through_a_grazing_film_order(fctx, ref_lib, pol):
     zeros = np.zeros(3, dtype=np.int32)
     _lib.check(b.lib.srcwa_plan1d_field_layers(b.handle, 0, _lib.ptr(zeros)))
     assert zeros.tolist() == [0, 2, 0]
+
+
+def test_python_solve_conical_with_fields(ref_lib):
+    from oracle import ref
+    from stablercwa_amd import rcwa
+    st = lossy_stack()
+    r, f = rcwa.solve_conical_with_fields(st, .83, 27., 31., 3)
+    assert f.available() and len(r.DEr) == 7
+    o = ref.conical_fields(ref_lib, st.flat(), .83, 27., 31., 3, (0.6, 0.8j), ZS, nx=9, y=.2)
+    _close(f.field_at(ZS, (0.6, 0.8j)), o["harm"], "python conical harmonics")
+    _close(f.field_map(ZS, 9, .2, (0.6, 0.8j)), o["grid"], "python conical grid")
